@@ -1,0 +1,105 @@
+/*
+ * nemaktis_bpm.h - C ABI of the MI355X-native bpm-solver propagation step.
+ *
+ * Drop-in for the Python backend module `bpm_backend` of Nemaktis
+ * (reference: BeamPropagationSolver/src/PythonBackend/bpm_backend.h:4-13,
+ * bound by SWIG in bpm_backend.i:16-18 and called from
+ * HighLevelPythonInterface/nemaktis/light_propagator.py:225-228).
+ *
+ * Array conventions (identical to the reference):
+ *   lc_field_vals  (Nz, Ny, Nx, Nv) float64, C order, Nv = 3 (director) or
+ *                  6 (Q-tensor xx, yy, zz, xy, xz, yz)
+ *   mask_vals      (Nz, Ny, Nx) float64, >= 0 marks the LC domain
+ *   E_field_vals   (N_wavelengths, N_wavevectors, 4, Ny, Nx) complex128 as
+ *                  interleaved (re, im) float64 pairs; the 4 planes are
+ *                  (Ex, Ey) for x-polarised then for y-polarised input.  It is
+ *                  read on entry only with "Beam profile": "None", and
+ *                  overwritten in place with the micrograph-plane fields.
+ * Sizes are element counts (n_E_vals counts complex values).  Every entry
+ * point returns 0 on success and a non-zero code on failure, with the
+ * message copied into err_msg (if non-NULL); nothing is printed.
+ */
+#ifndef NEMAKTIS_BPM_H
+#define NEMAKTIS_BPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum nmk_bpm_status {
+    NMK_BPM_OK = 0,
+    NMK_BPM_EINPUT = 1,   /* invalid settings or array sizes (the reference's thrown strings) */
+    NMK_BPM_EDEVICE = 2,  /* HIP runtime failure */
+    NMK_BPM_ENOMEM = 3,   /* the run does not fit the device memory budget */
+    NMK_BPM_EINTERNAL = 4
+};
+
+/* Replaces run_backend_without_mask (PythonBackend/bpm_backend.h:4-7,
+ * bpm_backend.cpp:7-83): FP64 march, one wavelength at a time. */
+int nmk_bpm_run_backend_without_mask(const char *json_str, const double *lc_field_vals, int64_t n_lc_vals,
+                                     double *E_field_vals, int64_t n_E_vals, char *err_msg, size_t err_len);
+
+/* Replaces run_backend_with_mask (PythonBackend/bpm_backend.h:9-13,
+ * bpm_backend.cpp:85-164); mask_formula may be "" (no interface weights). */
+int nmk_bpm_run_backend_with_mask(const char *json_str, const char *mask_formula, const double *lc_field_vals,
+                                  int64_t n_lc_vals, const double *mask_vals, int64_t n_mask_vals,
+                                  double *E_field_vals, int64_t n_E_vals, char *err_msg, size_t err_len);
+
+/* ---- extensions beyond the reference interface ---------------------------- */
+
+typedef struct {
+    int32_t precision;   /* 64: complex128 march (default); 32: complex64 march
+                            with FP64 permittivity / coefficient precompute */
+    int32_t wl_batch;    /* wavelengths marched together; <= 0: all at once */
+    int32_t use_graph;   /* 1: capture the z-march as a hipGraph and replay it */
+    int32_t device;      /* HIP device ordinal, -1: current device */
+    int32_t profile;     /* 1: time every kernel family (disables the graph) */
+    int32_t reserved[3];
+} nmk_bpm_options;
+
+typedef struct {
+    double setup_ms, march_ms, output_ms, total_ms; /* device time of the last propagate */
+    double updates;                                  /* voxel-slice updates of one run */
+    double updates_per_s;
+    int64_t footprint_bytes;                         /* HBM held by the run */
+    double family_ms[10];                            /* profile mode: per kernel family */
+    int64_t family_launches[10];
+} nmk_bpm_stats;
+
+void nmk_bpm_default_options(nmk_bpm_options *opts);
+
+/* One-shot run with options (mask_vals == NULL selects the unmasked form). */
+int nmk_bpm_run(const char *json_str, const char *mask_formula, const double *lc_field_vals, int64_t n_lc_vals,
+                const double *mask_vals, int64_t n_mask_vals, double *E_field_vals, int64_t n_E_vals,
+                const nmk_bpm_options *opts, nmk_bpm_stats *stats, char *err_msg, size_t err_len);
+
+/* Context API: inputs uploaded once and kept resident in HBM, so that repeated
+ * propagations (benchmarks, parameter sweeps) time the device work alone. */
+typedef struct nmk_bpm_ctx nmk_bpm_ctx;
+nmk_bpm_ctx *nmk_bpm_create(const char *json_str, const char *mask_formula, const double *lc_field_vals,
+                            int64_t n_lc_vals, const double *mask_vals, int64_t n_mask_vals, int64_t n_E_vals,
+                            const nmk_bpm_options *opts, char *err_msg, size_t err_len);
+int nmk_bpm_upload_fields(nmk_bpm_ctx *ctx, const double *E_field_vals, int64_t n_E_vals, char *err_msg,
+                          size_t err_len);
+int nmk_bpm_propagate(nmk_bpm_ctx *ctx, nmk_bpm_stats *stats, char *err_msg, size_t err_len);
+int nmk_bpm_download(nmk_bpm_ctx *ctx, double *E_field_vals, int64_t n_E_vals, char *err_msg, size_t err_len);
+void nmk_bpm_destroy(nmk_bpm_ctx *ctx);
+
+/* HBM bytes a run of these settings would hold (serving admission). */
+int nmk_bpm_estimate_footprint(const char *json_str, int64_t n_lc_vals, int has_mask, const nmk_bpm_options *opts,
+                               int64_t *bytes, char *err_msg, size_t err_len);
+
+/* Evaluates an index expression (muParser syntax, variable `lambda` in um) as
+ * the backend does for "no", "ne", ... (PhysicsLib/PhysicsCoefficients.cpp:27-103). */
+int nmk_bpm_eval_expression(const char *expr, double lambda, double *value, char *err_msg, size_t err_len);
+
+const char *nmk_bpm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,96 @@
+"""ctypes binding of libnemaktis_bpm.so (the C ABI in include/nemaktis_bpm.h).
+
+The library is built in-tree (``python -m nemaktis_amd.build``); loading
+fails loudly when it is missing - there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnemaktis_bpm.so")
+
+_lib = None
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("wl_batch", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("profile", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("setup_ms", ctypes.c_double), ("march_ms", ctypes.c_double), ("output_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("updates", ctypes.c_double), ("updates_per_s", ctypes.c_double),
+                ("footprint_bytes", ctypes.c_int64), ("family_ms", ctypes.c_double * 10),
+                ("family_launches", ctypes.c_int64 * 10)]
+
+    def as_dict(self):
+        names = ["material", "coefs", "shift", "phase", "explicit", "solve_x", "solve_y", "output", "filter", "setup"]
+        return {
+            "setup_ms": self.setup_ms, "march_ms": self.march_ms, "output_ms": self.output_ms,
+            "total_ms": self.total_ms, "updates": self.updates, "updates_per_s": self.updates_per_s,
+            "footprint_bytes": self.footprint_bytes,
+            "family_ms": {n: self.family_ms[i] for i, n in enumerate(names)},
+            "family_launches": {n: self.family_launches[i] for i, n in enumerate(names)},
+        }
+
+
+EXPORTED = [
+    "nmk_bpm_run_backend_without_mask", "nmk_bpm_run_backend_with_mask", "nmk_bpm_default_options",
+    "nmk_bpm_run", "nmk_bpm_create", "nmk_bpm_upload_fields", "nmk_bpm_propagate", "nmk_bpm_download",
+    "nmk_bpm_destroy", "nmk_bpm_estimate_footprint", "nmk_bpm_eval_expression", "nmk_bpm_version",
+]
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libnemaktis_bpm.so is not built (run `python -m nemaktis_amd.build`): "
+                           "the bpm backend has no CPU fallback")
+    lib = ctypes.CDLL(LIB_PATH)
+    i64, cp, sz = ctypes.c_int64, ctypes.c_char_p, ctypes.c_size_t
+    lib.nmk_bpm_run_backend_without_mask.argtypes = [cp, c_double_p, i64, c_double_p, i64, cp, sz]
+    lib.nmk_bpm_run_backend_with_mask.argtypes = [cp, cp, c_double_p, i64, c_double_p, i64, c_double_p, i64, cp, sz]
+    lib.nmk_bpm_default_options.argtypes = [ctypes.POINTER(Options)]
+    lib.nmk_bpm_run.argtypes = [cp, cp, c_double_p, i64, c_double_p, i64, c_double_p, i64,
+                                ctypes.POINTER(Options), ctypes.POINTER(Stats), cp, sz]
+    lib.nmk_bpm_create.argtypes = [cp, cp, c_double_p, i64, c_double_p, i64, i64, ctypes.POINTER(Options), cp, sz]
+    lib.nmk_bpm_create.restype = ctypes.c_void_p
+    lib.nmk_bpm_upload_fields.argtypes = [ctypes.c_void_p, c_double_p, i64, cp, sz]
+    lib.nmk_bpm_propagate.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats), cp, sz]
+    lib.nmk_bpm_download.argtypes = [ctypes.c_void_p, c_double_p, i64, cp, sz]
+    lib.nmk_bpm_destroy.argtypes = [ctypes.c_void_p]
+    lib.nmk_bpm_estimate_footprint.argtypes = [cp, i64, ctypes.c_int, ctypes.POINTER(Options),
+                                               ctypes.POINTER(i64), cp, sz]
+    lib.nmk_bpm_eval_expression.argtypes = [cp, ctypes.c_double, c_double_p, cp, sz]
+    lib.nmk_bpm_version.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def dptr(a):
+    return None if a is None else a.ctypes.data_as(c_double_p)
+
+
+def options(precision=64, wl_batch=1, use_graph=False, device=-1, profile=False):
+    o = Options()
+    load().nmk_bpm_default_options(ctypes.byref(o))
+    o.precision = int(precision)
+    o.wl_batch = int(wl_batch)
+    o.use_graph = 1 if use_graph else 0
+    o.device = int(device)
+    o.profile = 1 if profile else 0
+    return o
+
+
+def err_buf():
+    return ctypes.create_string_buffer(4096)
+
+
+def as_lc(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))

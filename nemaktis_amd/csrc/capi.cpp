@@ -1,0 +1,219 @@
+// extern "C" entry points of libnemaktis_bpm.so (declared in include/nemaktis_bpm.h).
+#include "../../include/nemaktis_bpm.h"
+
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "engine.h"
+#include "expr.h"
+#include "settings.h"
+
+using namespace nmk;
+
+namespace {
+
+void set_err(char *buf, size_t len, const std::string &msg) {
+    if (!buf || len == 0) return;
+    size_t n = std::min(len - 1, msg.size());
+    std::memcpy(buf, msg.data(), n);
+    buf[n] = '\0';
+}
+
+Options to_options(const nmk_bpm_options *o) {
+    Options r;
+    if (o) {
+        r.precision = o->precision;
+        r.wl_batch = o->wl_batch;
+        r.use_graph = o->use_graph;
+        r.device = o->device;
+        r.profile = o->profile;
+    }
+    return r;
+}
+
+void fill_stats(nmk_bpm_stats *out, const Stats &s) {
+    if (!out) return;
+    std::memset(out, 0, sizeof(*out));
+    out->setup_ms = s.setup_ms;
+    out->march_ms = s.march_ms;
+    out->output_ms = s.output_ms;
+    out->total_ms = s.total_ms;
+    out->updates = s.updates;
+    out->updates_per_s = s.total_ms > 0 ? s.updates / (s.total_ms * 1e-3) : 0;
+    out->footprint_bytes = s.footprint_bytes;
+    for (int i = 0; i < KF_COUNT && i < 10; i++) {
+        out->family_ms[i] = s.family_ms[i];
+        out->family_launches[i] = s.family_launches[i];
+    }
+}
+
+// maps exceptions to status codes and messages
+template <class F>
+int guarded(char *err, size_t len, F &&f) {
+    try {
+        f();
+        set_err(err, len, "");
+        return NMK_BPM_OK;
+    } catch (const BpmError &e) {
+        set_err(err, len, e.what());
+        return NMK_BPM_EINPUT;
+    } catch (const std::bad_alloc &) {
+        set_err(err, len, "host out of memory");
+        return NMK_BPM_ENOMEM;
+    } catch (const std::runtime_error &e) {
+        std::string m = e.what();
+        set_err(err, len, m);
+        if (m.find("out of memory") != std::string::npos || m.find("hipErrorOutOfMemory") != std::string::npos)
+            return NMK_BPM_ENOMEM;
+        return m.rfind("HIP", 0) == 0 ? NMK_BPM_EDEVICE : NMK_BPM_EINTERNAL;
+    } catch (const std::exception &e) {
+        set_err(err, len, e.what());
+        return NMK_BPM_EINTERNAL;
+    } catch (...) {
+        set_err(err, len, "Unknown exception!");
+        return NMK_BPM_EINTERNAL;
+    }
+}
+
+Problem problem_from(const char *json, const char *formula, const double *lc, int64_t n_lc, const double *mask,
+                     int64_t n_mask, int64_t n_E, std::string *warning) {
+    if (!json) throw BpmError("null settings string");
+    Settings s = parse_settings(json);
+    return build_problem(s, lc, n_lc, mask, n_mask, mask ? (formula ? formula : "") : nullptr, n_E, warning);
+}
+
+}  // namespace
+
+struct nmk_bpm_ctx {
+    Problem problem;
+    std::vector<double> lc_copy;
+    std::unique_ptr<Runner> runner;
+};
+
+extern "C" {
+
+void nmk_bpm_default_options(nmk_bpm_options *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->precision = 64;
+    o->wl_batch = 1;
+    o->use_graph = 0;
+    o->device = -1;
+}
+
+int nmk_bpm_run(const char *json, const char *formula, const double *lc, int64_t n_lc, const double *mask,
+                int64_t n_mask, double *E, int64_t n_E, const nmk_bpm_options *opts, nmk_bpm_stats *stats, char *err,
+                size_t errlen) {
+    return guarded(err, errlen, [&] {
+        std::string warning;
+        Problem p = problem_from(json, formula, lc, n_lc, mask, n_mask, n_E, &warning);
+        Options o = to_options(opts);
+        auto r = make_runner(p, o);
+        if (p.beam == BeamKind::None) r->upload_fields(reinterpret_cast<const c64 *>(E));
+        Stats st;
+        r->propagate(&st);
+        r->download(reinterpret_cast<c64 *>(E));
+        fill_stats(stats, st);
+    });
+}
+
+int nmk_bpm_run_backend_without_mask(const char *json, const double *lc, int64_t n_lc, double *E, int64_t n_E,
+                                     char *err, size_t errlen) {
+    nmk_bpm_options o;
+    nmk_bpm_default_options(&o);
+    return nmk_bpm_run(json, nullptr, lc, n_lc, nullptr, 0, E, n_E, &o, nullptr, err, errlen);
+}
+
+int nmk_bpm_run_backend_with_mask(const char *json, const char *formula, const double *lc, int64_t n_lc,
+                                  const double *mask, int64_t n_mask, double *E, int64_t n_E, char *err,
+                                  size_t errlen) {
+    if (!mask) {
+        set_err(err, errlen, "Wrong dimension for the mask pointer array");
+        return NMK_BPM_EINPUT;
+    }
+    nmk_bpm_options o;
+    nmk_bpm_default_options(&o);
+    return nmk_bpm_run(json, formula ? formula : "", lc, n_lc, mask, n_mask, E, n_E, &o, nullptr, err, errlen);
+}
+
+nmk_bpm_ctx *nmk_bpm_create(const char *json, const char *formula, const double *lc, int64_t n_lc,
+                            const double *mask, int64_t n_mask, int64_t n_E, const nmk_bpm_options *opts, char *err,
+                            size_t errlen) {
+    nmk_bpm_ctx *ctx = nullptr;
+    int rc = guarded(err, errlen, [&] {
+        std::unique_ptr<nmk_bpm_ctx> c(new nmk_bpm_ctx);
+        c->problem = problem_from(json, formula, lc, n_lc, mask, n_mask, n_E, nullptr);
+        c->runner = make_runner(c->problem, to_options(opts));
+        c->problem.lc = nullptr;  // the director now lives in HBM
+        ctx = c.release();
+    });
+    return rc == NMK_BPM_OK ? ctx : nullptr;
+}
+
+int nmk_bpm_upload_fields(nmk_bpm_ctx *ctx, const double *E, int64_t n_E, char *err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!ctx) throw BpmError("null context");
+        if (n_E != ctx->problem.n_out_values()) throw BpmError("Wrong dimension for the raw fields array");
+        ctx->runner->upload_fields(reinterpret_cast<const c64 *>(E));
+    });
+}
+
+int nmk_bpm_propagate(nmk_bpm_ctx *ctx, nmk_bpm_stats *stats, char *err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!ctx) throw BpmError("null context");
+        Stats st;
+        ctx->runner->propagate(&st);
+        fill_stats(stats, st);
+    });
+}
+
+int nmk_bpm_download(nmk_bpm_ctx *ctx, double *E, int64_t n_E, char *err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!ctx) throw BpmError("null context");
+        if (n_E != ctx->problem.n_out_values()) throw BpmError("Wrong dimension for the raw fields array");
+        ctx->runner->download(reinterpret_cast<c64 *>(E));
+    });
+}
+
+void nmk_bpm_destroy(nmk_bpm_ctx *ctx) { delete ctx; }
+
+int nmk_bpm_estimate_footprint(const char *json, int64_t n_lc, int has_mask, const nmk_bpm_options *opts,
+                               int64_t *bytes, char *err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        Settings s = parse_settings(json ? json : "");
+        Problem p;
+        p.Nx = s.Nx;
+        p.Ny = s.Ny;
+        p.Nz = s.Nz;
+        int64_t nv = p.voxels();
+        p.field_dim = nv > 0 ? (int)(n_lc / nv) : 3;
+        p.wl = s.wavelengths;
+        for (auto &q : s.q) {
+            p.qx.push_back(q.first);
+            p.qy.push_back(q.second);
+        }
+        if (has_mask) p.mask.assign(1, 1);  // marks presence; size added below
+        int64_t b = estimate_footprint(p, to_options(opts));
+        if (has_mask) b += nv - 1;
+        if (bytes) *bytes = b;
+    });
+}
+
+int nmk_bpm_eval_expression(const char *expr, double lambda, double *value, char *err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!expr) throw BpmError("null expression");
+        try {
+            double v = eval_index_expression(expr, lambda);
+            if (value) *value = v;
+        } catch (const ExprError &e) {
+            throw BpmError(e.what());
+        }
+    });
+}
+
+const char *nmk_bpm_version(void) { return "nemaktis-amd bpm 0.1 (gfx950)"; }
+
+}  // extern "C"

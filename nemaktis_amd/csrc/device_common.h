@@ -1,0 +1,118 @@
+// Complex arithmetic and the parameter blocks shared by the HIP kernels of
+// the bpm-solver march.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nmk {
+
+template <class T>
+struct alignas(2 * sizeof(T)) C2 {
+    T x, y;
+};
+using c32 = C2<float>;
+using c64 = C2<double>;
+
+template <class T>
+__host__ __device__ __forceinline__ C2<T> mk(T a, T b) {
+    return C2<T>{a, b};
+}
+template <class T>
+__host__ __device__ __forceinline__ C2<T> operator+(C2<T> a, C2<T> b) {
+    return {a.x + b.x, a.y + b.y};
+}
+template <class T>
+__host__ __device__ __forceinline__ C2<T> operator-(C2<T> a, C2<T> b) {
+    return {a.x - b.x, a.y - b.y};
+}
+template <class T>
+__host__ __device__ __forceinline__ C2<T> operator-(C2<T> a) {
+    return {-a.x, -a.y};
+}
+template <class T>
+__host__ __device__ __forceinline__ C2<T> operator*(C2<T> a, C2<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <class T>
+__host__ __device__ __forceinline__ C2<T> operator*(T s, C2<T> a) {
+    return {s * a.x, s * a.y};
+}
+// acc + a*b
+template <class T>
+__host__ __device__ __forceinline__ C2<T> cmad(C2<T> acc, C2<T> a, C2<T> b) {
+    return {acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x};
+}
+template <class T>
+__host__ __device__ __forceinline__ C2<T> conj(C2<T> a) {
+    return {a.x, -a.y};
+}
+// 1/a with scaling against overflow
+template <class T>
+__host__ __device__ __forceinline__ C2<T> crcp(C2<T> a) {
+    T s = (a.x < 0 ? -a.x : a.x) + (a.y < 0 ? -a.y : a.y);
+    T ar = a.x / s, ai = a.y / s;
+    T d = s * (ar * ar + ai * ai);
+    return {ar / d, -ai / d};
+}
+template <class T>
+__host__ __device__ __forceinline__ C2<T> cdiv(C2<T> a, C2<T> b) {
+    return a * crcp(b);
+}
+template <class T, class U>
+__host__ __device__ __forceinline__ C2<T> cast2(C2<U> a) {
+    return {(T)a.x, (T)a.y};
+}
+
+// ---- launch-time parameter blocks ----------------------------------------
+
+// geometry of the run: full grid (with the duplicate column/row) and the
+// march grid (periodic interior along axes with more than 3 points)
+struct Geo {
+    int Nx, Ny, Nz;
+    int Mx, My;       // march grid
+    int px, py;       // 1 if the axis is periodic (N > 3)
+    int field_dim;    // 3 director, 6 Q-tensor
+    int64_t Nfull;    // Nx*Ny
+    int64_t NP;       // Mx*My
+};
+
+// per-wavelength scalars of one batch slot
+struct SlotParams {
+    double k0, dX, dY, dZ;
+    double no, ne, ne_imag, nhost;
+    double eperp, ea, ehost;
+    double alpha_im;      // absorption factor alpha = 1 + i*alpha_im
+    double nin, nout_fresnel, nout_filter;
+    double nref;          // filled on the device by the reduction
+    int iwl;              // wavelength index in the caller's array
+    int pad;
+};
+
+// per-propagation (wavelength slot, wavevector, polarisation) scalars
+struct PropParams {
+    int slot, iq, pol, pad;
+    int ox, oy;           // first tap offset of the Catmull-Rom shift (floor(s)-1)
+    double wx[4], wy[4];  // Catmull-Rom weights along x and y
+    double qx, qy;
+    double ex, ey;        // input polarisation (cos, sin of pol*pi/2)
+    double ph_re, ph_im;  // accumulated global phase of all shift half-steps
+};
+
+// coefficient plane slots (per interior point, stored as mu*R and P)
+enum : int {
+    CF_DX = 0,     // 4 blocks x {minus, centre, plus}
+    CF_DY = 12,    // 4 blocks x {minus, centre, plus}
+    CF_XY = 24,    // 4 blocks x {mxmy, pxmy, mxpy, pxpy}
+    CF_P = 40,     // op_xx, op_yy, op_xy
+    CF_COUNT = 43
+};
+// slab-material plane slots (per full-grid point, FP64)
+enum : int {
+    SM_EINV = 0, SM_KXX, SM_KYY, SM_KXY, SM_IXX, SM_IYY, SM_IXY, SM_VX, SM_VY,
+    SM_PXX_RE, SM_PXX_IM, SM_PYY_RE, SM_PYY_IM, SM_PXY_RE, SM_PXY_IM,
+    SM_COUNT
+};
+
+}  // namespace nmk

@@ -1,0 +1,460 @@
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <utility>
+#include <vector>
+
+#include "kernels.h"
+
+namespace nmk {
+
+#define HIPCHK(x)                                                                                      \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess)                                                                          \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+    } while (0)
+
+namespace {
+
+const double kPi = 3.1415926535897932;
+
+template <class P>
+struct DevBuf {
+    P *ptr = nullptr;
+    size_t count = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() {
+        if (ptr) (void)hipFree(ptr);
+    }
+    void alloc(size_t n) {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = n;
+        if (n) HIPCHK(hipMalloc((void **)&ptr, n * sizeof(P)));
+    }
+    size_t bytes() const { return count * sizeof(P); }
+};
+
+// Catmull-Rom weights of a fractional shift nu (SimpleShiftOperator.cpp:27-30)
+void cr_weights(double nu, double *w) {
+    w[0] = -nu * ((1 - nu) * (1 - nu)) / 2;
+    w[1] = (1 - nu) * (1 + nu - 3 * nu * nu / 2);
+    w[2] = nu * (1 + 4 * nu - 3 * nu * nu) / 2;
+    w[3] = -nu * nu * (1 - nu) / 2;
+}
+
+struct Sizes {
+    int nwl, nq, batch, nslots_max, nprops_max;
+    int64_t Nfull, NP;
+};
+
+Sizes sizes_of(const Problem &p, const Options &o) {
+    Sizes s;
+    s.nwl = (int)p.wl.size();
+    s.nq = (int)p.qx.size();
+    s.batch = o.wl_batch <= 0 ? s.nwl : std::min(o.wl_batch, s.nwl);
+    s.nslots_max = s.batch;
+    s.nprops_max = s.batch * s.nq * 2;
+    s.Nfull = (int64_t)p.Nx * p.Ny;
+    int Mx = p.Nx > 3 ? p.Nx - 1 : p.Nx, My = p.Ny > 3 ? p.Ny - 1 : p.Ny;
+    s.NP = (int64_t)Mx * My;
+    return s;
+}
+
+constexpr int kNrefBlocks = 1024;
+
+template <class T>
+class RunnerT final : public Runner {
+public:
+    RunnerT(const Problem &p, const Options &o, hipStream_t external) : p_(p), o_(o) {
+        if (o.device >= 0) HIPCHK(hipSetDevice(o.device));
+        if (external) {
+            st_ = external;
+        } else {
+            HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+            own_stream_ = true;
+        }
+        sz_ = sizes_of(p, o);
+        g_.Nx = p.Nx;
+        g_.Ny = p.Ny;
+        g_.Nz = p.Nz;
+        g_.px = p.Nx > 3;
+        g_.py = p.Ny > 3;
+        g_.Mx = g_.px ? p.Nx - 1 : p.Nx;
+        g_.My = g_.py ? p.Ny - 1 : p.Ny;
+        g_.field_dim = p.field_dim;
+        g_.Nfull = sz_.Nfull;
+        g_.NP = sz_.NP;
+        if (!fft_supported(p.Nx) || !fft_supported(p.Ny))
+            throw BpmError("Mesh dimensions must factor into primes <= 13 for the screen FFT");
+        if (p.Nx > 2048 || p.Ny > 2048) throw BpmError("Mesh dimensions above 2048 points are not supported");
+        plan_x_ = make_fft_plan(p.Nx);
+        plan_y_ = make_fft_plan(p.Ny);
+
+        // inputs resident in HBM
+        lc_.alloc((size_t)p.voxels() * p.field_dim);
+        HIPCHK(hipMemcpyAsync(lc_.ptr, p.lc, lc_.bytes(), hipMemcpyHostToDevice, st_));
+        if (!p.mask.empty()) {
+            mask_.alloc(p.mask.size());
+            HIPCHK(hipMemcpyAsync(mask_.ptr, p.mask.data(), mask_.bytes(), hipMemcpyHostToDevice, st_));
+        }
+        if (!p.weights.empty()) {
+            weights_.alloc(p.weights.size());
+            HIPCHK(hipMemcpyAsync(weights_.ptr, p.weights.data(), weights_.bytes(), hipMemcpyHostToDevice, st_));
+        }
+        std::vector<double> qxy;
+        for (int i = 0; i < sz_.nq; i++) {
+            qxy.push_back(p.qx[i]);
+            qxy.push_back(p.qy[i]);
+        }
+        q_.alloc(qxy.size());
+        HIPCHK(hipMemcpyAsync(q_.ptr, qxy.data(), q_.bytes(), hipMemcpyHostToDevice, st_));
+        iso_t_.alloc(std::max<size_t>(1, p.iso_thick.size()));
+        iso_n_.alloc(std::max<size_t>(1, p.iso_index.size()));
+        if (!p.iso_thick.empty()) {
+            HIPCHK(hipMemcpyAsync(iso_t_.ptr, p.iso_thick.data(), p.iso_thick.size() * 8, hipMemcpyHostToDevice, st_));
+            HIPCHK(hipMemcpyAsync(iso_n_.ptr, p.iso_index.data(), p.iso_index.size() * 8, hipMemcpyHostToDevice, st_));
+        }
+        auto twiddles = [&](int n, DevBuf<c64> &d, std::vector<c64> &h) {
+            h.resize(n);
+            for (int k = 0; k < n; k++) {
+                // exp(-2 pi i k / n), argument reduced to [0, pi/4] octants for accuracy
+                long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+                h[k] = mk((double)std::cos(a), (double)std::sin(a));
+            }
+            d.alloc(n);
+            HIPCHK(hipMemcpyAsync(d.ptr, h.data(), d.bytes(), hipMemcpyHostToDevice, st_));
+        };
+        twiddles(p.Nx, twx_, htwx_);
+        twiddles(p.Ny, twy_, htwy_);
+
+        sp_.alloc(sz_.nslots_max);
+        pp_.alloc(sz_.nprops_max);
+        partials_.alloc((size_t)kNrefBlocks * sz_.nslots_max);
+        sm_.alloc((size_t)sz_.nslots_max * SM_COUNT * sz_.Nfull);
+        cf_.alloc((size_t)sz_.nslots_max * CF_COUNT * sz_.NP);
+        for (auto &b : fb_) b.alloc((size_t)sz_.nprops_max * 2 * sz_.NP);
+        E_.alloc((size_t)sz_.nwl * sz_.nq * 4 * sz_.Nfull);
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    ~RunnerT() override {
+        for (auto &kv : graphs_) {
+            (void)hipGraphExecDestroy(kv.second.exec);
+            (void)hipGraphDestroy(kv.second.graph);
+        }
+        for (auto &e : evs_) (void)hipEventDestroy(e);
+        if (own_stream_) (void)hipStreamDestroy(st_);
+    }
+
+    hipStream_t stream() const override { return st_; }
+    const c64 *device_output() const override { return E_.ptr; }
+
+    int64_t footprint() const override {
+        return (int64_t)(lc_.bytes() + mask_.bytes() + weights_.bytes() + sm_.bytes() + cf_.bytes() +
+                         4 * fb_[0].bytes() + E_.bytes() + partials_.bytes());
+    }
+
+    void upload_fields(const c64 *E_host) override {
+        HIPCHK(hipMemcpyAsync(E_.ptr, E_host, E_.bytes(), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    void download(c64 *E_host) override {
+        HIPCHK(hipMemcpyAsync(E_host, E_.ptr, E_.bytes(), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    void propagate(Stats *stats) override {
+        hipEvent_t e0 = ev(0), e1 = ev(1);
+        HIPCHK(hipEventRecord(e0, st_));
+        double march_ms = 0, setup_ms = 0, out_ms = 0;
+        if (stats) *stats = Stats();
+        stats_ = stats;
+        for (int w0 = 0; w0 < sz_.nwl; w0 += sz_.batch) {
+            const int nslots = std::min(sz_.batch, sz_.nwl - w0);
+            const int nprops = nslots * sz_.nq * 2;
+            hipEvent_t a = ev(2), b = ev(3), c = ev(4), d = ev(5);
+            HIPCHK(hipEventRecord(a, st_));
+            setup_batch(w0, nslots, nprops);
+            HIPCHK(hipEventRecord(b, st_));
+            C2<T> *state = march(nslots, nprops);
+            HIPCHK(hipEventRecord(c, st_));
+            output_batch(w0, nslots, nprops, state);
+            HIPCHK(hipEventRecord(d, st_));
+            HIPCHK(hipEventSynchronize(d));
+            float t;
+            HIPCHK(hipEventElapsedTime(&t, a, b));
+            setup_ms += t;
+            HIPCHK(hipEventElapsedTime(&t, b, c));
+            march_ms += t;
+            HIPCHK(hipEventElapsedTime(&t, c, d));
+            out_ms += t;
+        }
+        HIPCHK(hipEventRecord(e1, st_));
+        HIPCHK(hipEventSynchronize(e1));
+        if (stats) {
+            float t;
+            HIPCHK(hipEventElapsedTime(&t, e0, e1));
+            stats->total_ms = t;
+            stats->setup_ms = setup_ms;
+            stats->march_ms = march_ms;
+            stats->output_ms = out_ms;
+            stats->updates = (double)p_.Nx * p_.Ny * (p_.Nz - 1) * (double)sz_.nwl * sz_.nq * 2;
+            stats->footprint_bytes = footprint();
+        }
+        stats_ = nullptr;
+    }
+
+private:
+    Problem p_;
+    Options o_;
+    Sizes sz_;
+    Geo g_;
+    hipStream_t st_ = nullptr;
+    bool own_stream_ = false;
+    FftPlan plan_x_, plan_y_;
+    DevBuf<double> lc_, weights_, q_, iso_t_, iso_n_, partials_, sm_;
+    DevBuf<uint8_t> mask_;
+    DevBuf<c64> twx_, twy_, E_;
+    std::vector<c64> htwx_, htwy_;
+    DevBuf<SlotParams> sp_;
+    DevBuf<PropParams> pp_;
+    DevBuf<C2<T>> cf_;
+    DevBuf<C2<T>> fb_[4];
+    std::vector<SlotParams> hsp_;
+    std::vector<PropParams> hpp_;
+    std::vector<hipEvent_t> evs_;
+    struct Captured {
+        hipGraphExec_t exec;
+        hipGraph_t graph;
+        C2<T> *state;  // buffer holding the final state after a replay
+    };
+    std::map<std::pair<int, int>, Captured> graphs_;
+    Stats *stats_ = nullptr;
+
+    hipEvent_t ev(size_t i) {
+        while (evs_.size() <= i) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            evs_.push_back(e);
+        }
+        return evs_[i];
+    }
+
+    // kernel-family timing in profile mode
+    template <class F>
+    void timed(int fam, F &&launch) {
+        if (!o_.profile || !stats_) {
+            launch();
+            return;
+        }
+        hipEvent_t a = ev(6), b = ev(7);
+        HIPCHK(hipEventRecord(a, st_));
+        launch();
+        HIPCHK(hipEventRecord(b, st_));
+        HIPCHK(hipEventSynchronize(b));
+        float t;
+        HIPCHK(hipEventElapsedTime(&t, a, b));
+        stats_->family_ms[fam] += t;
+        stats_->family_launches[fam] += 1;
+    }
+
+    void setup_batch(int w0, int nslots, int nprops) {
+        const Problem &p = p_;
+        hsp_.assign(nslots, SlotParams());
+        for (int s = 0; s < nslots; s++) {
+            const int w = w0 + s;
+            SlotParams &sp = hsp_[s];
+            std::memset(&sp, 0, sizeof(sp));
+            sp.k0 = 2 * kPi / p.wl[w];
+            sp.dX = sp.k0 * p.dx;
+            sp.dY = sp.k0 * p.dy;
+            sp.dZ = sp.k0 * p.dz;
+            sp.no = p.no[w];
+            sp.ne = p.ne[w];
+            sp.ne_imag = p.ne_imag[w];
+            sp.nhost = p.nhost[w];
+            sp.eperp = sp.no * sp.no;
+            sp.ea = sp.ne * sp.ne - sp.eperp;
+            sp.ehost = sp.nhost * sp.nhost;
+            sp.alpha_im = p.ne_imag[w] / (p.ne[w] - p.no[w]);  // NaN/inf when ne == no, as the reference
+            sp.nin = p.nin[w];
+            sp.nout_fresnel = p.nout_fresnel[w];
+            sp.nout_filter = p.nout[w];
+            sp.iwl = w;
+        }
+        timed(KF_SETUP, [&] {
+            HIPCHK(hipMemcpyAsync(sp_.ptr, hsp_.data(), nslots * sizeof(SlotParams), hipMemcpyHostToDevice, st_));
+            launch_nref(lc_.ptr, mask_.ptr, g_, sp_.ptr, nslots, partials_.ptr, kNrefBlocks, st_);
+            HIPCHK(hipMemcpyAsync(hsp_.data(), sp_.ptr, nslots * sizeof(SlotParams), hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipStreamSynchronize(st_));
+        });
+        // per-propagation shift parameters (SimpleShiftOperator.cpp:3-50) and
+        // the product of all shift phase factors, in FP64
+        hpp_.assign(nprops, PropParams());
+        for (int s = 0; s < nslots; s++)
+            for (int iq = 0; iq < sz_.nq; iq++)
+                for (int pol = 0; pol < 2; pol++) {
+                    PropParams &pp = hpp_[pol + 2 * (iq + sz_.nq * s)];
+                    std::memset(&pp, 0, sizeof(pp));
+                    const SlotParams &sp = hsp_[s];
+                    const double nref = sp.nref, qx = p.qx[iq], qy = p.qy[iq];
+                    pp.slot = s;
+                    pp.iq = iq;
+                    pp.pol = pol;
+                    pp.qx = qx;
+                    pp.qy = qy;
+                    const double th = 0.5 * kPi * pol;
+                    pp.ex = std::cos(th);
+                    pp.ey = std::sin(th);
+                    if (p.Nx > 3) {
+                        double xs = -sp.dZ * qx / (2 * nref * sp.dX);
+                        double ms = std::floor(xs);
+                        pp.ox = (int)ms - 1;
+                        cr_weights(xs - ms, pp.wx);
+                    } else {
+                        pp.wx[1] = 1;
+                    }
+                    if (p.Ny > 3) {
+                        double ys = -sp.dZ * qy / (2 * nref * sp.dX);  // deltaX, as the reference (:34)
+                        double ms = std::floor(ys);
+                        pp.oy = (int)ms - 1;
+                        cr_weights(ys - ms, pp.wy);
+                    } else {
+                        pp.wy[1] = 1;
+                    }
+                    const double phi = -sp.dZ * (qx * qx + qy * qy) / (4 * nref) * 2.0 * (p.Nz - 1);
+                    pp.ph_re = std::cos(phi);
+                    pp.ph_im = std::sin(phi);
+                }
+        timed(KF_SETUP, [&] {
+            HIPCHK(hipMemcpyAsync(pp_.ptr, hpp_.data(), nprops * sizeof(PropParams), hipMemcpyHostToDevice, st_));
+            C2<T> *F = fb_[0].ptr;
+            if (p.beam == BeamKind::None) {
+                const c64 *Ein = E_.ptr + (size_t)w0 * sz_.nq * 4 * sz_.Nfull;
+                launch_pack_fields<T>(Ein, g_, pp_.ptr, nprops, sz_.nq, F, st_);
+            } else {
+                launch_init_fields<T>(lc_.ptr, mask_.ptr, g_, sp_.ptr, pp_.ptr, nprops,
+                                      p.beam == BeamKind::Gaussian ? 1 : 0, p.waist, p.dx, p.dy, F, st_);
+            }
+        });
+    }
+
+    // the z-march over all slabs; returns the buffer holding the final state
+    C2<T> *march(int nslots, int nprops) {
+        if (o_.use_graph && !o_.profile) {
+            auto key = std::make_pair(nslots, nprops);
+            auto it = graphs_.find(key);
+            if (it == graphs_.end()) {
+                hipGraph_t graph;
+                HIPCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+                C2<T> *state = nullptr;
+                try {
+                    state = march_launches(nslots, nprops);
+                } catch (...) {
+                    hipGraph_t junk;
+                    (void)hipStreamEndCapture(st_, &junk);
+                    if (junk) (void)hipGraphDestroy(junk);
+                    throw;
+                }
+                HIPCHK(hipStreamEndCapture(st_, &graph));
+                hipGraphExec_t exec;
+                HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                it = graphs_.emplace(key, Captured{exec, graph, state}).first;
+            }
+            HIPCHK(hipGraphLaunch(it->second.exec, st_));
+            return it->second.state;
+        }
+        return march_launches(nslots, nprops);
+    }
+
+    C2<T> *march_launches(int nslots, int nprops) {
+        C2<T> *F = fb_[0].ptr, *T1 = fb_[1].ptr, *T2 = fb_[2].ptr, *T3 = fb_[3].ptr;
+        const int nw = p_.n_woodbury;
+        for (int iz = 0; iz < p_.Nz - 1; iz++) {
+            timed(KF_MATERIAL, [&] {
+                launch_slab_material(lc_.ptr, mask_.ptr, weights_.ptr, g_, sp_.ptr, nslots, iz, sm_.ptr, st_);
+            });
+            timed(KF_COEFS, [&] { launch_coefs<T>(sm_.ptr, g_, sp_.ptr, nslots, cf_.ptr, st_); });
+            // Sh, P
+            timed(KF_SHIFT, [&] { launch_shift<T>(F, T1, g_, pp_.ptr, nprops, st_); });
+            timed(KF_PHASE, [&] { launch_phase<T>(T1, cf_.ptr, g_, pp_.ptr, nprops, st_); });
+            // forward y half-step
+            timed(KF_EXPLICIT, [&] { launch_explicit<T>(1, T1, T2, cf_.ptr, g_, pp_.ptr, nprops, st_); });
+            // backward x half-step with its Neumann series
+            if (g_.px) {
+                timed(KF_SOLVE_X, [&] { launch_line_solve<T>(0, 0, T2, T1, F, cf_.ptr, g_, pp_.ptr, nprops, st_); });
+                C2<T> *a = T1, *b = T3;
+                for (int it = 0; it < nw; it++) {
+                    timed(KF_SOLVE_X, [&] { launch_line_solve<T>(0, 1, a, b, F, cf_.ptr, g_, pp_.ptr, nprops, st_); });
+                    std::swap(a, b);
+                }
+            } else {
+                std::swap(F, T2);
+            }
+            // forward x half-step
+            timed(KF_EXPLICIT, [&] { launch_explicit<T>(0, F, T2, cf_.ptr, g_, pp_.ptr, nprops, st_); });
+            // backward y half-step with its Neumann series
+            if (g_.py) {
+                timed(KF_SOLVE_Y, [&] { launch_line_solve<T>(1, 0, T2, T1, F, cf_.ptr, g_, pp_.ptr, nprops, st_); });
+                C2<T> *a = T1, *b = T3;
+                for (int it = 0; it < nw; it++) {
+                    timed(KF_SOLVE_Y, [&] { launch_line_solve<T>(1, 1, a, b, F, cf_.ptr, g_, pp_.ptr, nprops, st_); });
+                    std::swap(a, b);
+                }
+            } else {
+                std::swap(F, T2);
+            }
+            // P, Sh
+            timed(KF_PHASE, [&] { launch_phase<T>(F, cf_.ptr, g_, pp_.ptr, nprops, st_); });
+            timed(KF_SHIFT, [&] { launch_shift<T>(F, T1, g_, pp_.ptr, nprops, st_); });
+            std::swap(F, T1);
+        }
+        return F;
+    }
+
+    void output_batch(int w0, int nslots, int nprops, const C2<T> *state) {
+        c64 *Eb = E_.ptr + (size_t)w0 * sz_.nq * 4 * sz_.Nfull;
+        timed(KF_OUTPUT, [&] {
+            launch_output<T>(state, lc_.ptr, mask_.ptr, g_, sp_.ptr, pp_.ptr, nprops, sz_.nq, Eb, st_);
+        });
+        timed(KF_FILTER, [&] {
+            const int planes = nslots * sz_.nq * 4;
+            launch_fft2(Eb, p_.Nx, p_.Ny, planes, -1, twx_.ptr, twy_.ptr, plan_x_, plan_y_, st_);
+            launch_filter(Eb, g_, sp_.ptr, nslots, q_.ptr, sz_.nq, iso_t_.ptr, iso_n_.ptr, (int)p_.iso_thick.size(),
+                          p_.dx, p_.dy, p_.z_foc, p_.NA, st_);
+            launch_fft2(Eb, p_.Nx, p_.Ny, planes, +1, twx_.ptr, twy_.ptr, plan_x_, plan_y_, st_);
+        });
+    }
+};
+
+}  // namespace
+
+int64_t estimate_footprint(const Problem &p, const Options &o) {
+    Sizes s = sizes_of(p, o);
+    const int64_t fsz = o.precision == 32 ? 8 : 16;
+    int64_t b = p.voxels() * p.field_dim * 8;
+    if (!p.mask.empty()) b += p.voxels();
+    b += (int64_t)p.weights.size() * 8;
+    b += (int64_t)s.nslots_max * SM_COUNT * s.Nfull * 8;
+    b += (int64_t)s.nslots_max * CF_COUNT * s.NP * fsz;
+    b += 4 * (int64_t)s.nprops_max * 2 * s.NP * fsz;
+    b += (int64_t)s.nwl * s.nq * 4 * s.Nfull * 16;
+    return b;
+}
+
+std::unique_ptr<Runner> make_runner(const Problem &p, const Options &o, hipStream_t stream) {
+    if (o.precision == 32) return std::unique_ptr<Runner>(new RunnerT<float>(p, o, stream));
+    if (o.precision == 64) return std::unique_ptr<Runner>(new RunnerT<double>(p, o, stream));
+    throw BpmError("precision must be 32 or 64");
+}
+
+}  // namespace nmk

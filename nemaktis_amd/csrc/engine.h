@@ -1,0 +1,58 @@
+// Device engine of the bpm-solver march: owns the HBM buffers of one problem
+// and runs BPMIteration::propagate_fields + ScreenOutput::apply_no_export
+// (BeamPropagationSolver/src/BeamPropagationLib/BPMIteration.cpp:53-177,
+// PostprocessorLib/ScreenOutput.cpp:184-224) on one GPU.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+
+#include "device_common.h"
+#include "settings.h"
+
+namespace nmk {
+
+struct Options {
+    int precision = 64;  // 64: complex128 march; 32: complex64 march, FP64 coefficients
+    int wl_batch = 1;    // wavelengths marched together (<= 0: all)
+    int use_graph = 0;   // capture the z-march as one hipGraph and replay it
+    int device = -1;     // HIP device (-1: current)
+    int profile = 0;     // time every kernel family with events (no graph)
+};
+
+// per-kernel-family device time of one propagate() (profile mode)
+enum KernelFamily {
+    KF_MATERIAL = 0, KF_COEFS, KF_SHIFT, KF_PHASE, KF_EXPLICIT, KF_SOLVE_X, KF_SOLVE_Y, KF_OUTPUT, KF_FILTER,
+    KF_SETUP, KF_COUNT
+};
+
+struct Stats {
+    double setup_ms = 0, march_ms = 0, output_ms = 0, total_ms = 0;
+    double updates = 0;
+    int64_t footprint_bytes = 0;
+    double family_ms[KF_COUNT] = {0};
+    int64_t family_launches[KF_COUNT] = {0};
+};
+
+class Runner {
+public:
+    virtual ~Runner() = default;
+    // caller-supplied input fields (beam profile "None"), complex128, caller layout
+    virtual void upload_fields(const c64 *E_host) = 0;
+    // full propagation on the device, inputs already resident
+    virtual void propagate(Stats *stats) = 0;
+    // output fields (caller layout) back to the host
+    virtual void download(c64 *E_host) = 0;
+    // device copy of the output (caller layout), valid after propagate
+    virtual const c64 *device_output() const = 0;
+    virtual int64_t footprint() const = 0;
+    virtual hipStream_t stream() const = 0;
+};
+
+std::unique_ptr<Runner> make_runner(const Problem &p, const Options &o, hipStream_t stream = nullptr);
+
+// bytes of HBM a run of this problem needs (used for serving admission)
+int64_t estimate_footprint(const Problem &p, const Options &o);
+
+}  // namespace nmk

@@ -1,0 +1,1090 @@
+// HIP kernels of the bpm-solver march for CDNA4 (gfx950).
+//
+// Data layout in HBM
+//   director     lc[c + fd*(ix + Nx*(iy + Ny*iz))]   float64 (caller layout)
+//   slab material sm[(slot*SM_COUNT + c)*Nfull + m]   float64, full grid
+//   coefficients cf[(slot*CF_COUNT + c)*NP + p]       C2<T>, march grid
+//   march state  F[(prop*2 + comp)*NP + p]            C2<T>, march grid
+//   output       E[(2*prop + comp)*Nfull + m]         complex128, full grid
+// with m = ix + Nx*iy on the full grid and p = jx + Mx*jy on the march grid
+// (the periodic interior without the duplicate column/row along periodic
+// axes).  Every kernel walks points with consecutive threads on consecutive
+// x so that loads and stores coalesce.
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace nmk {
+
+#define NMK_LAUNCH_CHECK()                                                                       \
+    do {                                                                                         \
+        hipError_t e_ = hipGetLastError();                                                       \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP launch failed: ") +     \
+                                                       hipGetErrorString(e_));                   \
+    } while (0)
+
+static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------------------
+// permittivity of one voxel (PhysicsLib/PermittivityTensorField.cpp:18-93):
+// e[0..5] eps xx,yy,zz,xy,xz,yz; e[6..8] K = transverse eps; e[9..11] sqrt(K)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double eps_voxel(const double *__restrict__ lc, const uint8_t *__restrict__ mask, int fd,
+                                            int64_t v, const SlotParams &s, double *e) {
+    const double *n = lc + (int64_t)fd * v;
+    const bool m = mask ? (mask[v] != 0) : true;
+    if (fd == 3) {
+        double n0 = n[0], n1 = n[1], n2 = n[2];
+        e[0] = m ? s.eperp + s.ea * (n0 * n0) : s.ehost;
+        e[1] = m ? s.eperp + s.ea * (n1 * n1) : s.ehost;
+        e[2] = m ? s.eperp + s.ea * (n2 * n2) : s.ehost;
+        e[3] = m ? s.ea * n0 * n1 : 0.0;
+        e[4] = m ? s.ea * n0 * n2 : 0.0;
+        e[5] = m ? s.ea * n1 * n2 : 0.0;
+    } else {
+        double ea_eff = 2 * s.ea / 3;
+        double eiso = s.eperp + s.ea / 3.;
+        e[0] = m ? eiso + ea_eff * n[0] : s.ehost;
+        e[1] = m ? eiso + ea_eff * n[1] : s.ehost;
+        e[2] = m ? eiso + ea_eff * n[2] : s.ehost;
+        e[3] = m ? ea_eff * n[3] : 0.0;
+        e[4] = m ? ea_eff * n[4] : 0.0;
+        e[5] = m ? ea_eff * n[5] : 0.0;
+    }
+    e[6] = e[0] - (e[4] * e[4]) / e[2];
+    e[7] = e[1] - (e[5] * e[5]) / e[2];
+    e[8] = e[3] - e[4] * e[5] / e[2];
+    double det = e[6] * e[7] - e[8] * e[8];
+    double tr = 0.5 * (e[6] + e[7]);
+    double neff;
+    if (tr * tr - det < 1e-8) {
+        neff = sqrt(tr);
+    } else {
+        double r = sqrt(tr * tr - det);
+        neff = (sqrt(tr + r) + sqrt(tr - r)) / 2.;
+    }
+    e[9] = neff + (e[6] - e[7]) / (4 * neff);
+    e[10] = neff + (e[7] - e[6]) / (4 * neff);
+    e[11] = e[8] / (2 * neff);
+    return neff;
+}
+
+// ---------------------------------------------------------------------------
+// nref = mean n_eff over every voxel (PermittivityTensorField.cpp:94-98);
+// fixed block ranges and fixed reduction trees make it deterministic
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_nref_partial(const double *__restrict__ lc, const uint8_t *__restrict__ mask,
+                                                      Geo g, const SlotParams *__restrict__ sp, double *partials,
+                                                      int nblocks) {
+    const int slot = blockIdx.y;
+    const SlotParams s = sp[slot];
+    const int64_t V = g.Nfull * g.Nz;
+    const int64_t chunk = (V + nblocks - 1) / nblocks;
+    const int64_t beg = (int64_t)blockIdx.x * chunk;
+    const int64_t end = min(V, beg + chunk);
+    double acc = 0;
+    double e[12];
+    for (int64_t v = beg + threadIdx.x; v < end; v += blockDim.x) acc += eps_voxel(lc, mask, g.field_dim, v, s, e);
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[(int64_t)slot * nblocks + blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_nref_final(SlotParams *sp, const double *__restrict__ partials, int nblocks,
+                                                    double V) {
+    const int slot = blockIdx.x;
+    double acc = 0;
+    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) acc += partials[(int64_t)slot * nblocks + i];
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sp[slot].nref = red[0] / V;
+}
+
+void launch_nref(const double *lc, const uint8_t *mask, const Geo &g, SlotParams *sp, int nslots, double *partials,
+                 int nblocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_nref_partial, dim3(nblocks, nslots), dim3(256), 0, st, lc, mask, g, sp, partials, nblocks);
+    NMK_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_nref_final, dim3(nslots), dim3(256), 0, st, sp, partials, nblocks,
+                       (double)g.Nfull * (double)g.Nz);
+    NMK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// slab material of slab iz (planes iz, iz+1), full grid, FP64:
+// averaged 1/eps_zz, K, sqrt(K)^-1, summed eps_az/eps_zz and the phase
+// operator exp(i alpha d) (PhaseEvolutionOperator.cpp:34-63)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_slab_material(const double *__restrict__ lc, const uint8_t *__restrict__ mask,
+                                                       const double *__restrict__ weights, Geo g,
+                                                       const SlotParams *__restrict__ sp, int iz,
+                                                       double *__restrict__ sm) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.Nfull) return;
+    const int slot = blockIdx.y;
+    const SlotParams s = sp[slot];
+    double a[12], b[12];
+    eps_voxel(lc, mask, g.field_dim, m + g.Nfull * iz, s, a);
+    eps_voxel(lc, mask, g.field_dim, m + g.Nfull * (iz + 1), s, b);
+    double *o = sm + (int64_t)slot * SM_COUNT * g.Nfull + m;
+    const int64_t N = g.Nfull;
+    o[SM_EINV * N] = (1. / a[2] + 1. / b[2]) / 2;
+    o[SM_KXX * N] = (a[6] + b[6]) / 2;
+    o[SM_KYY * N] = (a[7] + b[7]) / 2;
+    o[SM_KXY * N] = (a[8] + b[8]) / 2;
+    double sxx = (a[9] + b[9]) / 2, syy = (a[10] + b[10]) / 2, sxy = (a[11] + b[11]) / 2;
+    double det = sxx * syy - sxy * sxy;
+    o[SM_IXX * N] = syy / det;
+    o[SM_IYY * N] = sxx / det;
+    o[SM_IXY * N] = -sxy / det;
+    o[SM_VX * N] = a[4] / a[2] + b[4] / b[2];
+    o[SM_VY * N] = a[5] / a[2] + b[5] / b[2];
+
+    double w = 0.5, wpz = 0.5;
+    if (weights) {
+        const int64_t v = m + N * iz;
+        w = weights[2 * v];
+        wpz = weights[2 * v + 1];
+    }
+    const double dZ = s.dZ;
+    double dxx = 0.5 * dZ * (w * a[9] + wpz * b[9] - s.no);
+    double dyy = 0.5 * dZ * (w * a[10] + wpz * b[10] - s.no);
+    double dxy = 0.5 * dZ * (w * a[11] + wpz * b[11]);
+    double diff = (dxx - dyy) / 2., tr = (dxx + dyy) / 2.;
+    double dt = sqrt(dxy * dxy + diff * diff);
+    // alpha = 1 + i*k: sin/cos of the complex argument alpha*dt = (dt, k*dt)
+    const double kk = s.alpha_im;
+    double u = dt, vv = kk * dt;
+    double su = sin(u), cu = cos(u);
+    double chv = (vv == 0.0) ? 1.0 : cosh(vv), shv = (vv == 0.0) ? 0.0 : sinh(vv);
+    c64 sinc = fabs(dt) < 1e-8 ? mk(1.0, 0.0) : mk(su * chv / dt, cu * shv / dt);
+    c64 cs = mk(cu * chv, -su * shv);
+    // exp(i*alpha*tr) = exp(-k*tr) * (cos tr, sin tr)
+    double amp = (kk == 0.0) ? 1.0 : exp(-kk * tr);
+    c64 ph = mk(amp * cos(tr), amp * sin(tr));
+    c64 ids = mk(-diff * sinc.y, diff * sinc.x);   // i*diff*sinc
+    c64 ixs = mk(-dxy * sinc.y, dxy * sinc.x);     // i*dxy*sinc
+    c64 pxx = ph * (cs + ids), pyy = ph * (cs - ids), pxy = ph * ixs;
+    o[SM_PXX_RE * N] = pxx.x;
+    o[SM_PXX_IM * N] = pxx.y;
+    o[SM_PYY_RE * N] = pyy.x;
+    o[SM_PYY_IM * N] = pyy.y;
+    o[SM_PXY_RE * N] = pxy.x;
+    o[SM_PXY_IM * N] = pxy.y;
+}
+
+void launch_slab_material(const double *lc, const uint8_t *mask, const double *weights, const Geo &g,
+                          const SlotParams *sp, int nslots, int iz, double *sm, hipStream_t st) {
+    hipLaunchKernelGGL(k_slab_material, dim3(blocks_for(g.Nfull, 256), nslots), dim3(256), 0, st, lc, mask, weights,
+                       g, sp, iz, sm);
+    NMK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// mu*R coefficients (ParaxialPrimaryEvolutionOperator.cpp:212-490) assembled
+// per row point from the slab material of the point and its neighbours:
+//   D  = D0 + D1 K                      (D1 with the seam rule of :428-430)
+//   R  = i W + (D S^-1 + S^-1 D) / 4,    W = (v_a d_b + d_a v_b) / 2
+// for the three stencil families DX, DY, DXDY and their 2x2 blocks.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_coefs(const double *__restrict__ sm, Geo g, const SlotParams *__restrict__ sp,
+                                               C2<T> *__restrict__ cf) {
+    const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= g.NP) return;
+    const int slot = blockIdx.y;
+    const SlotParams s = sp[slot];
+    const int ix = (int)(ip % g.Mx), iy = (int)(ip / g.Mx);
+    const int64_t N = g.Nfull;
+    const double *S = sm + (int64_t)slot * SM_COUNT * N;
+    auto at = [&](int c, int jx, int jy) { return S[c * N + jx + (int64_t)g.Nx * jy]; };
+    const int xm = g.px ? (ix == 0 ? g.Mx - 1 : ix - 1) : ix;
+    const int xp = g.px ? (ix == g.Mx - 1 ? 0 : ix + 1) : ix;
+    const int ym = g.py ? (iy == 0 ? g.My - 1 : iy - 1) : iy;
+    const int yp = g.py ? (iy == g.My - 1 ? 0 : iy + 1) : iy;
+
+    double re[40], im[40];
+#pragma unroll
+    for (int k = 0; k < 40; k++) re[k] = im[k] = 0.0;
+
+    const double dX = s.dX, dY = s.dY;
+    const double ixx_r = at(SM_IXX, ix, iy), iyy_r = at(SM_IYY, ix, iy), ixy_r = at(SM_IXY, ix, iy);
+    const double vx_r = at(SM_VX, ix, iy), vy_r = at(SM_VY, ix, iy);
+    const double e_r = at(SM_EINV, ix, iy);
+
+    // For each stencil slot: D of the four blocks, the walk-off part i*W, then
+    // D.Sinv (Sinv at the column point) and Sinv.D (Sinv at the row point),
+    // accumulated in the reference's order.
+    if (g.px) {
+        // ---- DX family: slots 0 minus, 1 centre, 2 plus ----
+        const double e_mx = at(SM_EINV, xm, iy), e_px = at(SM_EINV, ix + 1, iy);  // ix+1: duplicate at the seam
+        const double cm = 0.5 * (e_mx + e_r) / (dX * dX), cp = 0.5 * (e_px + e_r) / (dX * dX);
+        const double d1[3] = {cm, -cm - cp, cp};
+        const double d0[3] = {1. / (dX * dX), -2. / (dX * dX), 1. / (dX * dX)};
+        const int qx[3] = {xm, ix, xp};
+        for (int sl = 0; sl < 3; sl++) {
+            const int q = qx[sl];
+            double D[4] = {d1[sl] * at(SM_KXX, q, iy), d1[sl] * at(SM_KXY, q, iy), 0.0, d0[sl]};
+            if (sl != 1) {
+                const double d = (sl == 0 ? -1.0 : 1.0) * (0.5 / dX);
+                const double vx_q = at(SM_VX, q, iy), vy_q = at(SM_VY, q, iy);
+                im[CF_DX + 0 * 3 + sl] += d * (0.25 * vx_r);
+                im[CF_DX + 2 * 3 + sl] += d * (0.25 * vy_r);
+                im[CF_DX + 0 * 3 + sl] += d * (0.25 * vx_q);
+                im[CF_DX + 1 * 3 + sl] += d * (0.25 * vy_q);
+            }
+            const double ixx = at(SM_IXX, q, iy), iyy = at(SM_IYY, q, iy), ixy = at(SM_IXY, q, iy);
+            for (int rb = 0; rb < 2; rb++) {
+                re[CF_DX + (rb * 2 + 0) * 3 + sl] += 0.25 * D[rb * 2 + 0] * ixx;
+                re[CF_DX + (rb * 2 + 0) * 3 + sl] += 0.25 * D[rb * 2 + 1] * ixy;
+                re[CF_DX + (rb * 2 + 1) * 3 + sl] += 0.25 * D[rb * 2 + 0] * ixy;
+                re[CF_DX + (rb * 2 + 1) * 3 + sl] += 0.25 * D[rb * 2 + 1] * iyy;
+            }
+            for (int cb = 0; cb < 2; cb++) {
+                re[CF_DX + (0 + cb) * 3 + sl] += 0.25 * ixx_r * D[0 + cb];
+                re[CF_DX + (0 + cb) * 3 + sl] += 0.25 * ixy_r * D[2 + cb];
+                re[CF_DX + (2 + cb) * 3 + sl] += 0.25 * ixy_r * D[0 + cb];
+                re[CF_DX + (2 + cb) * 3 + sl] += 0.25 * iyy_r * D[2 + cb];
+            }
+        }
+    }
+    if (g.py) {
+        // ---- DY family: slots 0 minus, 1 centre, 2 plus ----
+        const double e_my = at(SM_EINV, ix, ym), e_py = at(SM_EINV, ix, iy + 1);  // iy+1: duplicate at the seam
+        const double cm = 0.5 * (e_my + e_r) / (dY * dY), cp = 0.5 * (e_py + e_r) / (dY * dY);
+        const double d1[3] = {cm, -cm - cp, cp};
+        const double d0[3] = {1. / (dY * dY), -2. / (dY * dY), 1. / (dY * dY)};
+        const int qy[3] = {ym, iy, yp};
+        for (int sl = 0; sl < 3; sl++) {
+            const int q = qy[sl];
+            double D[4] = {d0[sl], 0.0, d1[sl] * at(SM_KXY, ix, q), d1[sl] * at(SM_KYY, ix, q)};
+            if (sl != 1) {
+                const double d = (sl == 0 ? -1.0 : 1.0) * (0.5 / dY);
+                const double vx_q = at(SM_VX, ix, q), vy_q = at(SM_VY, ix, q);
+                im[CF_DY + 1 * 3 + sl] += d * (0.25 * vx_r);
+                im[CF_DY + 3 * 3 + sl] += d * (0.25 * vy_r);
+                im[CF_DY + 2 * 3 + sl] += d * (0.25 * vx_q);
+                im[CF_DY + 3 * 3 + sl] += d * (0.25 * vy_q);
+            }
+            const double ixx = at(SM_IXX, ix, q), iyy = at(SM_IYY, ix, q), ixy = at(SM_IXY, ix, q);
+            for (int rb = 0; rb < 2; rb++) {
+                re[CF_DY + (rb * 2 + 0) * 3 + sl] += 0.25 * D[rb * 2 + 0] * ixx;
+                re[CF_DY + (rb * 2 + 0) * 3 + sl] += 0.25 * D[rb * 2 + 1] * ixy;
+                re[CF_DY + (rb * 2 + 1) * 3 + sl] += 0.25 * D[rb * 2 + 0] * ixy;
+                re[CF_DY + (rb * 2 + 1) * 3 + sl] += 0.25 * D[rb * 2 + 1] * iyy;
+            }
+            for (int cb = 0; cb < 2; cb++) {
+                re[CF_DY + (0 + cb) * 3 + sl] += 0.25 * ixx_r * D[0 + cb];
+                re[CF_DY + (0 + cb) * 3 + sl] += 0.25 * ixy_r * D[2 + cb];
+                re[CF_DY + (2 + cb) * 3 + sl] += 0.25 * ixy_r * D[0 + cb];
+                re[CF_DY + (2 + cb) * 3 + sl] += 0.25 * iyy_r * D[2 + cb];
+            }
+        }
+    }
+    if (g.px && g.py) {
+        // ---- DXDY family: slots 0 mxmy, 1 pxmy, 2 mxpy, 3 pxpy ----
+        const double dxy = dX * dY;
+        const double e_px = at(SM_EINV, ix + 1, iy), e_mx = at(SM_EINV, xm, iy);
+        const double e_py = at(SM_EINV, ix, iy + 1), e_my = at(SM_EINV, ix, ym);
+        const double pp = 0.25 * e_px / dxy, mm = 0.25 * e_mx / dxy;
+        const double qp = 0.25 * e_py / dxy, qm = 0.25 * e_my / dxy;
+        const double i0 = 0.25 / dxy;
+        const double m01[4] = {mm, -pp, -mm, pp};
+        const double m10[4] = {qm, -qm, -qp, qp};
+        const double dd[4] = {-i0, i0, i0, -i0};
+        const int cx[4] = {xm, xp, xm, xp}, cy[4] = {ym, ym, yp, yp};
+        for (int sl = 0; sl < 4; sl++) {
+            const int qxx = cx[sl], qyy = cy[sl];
+            const double kxx = at(SM_KXX, qxx, qyy), kyy = at(SM_KYY, qxx, qyy), kxy = at(SM_KXY, qxx, qyy);
+            double D[4] = {m01[sl] * kxy, dd[sl] + m01[sl] * kyy, dd[sl] + m10[sl] * kxx, m10[sl] * kxy};
+            const double ixx = at(SM_IXX, qxx, qyy), iyy = at(SM_IYY, qxx, qyy), ixy = at(SM_IXY, qxx, qyy);
+            for (int rb = 0; rb < 2; rb++) {
+                re[CF_XY + (rb * 2 + 0) * 4 + sl] += 0.25 * D[rb * 2 + 0] * ixx;
+                re[CF_XY + (rb * 2 + 0) * 4 + sl] += 0.25 * D[rb * 2 + 1] * ixy;
+                re[CF_XY + (rb * 2 + 1) * 4 + sl] += 0.25 * D[rb * 2 + 0] * ixy;
+                re[CF_XY + (rb * 2 + 1) * 4 + sl] += 0.25 * D[rb * 2 + 1] * iyy;
+            }
+            for (int cb = 0; cb < 2; cb++) {
+                re[CF_XY + (0 + cb) * 4 + sl] += 0.25 * ixx_r * D[0 + cb];
+                re[CF_XY + (0 + cb) * 4 + sl] += 0.25 * ixy_r * D[2 + cb];
+                re[CF_XY + (2 + cb) * 4 + sl] += 0.25 * ixy_r * D[0 + cb];
+                re[CF_XY + (2 + cb) * 4 + sl] += 0.25 * iyy_r * D[2 + cb];
+            }
+        }
+    }
+    // store mu*R with mu = i*dZ/2
+    const double hz = 0.5 * s.dZ;
+    C2<T> *o = cf + (int64_t)slot * CF_COUNT * g.NP + ip;
+#pragma unroll
+    for (int k = 0; k < 40; k++) o[(int64_t)k * g.NP] = C2<T>{(T)(-hz * im[k]), (T)(hz * re[k])};
+    o[(int64_t)(CF_P + 0) * g.NP] = C2<T>{(T)at(SM_PXX_RE, ix, iy), (T)at(SM_PXX_IM, ix, iy)};
+    o[(int64_t)(CF_P + 1) * g.NP] = C2<T>{(T)at(SM_PYY_RE, ix, iy), (T)at(SM_PYY_IM, ix, iy)};
+    o[(int64_t)(CF_P + 2) * g.NP] = C2<T>{(T)at(SM_PXY_RE, ix, iy), (T)at(SM_PXY_IM, ix, iy)};
+}
+
+template <class T>
+void launch_coefs(const double *sm, const Geo &g, const SlotParams *sp, int nslots, C2<T> *cf, hipStream_t st) {
+    hipLaunchKernelGGL(k_coefs<T>, dim3(blocks_for(g.NP, 256), nslots), dim3(256), 0, st, sm, g, sp, cf);
+    NMK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// input beam (PhysicsLib/BeamProfile.cpp:17-39) + input Fresnel
+// (BeamPropagationLib/FresnelOperator.cpp:16-38), written to the march grid
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_init_fields(const double *__restrict__ lc, const uint8_t *__restrict__ mask,
+                                                     Geo g, const SlotParams *__restrict__ sp,
+                                                     const PropParams *__restrict__ pp, int gaussian, double waist,
+                                                     double dx, double dy, C2<T> *__restrict__ F) {
+    const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= g.NP) return;
+    const int prop = blockIdx.y;
+    const PropParams p = pp[prop];
+    const SlotParams s = sp[p.slot];
+    const int ix = (int)(ip % g.Mx), iy = (int)(ip / g.Mx);
+    double e[12];
+    eps_voxel(lc, mask, g.field_dim, ix + (int64_t)g.Nx * iy, s, e);
+    double gx = 1.0;
+    if (gaussian) {
+        double x = (ix - (g.Nx - 1) / 2.) * dx, y = (iy - (g.Ny - 1) / 2.) * dy;
+        gx = exp(-(x * x + y * y) / (2 * waist * waist));
+    }
+    const double E0 = p.ex * gx, E1 = p.ey * gx;
+    const double n = s.nin, nxx = e[9], nyy = e[10], nxy = e[11];
+    const double det = (n + nxx) * (n + nyy) - nxy * nxy;
+    const double a = 2 * n * (n + nyy) / det * E0 - 2 * n * nxy / det * E1;
+    const double b = -2 * n * nxy / det * E0 + 2 * n * (n + nxx) / det * E1;
+    F[((int64_t)prop * 2 + 0) * g.NP + ip] = C2<T>{(T)a, (T)0};
+    F[((int64_t)prop * 2 + 1) * g.NP + ip] = C2<T>{(T)b, (T)0};
+}
+
+template <class T>
+void launch_init_fields(const double *lc, const uint8_t *mask, const Geo &g, const SlotParams *sp,
+                        const PropParams *pp, int nprops, int gaussian, double waist, double dx, double dy,
+                        C2<T> *F, hipStream_t st) {
+    hipLaunchKernelGGL(k_init_fields<T>, dim3(blocks_for(g.NP, 256), nprops), dim3(256), 0, st, lc, mask, g, sp, pp,
+                       gaussian, waist, dx, dy, F);
+    NMK_LAUNCH_CHECK();
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pack_fields(const c64 *__restrict__ E, Geo g, C2<T> *__restrict__ F) {
+    const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= g.NP) return;
+    const int prop = blockIdx.y;
+    const int ix = (int)(ip % g.Mx), iy = (int)(ip / g.Mx);
+    const int64_t m = ix + (int64_t)g.Nx * iy;
+    for (int c = 0; c < 2; c++) F[((int64_t)prop * 2 + c) * g.NP + ip] = cast2<T>(E[((int64_t)prop * 2 + c) * g.Nfull + m]);
+}
+
+template <class T>
+void launch_pack_fields(const c64 *E, const Geo &g, const PropParams *pp, int nprops, int nq, C2<T> *F,
+                        hipStream_t st) {
+    (void)pp;
+    (void)nq;
+    hipLaunchKernelGGL(k_pack_fields<T>, dim3(blocks_for(g.NP, 256), nprops), dim3(256), 0, st, E, g, F);
+    NMK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// Catmull-Rom half-step shift (BeamPropagationLib/SimpleShiftOperator.cpp:52-85):
+// 4 taps along x, then 4 taps along y, on the periodic march grid.  The
+// global phase factor of each half-step is accumulated on the host in FP64
+// and applied once at the output.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wrapm(int i, int M) {
+    int r = i % M;
+    return r < 0 ? r + M : r;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_shift(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, Geo g,
+                                               const PropParams *__restrict__ pp) {
+    const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= g.NP) return;
+    const int prop = blockIdx.y;
+    const PropParams p = pp[prop];
+    const int ix = (int)(ip % g.Mx), iy = (int)(ip / g.Mx);
+    int cx[4], cy[4];
+    T wx[4], wy[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        cx[k] = wrapm(ix + p.ox + k, g.Mx);
+        cy[k] = wrapm(iy + p.oy + k, g.My);
+        wx[k] = (T)p.wx[k];
+        wy[k] = (T)p.wy[k];
+    }
+    for (int c = 0; c < 2; c++) {
+        const C2<T> *f = in + ((int64_t)prop * 2 + c) * g.NP;
+        auto xpass = [&](int row) -> C2<T> {
+            const C2<T> *r = f + (int64_t)g.Mx * row;
+            if (!g.px) return r[ix];
+            C2<T> a = wx[0] * r[cx[0]];
+            a = a + wx[1] * r[cx[1]];
+            a = a + wx[2] * r[cx[2]];
+            a = a + wx[3] * r[cx[3]];
+            return a;
+        };
+        C2<T> acc;
+        if (g.py) {
+            acc = wy[0] * xpass(cy[0]);
+            acc = acc + wy[1] * xpass(cy[1]);
+            acc = acc + wy[2] * xpass(cy[2]);
+            acc = acc + wy[3] * xpass(cy[3]);
+        } else {
+            acc = xpass(iy);
+        }
+        out[((int64_t)prop * 2 + c) * g.NP + ip] = acc;
+    }
+}
+
+template <class T>
+void launch_shift(const C2<T> *in, C2<T> *out, const Geo &g, const PropParams *pp, int nprops, hipStream_t st) {
+    hipLaunchKernelGGL(k_shift<T>, dim3(blocks_for(g.NP, 256), nprops), dim3(256), 0, st, in, out, g, pp);
+    NMK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// pointwise 2x2 phase operator (PhaseEvolutionOperator.cpp:14-32)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_phase(C2<T> *__restrict__ f, const C2<T> *__restrict__ cf, Geo g,
+                                               const PropParams *__restrict__ pp) {
+    const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= g.NP) return;
+    const int prop = blockIdx.y;
+    const int slot = pp[prop].slot;
+    const C2<T> *c = cf + ((int64_t)slot * CF_COUNT + CF_P) * g.NP + ip;
+    const C2<T> pxx = c[0], pyy = c[g.NP], pxy = c[2 * g.NP];
+    C2<T> *a = f + ((int64_t)prop * 2) * g.NP + ip;
+    C2<T> *b = a + g.NP;
+    const C2<T> x = *a, y = *b;
+    *a = pxx * x + pxy * y;
+    *b = pxy * x + pyy * y;
+}
+
+template <class T>
+void launch_phase(C2<T> *f, const C2<T> *cf, const Geo &g, const PropParams *pp, int nprops, hipStream_t st) {
+    hipLaunchKernelGGL(k_phase<T>, dim3(blocks_for(g.NP, 256), nprops), dim3(256), 0, st, f, cf, g, pp);
+    NMK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// explicit half-steps (ParaxialPrimaryEvolutionOperator.cpp:153-160, 181-188)
+//   dir 1: out = in + mu A_y in,  A_y = [DY00 DY01; DY10+XY10 DY11+XY11]
+//   dir 0: out = in + mu A_x in,  A_x = [DX00+XY00 DX01+XY01; DX10 DX11]
+// ---------------------------------------------------------------------------
+struct Nbr {
+    int xm, xp, ym, yp;
+};
+__device__ __forceinline__ Nbr neighbours(const Geo &g, int ix, int iy) {
+    Nbr n;
+    n.xm = g.px ? (ix == 0 ? g.Mx - 1 : ix - 1) : ix;
+    n.xp = g.px ? (ix == g.Mx - 1 ? 0 : ix + 1) : ix;
+    n.ym = g.py ? (iy == 0 ? g.My - 1 : iy - 1) : iy;
+    n.yp = g.py ? (iy == g.My - 1 ? 0 : iy + 1) : iy;
+    return n;
+}
+
+// sum over a 3-point x stencil of coefficient block `c0` applied to field f
+template <class T>
+__device__ __forceinline__ C2<T> sx3(C2<T> acc, const C2<T> *c, int64_t NP, int k0, const C2<T> *f, int64_t row,
+                                     const Nbr &n, int ix) {
+    acc = cmad(acc, c[(int64_t)(k0 + 0) * NP], f[row + n.xm]);
+    acc = cmad(acc, c[(int64_t)(k0 + 1) * NP], f[row + ix]);
+    acc = cmad(acc, c[(int64_t)(k0 + 2) * NP], f[row + n.xp]);
+    return acc;
+}
+template <class T>
+__device__ __forceinline__ C2<T> sy3(C2<T> acc, const C2<T> *c, int64_t NP, int k0, const C2<T> *f, int Mx,
+                                     const Nbr &n, int ix, int iy) {
+    acc = cmad(acc, c[(int64_t)(k0 + 0) * NP], f[ix + (int64_t)Mx * n.ym]);
+    acc = cmad(acc, c[(int64_t)(k0 + 1) * NP], f[ix + (int64_t)Mx * iy]);
+    acc = cmad(acc, c[(int64_t)(k0 + 2) * NP], f[ix + (int64_t)Mx * n.yp]);
+    return acc;
+}
+template <class T>
+__device__ __forceinline__ C2<T> sxy4(C2<T> acc, const C2<T> *c, int64_t NP, int k0, const C2<T> *f, int Mx,
+                                      const Nbr &n) {
+    acc = cmad(acc, c[(int64_t)(k0 + 0) * NP], f[n.xm + (int64_t)Mx * n.ym]);
+    acc = cmad(acc, c[(int64_t)(k0 + 1) * NP], f[n.xp + (int64_t)Mx * n.ym]);
+    acc = cmad(acc, c[(int64_t)(k0 + 2) * NP], f[n.xm + (int64_t)Mx * n.yp]);
+    acc = cmad(acc, c[(int64_t)(k0 + 3) * NP], f[n.xp + (int64_t)Mx * n.yp]);
+    return acc;
+}
+
+template <class T, int DIR>
+__global__ __launch_bounds__(256) void k_explicit(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
+                                                  const C2<T> *__restrict__ cf, Geo g,
+                                                  const PropParams *__restrict__ pp) {
+    const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= g.NP) return;
+    const int prop = blockIdx.y;
+    const int slot = pp[prop].slot;
+    const int ix = (int)(ip % g.Mx), iy = (int)(ip / g.Mx);
+    const Nbr n = neighbours(g, ix, iy);
+    const int64_t NP = g.NP;
+    const C2<T> *c = cf + (int64_t)slot * CF_COUNT * NP + ip;
+    const C2<T> *f0 = in + ((int64_t)prop * 2) * NP;
+    const C2<T> *f1 = f0 + NP;
+    const int64_t row = (int64_t)g.Mx * iy;
+    C2<T> a0{0, 0}, a1{0, 0};
+    if (DIR == 1) {
+        a0 = sy3(a0, c, NP, CF_DY + 0 * 3, f0, g.Mx, n, ix, iy);
+        a0 = sy3(a0, c, NP, CF_DY + 1 * 3, f1, g.Mx, n, ix, iy);
+        a1 = sy3(a1, c, NP, CF_DY + 3 * 3, f1, g.Mx, n, ix, iy);
+        a1 = sxy4(a1, c, NP, CF_XY + 3 * 4, f1, g.Mx, n);
+        a1 = sy3(a1, c, NP, CF_DY + 2 * 3, f0, g.Mx, n, ix, iy);
+        a1 = sxy4(a1, c, NP, CF_XY + 2 * 4, f0, g.Mx, n);
+    } else {
+        a0 = sx3(a0, c, NP, CF_DX + 0 * 3, f0, row, n, ix);
+        a0 = sxy4(a0, c, NP, CF_XY + 0 * 4, f0, g.Mx, n);
+        a0 = sx3(a0, c, NP, CF_DX + 1 * 3, f1, row, n, ix);
+        a0 = sxy4(a0, c, NP, CF_XY + 1 * 4, f1, g.Mx, n);
+        a1 = sx3(a1, c, NP, CF_DX + 3 * 3, f1, row, n, ix);
+        a1 = sx3(a1, c, NP, CF_DX + 2 * 3, f0, row, n, ix);
+    }
+    C2<T> *o0 = out + ((int64_t)prop * 2) * NP;
+    o0[ip] = f0[ip] + a0;
+    o0[NP + ip] = f1[ip] + a1;
+}
+
+template <class T>
+void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, const Geo &g, const PropParams *pp,
+                     int nprops, hipStream_t st) {
+    dim3 grid(blocks_for(g.NP, 256), nprops);
+    if (dir == 0) hipLaunchKernelGGL((k_explicit<T, 0>), grid, dim3(256), 0, st, in, out, cf, g, pp);
+    else hipLaunchKernelGGL((k_explicit<T, 1>), grid, dim3(256), 0, st, in, out, cf, g, pp);
+    NMK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// cyclic line solves (I - mu T) x = d along x (T = DX00 / DX11) or y
+// (T = DY00 / DY11), the implicit halves of the ADI step with their Neumann
+// series (ParaxialPrimaryEvolutionOperator.cpp:162-179, 190-207; line solver
+// BlockSparseMatrix.cpp:194-258, 385-450).
+//
+// One workgroup solves one periodic line of length M for one component of
+// one propagation.  The line is cut into P chunks (P a power of two); each
+// thread runs a Thomas elimination on its chunk's interior for the right-hand
+// side and for the two coupling responses, the P chunk-boundary unknowns form
+// a cyclic tridiagonal system solved by parallel cyclic reduction in LDS,
+// and each thread then rebuilds its chunk.
+// In Neumann mode the right-hand side is mu*O*src, evaluated on the fly:
+//   x lines: O_x = [XY00  DX01+XY01; DX10  0]
+//   y lines: O_y = [0  DY01; DY10+XY10  XY11]
+// ---------------------------------------------------------------------------
+constexpr int kMaxLinePar = 256;
+
+template <class T, int DIR>
+__device__ __forceinline__ C2<T> neumann_rhs(int comp, int jx, int jy, const C2<T> *__restrict__ cfs,
+                                             const C2<T> *__restrict__ s0, const C2<T> *__restrict__ s1,
+                                             const Geo &g) {
+    const int64_t NP = g.NP;
+    const int64_t ip = jx + (int64_t)g.Mx * jy;
+    const C2<T> *c = cfs + ip;
+    const Nbr n = neighbours(g, jx, jy);
+    C2<T> a{0, 0};
+    if (DIR == 0) {
+        const int64_t row = (int64_t)g.Mx * jy;
+        if (comp == 0) {
+            a = sx3(a, c, NP, CF_DX + 1 * 3, s1, row, n, jx);
+            a = sxy4(a, c, NP, CF_XY + 1 * 4, s1, g.Mx, n);
+            a = sxy4(a, c, NP, CF_XY + 0 * 4, s0, g.Mx, n);
+        } else {
+            a = sx3(a, c, NP, CF_DX + 2 * 3, s0, row, n, jx);
+        }
+    } else {
+        if (comp == 0) {
+            a = sy3(a, c, NP, CF_DY + 1 * 3, s1, g.Mx, n, jx, jy);
+        } else {
+            a = sy3(a, c, NP, CF_DY + 2 * 3, s0, g.Mx, n, jx, jy);
+            a = sxy4(a, c, NP, CF_XY + 2 * 4, s0, g.Mx, n);
+            a = sxy4(a, c, NP, CF_XY + 3 * 4, s1, g.Mx, n);
+        }
+    }
+    return a;
+}
+
+template <class T, int LMAX, int DIR, int NEU>
+__global__ __launch_bounds__(256) void k_line_solve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
+                                                    C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf, Geo g,
+                                                    const PropParams *__restrict__ pp, int P) {
+    using C = C2<T>;
+    const int line = blockIdx.x, comp = blockIdx.y, prop = blockIdx.z;
+    const int slot = pp[prop].slot;
+    const int M = DIR == 0 ? g.Mx : g.My;
+    const int64_t NP = g.NP;
+    const C *cfs = cf + (int64_t)slot * CF_COUNT * NP;
+    const int fam = DIR == 0 ? CF_DX : CF_DY;
+    const int blk = comp == 0 ? 0 : 3;
+    const C *ca = cfs + (int64_t)(fam + blk * 3 + 0) * NP;
+    const C *cb = cfs + (int64_t)(fam + blk * 3 + 1) * NP;
+    const C *cc = cfs + (int64_t)(fam + blk * 3 + 2) * NP;
+    const C *s0 = src + ((int64_t)prop * 2) * NP;
+    const C *s1 = s0 + NP;
+    C *d_out = dst + ((int64_t)prop * 2 + comp) * NP;
+    C *a_out = acc + ((int64_t)prop * 2 + comp) * NP;
+
+    auto pt = [&](int j) -> int64_t { return DIR == 0 ? (int64_t)j + (int64_t)g.Mx * line : (int64_t)line + (int64_t)g.Mx * j; };
+    auto rhs = [&](int j) -> C {
+        if (NEU == 0) return (comp == 0 ? s0 : s1)[pt(j)];
+        return DIR == 0 ? neumann_rhs<T, 0>(comp, j, line, cfs, s0, s1, g) : neumann_rhs<T, 1>(comp, line, j, cfs, s0, s1, g);
+    };
+    // (I - mu T): a_j = -muT(minus), b_j = 1 - muT(centre), c_j = -muT(plus)
+    auto coef_a = [&](int j) { C v = ca[pt(j)]; return C{-v.x, -v.y}; };
+    auto coef_b = [&](int j) { C v = cb[pt(j)]; return C{(T)1 - v.x, -v.y}; };
+    auto coef_c = [&](int j) { C v = cc[pt(j)]; return C{-v.x, -v.y}; };
+
+    __shared__ C s_uf[kMaxLinePar], s_gf[kMaxLinePar], s_hf[kMaxLinePar];
+    __shared__ C s_A[kMaxLinePar], s_B[kMaxLinePar], s_C[kMaxLinePar], s_D[kMaxLinePar], s_X[kMaxLinePar];
+
+    const int p = threadIdx.x;
+    const bool active = p < P;
+    const int base = M / P, rem = M % P;
+    const int Lp = base + (p < rem ? 1 : 0);
+    const int s = p * base + (p < rem ? p : rem);
+    const int n = Lp - 1;  // interior unknowns of the chunk (>= 1)
+
+    C gam[LMAX], yu[LMAX], yg[LMAX];
+    C binv{0, 0}, cprev{0, 0};
+    C ul{0, 0}, gl{0, 0}, hl{0, 0}, uf{0, 0}, gf{0, 0}, hf{0, 0};
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < LMAX; i++) {
+            if (i < n) {
+                const int j = s + i;
+                const C a = coef_a(j), b = coef_b(j), c = coef_c(j), d = rhs(j);
+                if (i == 0) {
+                    const C bi = crcp(b);
+                    yu[0] = d * bi;
+                    yg[0] = -(a * bi);
+                    binv = bi;
+                } else {
+                    const C gm = cprev * binv;
+                    gam[i - 1] = gm;
+                    const C bi = crcp(b - a * gm);
+                    yu[i] = (d - a * yu[i - 1]) * bi;
+                    yg[i] = -(a * yg[i - 1]) * bi;
+                    binv = bi;
+                }
+                cprev = c;
+            }
+        }
+        hl = -(cprev * binv);
+        C u{0, 0}, gg{0, 0}, h{0, 0};
+#pragma unroll
+        for (int k = 0; k < LMAX; k++) {
+            const int i = LMAX - 1 - k;
+            if (i == n - 1) {
+                u = yu[i];
+                gg = yg[i];
+                h = hl;
+                ul = u;
+                gl = gg;
+            } else if (i < n - 1) {
+                u = yu[i] - gam[i] * u;
+                gg = yg[i] - gam[i] * gg;
+                h = -(gam[i] * h);
+            }
+        }
+        uf = u;
+        gf = gg;
+        hf = h;
+        s_uf[p] = uf;
+        s_gf[p] = gf;
+        s_hf[p] = hf;
+    }
+    __syncthreads();
+    C xi{0, 0};
+    if (active) {
+        const int e = s + n;
+        const int nx = (p + 1) & (P - 1);
+        const C ae = coef_a(e), be = coef_b(e), ce = coef_c(e), de = rhs(e);
+        s_A[p] = ae * gl;
+        s_B[p] = be + ae * hl + ce * s_gf[nx];
+        s_C[p] = ce * s_hf[nx];
+        s_D[p] = de - ae * ul - ce * s_uf[nx];
+    }
+    __syncthreads();
+    for (int st = 1; 2 * st < P; st <<= 1) {
+        C nA{0, 0}, nB{0, 0}, nC{0, 0}, nD{0, 0};
+        if (active) {
+            const int im = (p - st) & (P - 1), ipp = (p + st) & (P - 1);
+            const C k1 = cdiv(s_A[p], s_B[im]), k2 = cdiv(s_C[p], s_B[ipp]);
+            nA = -(k1 * s_A[im]);
+            nC = -(k2 * s_C[ipp]);
+            nB = s_B[p] - k1 * s_C[im] - k2 * s_A[ipp];
+            nD = s_D[p] - k1 * s_D[im] - k2 * s_D[ipp];
+        }
+        __syncthreads();
+        if (active) {
+            s_A[p] = nA;
+            s_B[p] = nB;
+            s_C[p] = nC;
+            s_D[p] = nD;
+        }
+        __syncthreads();
+    }
+    if (active) {
+        if (P == 1) {
+            xi = cdiv(s_D[0], s_A[0] + s_B[0] + s_C[0]);
+        } else {
+            const int q = (p + P / 2) & (P - 1);
+            const C apc = s_A[p] + s_C[p], aqc = s_A[q] + s_C[q];
+            xi = cdiv(s_D[p] * s_B[q] - apc * s_D[q], s_B[p] * s_B[q] - apc * aqc);
+        }
+        s_X[p] = xi;
+    }
+    __syncthreads();
+    if (active) {
+        const C xprev = s_X[(p - 1) & (P - 1)];
+        C u{0, 0}, gg{0, 0}, h{0, 0};
+#pragma unroll
+        for (int k = 0; k < LMAX; k++) {
+            const int i = LMAX - 1 - k;
+            bool live = false;
+            if (i == n - 1) {
+                u = yu[i];
+                gg = yg[i];
+                h = hl;
+                live = true;
+            } else if (i < n - 1) {
+                u = yu[i] - gam[i] * u;
+                gg = yg[i] - gam[i] * gg;
+                h = -(gam[i] * h);
+                live = true;
+            }
+            if (live) {
+                const C x = u + xprev * gg + xi * h;
+                const int64_t q = pt(s + i);
+                d_out[q] = x;
+                if (NEU) a_out[q] = a_out[q] + x;
+                else a_out[q] = x;
+            }
+        }
+        const int64_t qe = pt(s + n);
+        d_out[qe] = xi;
+        if (NEU) a_out[qe] = a_out[qe] + xi;
+        else a_out[qe] = xi;
+    }
+}
+
+template <class T, int DIR, int NEU>
+static void line_solve_dispatch(int lmax, dim3 grid, int bs, const C2<T> *src, C2<T> *dst, C2<T> *acc,
+                                const C2<T> *cf, const Geo &g, const PropParams *pp, int P, hipStream_t st) {
+    switch (lmax) {
+        case 4: hipLaunchKernelGGL((k_line_solve<T, 4, DIR, NEU>), grid, dim3(bs), 0, st, src, dst, acc, cf, g, pp, P); break;
+        case 8: hipLaunchKernelGGL((k_line_solve<T, 8, DIR, NEU>), grid, dim3(bs), 0, st, src, dst, acc, cf, g, pp, P); break;
+        case 16: hipLaunchKernelGGL((k_line_solve<T, 16, DIR, NEU>), grid, dim3(bs), 0, st, src, dst, acc, cf, g, pp, P); break;
+        case 32: hipLaunchKernelGGL((k_line_solve<T, 32, DIR, NEU>), grid, dim3(bs), 0, st, src, dst, acc, cf, g, pp, P); break;
+        default: throw std::runtime_error("line solver: unsupported chunk length");
+    }
+    NMK_LAUNCH_CHECK();
+}
+
+// chunking of a periodic line of M unknowns: P (power of two) chunks of at
+// least 2 points, at most kMaxLinePar chunks, chunk length bound LMAX
+static void line_plan(int M, int &P, int &lmax, int &bs) {
+    int cap = std::min(kMaxLinePar, std::max(1, M / 2));
+    int pw = 1;
+    while (pw * 2 <= cap) pw *= 2;
+    // prefer one wavefront per line while chunks stay short
+    P = std::min(pw, 64);
+    if ((M + P - 1) / P > 16) P = pw;
+    int need = (M + P - 1) / P;
+    lmax = need <= 4 ? 4 : need <= 8 ? 8 : need <= 16 ? 16 : need <= 32 ? 32 : 0;
+    if (lmax == 0) throw std::runtime_error("line solver: mesh dimension too large (max 8192 points per axis)");
+    bs = P <= 64 ? 64 : P;
+}
+
+template <class T>
+void launch_line_solve(int dir, int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf,
+                       const Geo &g, const PropParams *pp, int nprops, hipStream_t st) {
+    const int M = dir == 0 ? g.Mx : g.My;
+    const int lines = dir == 0 ? g.My : g.Mx;
+    int P, lmax, bs;
+    line_plan(M, P, lmax, bs);
+    dim3 grid(lines, 2, nprops);
+    if (dir == 0) {
+        if (neumann) line_solve_dispatch<T, 0, 1>(lmax, grid, bs, src, dst, acc, cf, g, pp, P, st);
+        else line_solve_dispatch<T, 0, 0>(lmax, grid, bs, src, dst, acc, cf, g, pp, P, st);
+    } else {
+        if (neumann) line_solve_dispatch<T, 1, 1>(lmax, grid, bs, src, dst, acc, cf, g, pp, P, st);
+        else line_solve_dispatch<T, 1, 0>(lmax, grid, bs, src, dst, acc, cf, g, pp, P, st);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// output: march grid -> full grid (duplicates mirror column/row 0), global
+// shift phase, output Fresnel with each point's own eps at the last plane
+// (BeamPropagationLib/FresnelOperator.cpp:45-69)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_output(const C2<T> *__restrict__ F, const double *__restrict__ lc,
+                                                const uint8_t *__restrict__ mask, Geo g,
+                                                const SlotParams *__restrict__ sp, const PropParams *__restrict__ pp,
+                                                c64 *__restrict__ E) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.Nfull) return;
+    const int prop = blockIdx.y;
+    const PropParams p = pp[prop];
+    const SlotParams s = sp[p.slot];
+    const int ix = (int)(m % g.Nx), iy = (int)(m / g.Nx);
+    const int jx = g.px ? ix % g.Mx : ix, jy = g.py ? iy % g.My : iy;
+    const int64_t ip = jx + (int64_t)g.Mx * jy;
+    const c64 ph = mk(p.ph_re, p.ph_im);
+    const c64 a = ph * cast2<double>(F[((int64_t)prop * 2) * g.NP + ip]);
+    const c64 b = ph * cast2<double>(F[((int64_t)prop * 2 + 1) * g.NP + ip]);
+    double e[12];
+    eps_voxel(lc, mask, g.field_dim, m + g.Nfull * (g.Nz - 1), s, e);
+    const double n = s.nout_fresnel, nxx = e[9], nyy = e[10], nxy = e[11];
+    const double det = (n + nxx) * (n + nyy) - nxy * nxy;
+    const double cxx = 1. - n * (n + nyy) / det, cyy = 1. - n * (n + nxx) / det, cxy = n * nxy / det;
+    E[((int64_t)2 * prop) * g.Nfull + m] = 2.0 * (cxx * a + cxy * b);
+    E[((int64_t)2 * prop + 1) * g.Nfull + m] = 2.0 * (cyy * b + cxy * a);
+}
+
+template <class T>
+void launch_output(const C2<T> *F, const double *lc, const uint8_t *mask, const Geo &g, const SlotParams *sp,
+                   const PropParams *pp, int nprops, int nq, c64 *E, hipStream_t st) {
+    (void)nq;
+    hipLaunchKernelGGL(k_output<T>, dim3(blocks_for(g.Nfull, 256), nprops), dim3(256), 0, st, F, lc, mask, g, sp, pp, E);
+    NMK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// screen filter FFT: mixed-radix Stockham autosort transform of one line per
+// workgroup, ping-ponging between two LDS buffers.  Stage with radix r after
+// sub-transforms of length L:
+//   y'[j + L p + L r k] = sum_q w_r^{pq} ( w_{Lr}^{jq} y[j + L k + (n/r) q] )
+// ---------------------------------------------------------------------------
+constexpr int kFftMaxRadix = 13;
+
+bool fft_supported(int n) {
+    if (n < 1) return false;
+    int m = n;
+    for (int r : {4, 2, 3, 5, 7, 11, 13})
+        while (m % r == 0) m /= r;
+    return m == 1;
+}
+
+FftPlan make_fft_plan(int n) {
+    FftPlan pl{};
+    pl.n = n;
+    int m = n;
+    pl.nst = 0;
+    for (int r : {4, 2, 3, 5, 7, 11, 13})
+        while (m % r == 0 && m > 1) {
+            if (pl.nst >= 24) throw std::runtime_error("FFT plan too deep");
+            pl.radix[pl.nst++] = r;
+            m /= r;
+        }
+    if (m != 1) throw std::runtime_error("FFT length " + std::to_string(n) + " has a prime factor above 13");
+    return pl;
+}
+
+__global__ __launch_bounds__(256) void k_fft_lines(c64 *__restrict__ data, int64_t line_stride, int64_t elem_stride,
+                                                   int64_t plane_stride, FftPlan pl, const c64 *__restrict__ tw,
+                                                   int sign) {
+    extern __shared__ c64 fbuf[];
+    const int n = pl.n;
+    c64 *b0 = fbuf, *b1 = fbuf + n;
+    const int64_t base = (int64_t)blockIdx.y * plane_stride + (int64_t)blockIdx.x * line_stride;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) b0[i] = data[base + (int64_t)i * elem_stride];
+    __syncthreads();
+    auto W = [&](int idx) -> c64 {
+        c64 w = tw[idx];
+        return sign > 0 ? conj(w) : w;
+    };
+    int L = 1;
+    for (int st = 0; st < pl.nst; st++) {
+        const int r = pl.radix[st];
+        const int Lr = L * r, nb = n / r, tstep = n / Lr, rstep = n / r;
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+            const int j = b % L, k = b / L;
+            c64 u[kFftMaxRadix];
+#pragma unroll
+            for (int q = 0; q < kFftMaxRadix; q++)
+                if (q < r) {
+                    c64 v = b0[j + L * k + rstep * q];
+                    u[q] = (q == 0 || j == 0) ? v : v * W(j * q * tstep);
+                }
+            if (r == 2) {
+                b1[j + L * 0 + Lr * k] = u[0] + u[1];
+                b1[j + L * 1 + Lr * k] = u[0] - u[1];
+            } else if (r == 4) {
+                // w_4 = -i (forward) or +i (backward)
+                const c64 t0 = u[0] + u[2], t1 = u[0] - u[2], t2 = u[1] + u[3], d = u[1] - u[3];
+                const c64 t3 = sign > 0 ? mk(-d.y, d.x) : mk(d.y, -d.x);
+                b1[j + L * 0 + Lr * k] = t0 + t2;
+                b1[j + L * 1 + Lr * k] = t1 + t3;
+                b1[j + L * 2 + Lr * k] = t0 - t2;
+                b1[j + L * 3 + Lr * k] = t1 - t3;
+            } else {
+#pragma unroll
+                for (int pq = 0; pq < kFftMaxRadix; pq++)
+                    if (pq < r) {
+                        c64 a = u[0];
+#pragma unroll
+                        for (int q = 1; q < kFftMaxRadix; q++)
+                            if (q < r) a = cmad(a, u[q], W(((pq * q) % r) * rstep));
+                        b1[j + L * pq + Lr * k] = a;
+                    }
+            }
+        }
+        __syncthreads();
+        c64 *t = b0;
+        b0 = b1;
+        b1 = t;
+        L = Lr;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) data[base + (int64_t)i * elem_stride] = b0[i];
+}
+
+void launch_fft2(c64 *data, int Nx, int Ny, int planes, int sign, const c64 *tw_x, const c64 *tw_y,
+                 const FftPlan &px, const FftPlan &py, hipStream_t st) {
+    const int64_t plane = (int64_t)Nx * Ny;
+    if (Nx > 1) {
+        hipLaunchKernelGGL(k_fft_lines, dim3(Ny, planes), dim3(256), 2 * Nx * sizeof(c64), st, data, (int64_t)Nx,
+                           (int64_t)1, plane, px, tw_x, sign);
+        NMK_LAUNCH_CHECK();
+    }
+    if (Ny > 1) {
+        hipLaunchKernelGGL(k_fft_lines, dim3(Nx, planes), dim3(256), 2 * Ny * sizeof(c64), st, data, (int64_t)1,
+                           (int64_t)Nx, plane, py, tw_y, sign);
+        NMK_LAUNCH_CHECK();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-k 2x2 transfer filter (PostprocessorLib/ScreenOutput.cpp:226-302) and
+// the 1/(Nx*Ny) normalisation, applied to both polarisations (:206-219)
+// ---------------------------------------------------------------------------
+constexpr int kMaxIsoLayers = 16;
+
+__global__ __launch_bounds__(256) void k_filter(c64 *__restrict__ E, Geo g, const SlotParams *__restrict__ sp,
+                                                const double *__restrict__ qxy, int nq,
+                                                const double *__restrict__ iso_thick,
+                                                const double *__restrict__ iso_index, int n_iso, double dx, double dy,
+                                                double z_foc, double NA) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.Nfull) return;
+    const int slot = blockIdx.y / nq, iq = blockIdx.y % nq;
+    const SlotParams s = sp[slot];
+    const double pi = 3.1415926535897932;
+    const int ix = (int)(m % g.Nx), iy = (int)(m / g.Nx);
+    const int Nx = g.Nx, Ny = g.Ny;
+    const double k0 = s.k0;
+    const double kx = (ix < double(Nx / 2.)) ? 2 * pi * ix / (dx * Nx) : -2 * pi * (Nx - ix) / (dx * Nx);
+    const double ky = (iy < double(Ny / 2.)) ? 2 * pi * iy / (dy * Ny) : -2 * pi * (Ny - iy) / (dy * Ny);
+    const double qx0 = qxy[2 * iq], qy0 = qxy[2 * iq + 1];
+    const double q0 = kx / k0 + qx0, q1 = ky / k0 + qy0;
+    const double qn = sqrt(q0 * q0 + q1 * q1);
+    const double qns = qn * qn;
+    const double q0n = sqrt(qx0 * qx0 + qy0 * qy0);  // |q0|, as the reference (:252)
+    c64 F[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    if (qn < NA) {
+        c64 T[4][4];
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) T[i][j] = mk(i == j ? 1.0 : 0.0, 0.0);
+        for (int l = 0; l < n_iso; l++) {
+            const double np = iso_index[l];
+            const double nn = (l + 1 < n_iso) ? iso_index[l + 1] : s.nout_filter;
+            const double neff = sqrt(np * np - qns), nneff = sqrt(nn * nn - qns);
+            const double im_[2][2] = {{neff + q0 * q0 / neff, q0 * q1 / neff}, {q0 * q1 / neff, neff + q1 * q1 / neff}};
+            const double ni[2][2] = {{(1 - (q0 / nn) * (q0 / nn)) / nneff, -q0 * q1 / (nn * nn * nneff)},
+                                     {-q0 * q1 / (nn * nn * nneff), (1 - (q1 / nn) * (q1 / nn)) / nneff}};
+            double tp[2][2], tm[2][2];
+            for (int i = 0; i < 2; i++)
+                for (int j = 0; j < 2; j++) {
+                    const double prod = ni[i][0] * im_[0][j] + ni[i][1] * im_[1][j];
+                    tp[i][j] = 0.5 * ((i == j ? 1. : 0.) + prod);
+                    tm[i][j] = (i == j ? 1. : 0.) - tp[i][j];
+                }
+            const double phase = (neff - sqrt(np * np - q0n)) * k0 * iso_thick[l];
+            const c64 gph = mk(cos(phase), sin(phase)), gc = conj(gph);
+            c64 Lm[4][4];
+            for (int i = 0; i < 2; i++)
+                for (int j = 0; j < 2; j++) {
+                    Lm[i][j] = tp[i][j] * gph;
+                    Lm[i][j + 2] = tm[i][j] * gc;
+                    Lm[i + 2][j] = tm[i][j] * gph;
+                    Lm[i + 2][j + 2] = tp[i][j] * gc;
+                }
+            c64 Tn[4][4];
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                    c64 a = mk(0.0, 0.0);
+                    for (int k = 0; k < 4; k++) a = cmad(a, Lm[i][k], T[k][j]);
+                    Tn[i][j] = a;
+                }
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) T[i][j] = Tn[i][j];
+        }
+        const c64 det = T[2][2] * T[3][3] - T[2][3] * T[3][2];
+        const c64 id = crcp(det);
+        const c64 inv[2][2] = {{T[3][3] * id, -(T[2][3] * id)}, {-(T[3][2] * id), T[2][2] * id}};
+        const double nout = s.nout_filter;
+        const double fz = (sqrt(nout * nout - qns) - sqrt(nout * nout - q0n)) * k0 * z_foc;
+        const c64 ph = mk(cos(fz), sin(fz));
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++) {
+                c64 a = T[i][j];
+                for (int k = 0; k < 2; k++)
+                    for (int l2 = 0; l2 < 2; l2++) a = a - T[i][k + 2] * inv[k][l2] * T[l2 + 2][j];
+                F[2 * i + j] = a * ph;
+            }
+    }
+    const double norm = double(Nx * (int64_t)Ny);
+    for (int pol = 0; pol < 2; pol++) {
+        c64 *e0 = E + (((int64_t)(slot * nq + iq) * 2 + pol) * 2) * g.Nfull + m;
+        c64 *e1 = e0 + g.Nfull;
+        const c64 x = *e0, y = *e1;
+        const c64 a = F[0] * x + F[1] * y, b = F[2] * x + F[3] * y;
+        *e0 = mk(a.x / norm, a.y / norm);
+        *e1 = mk(b.x / norm, b.y / norm);
+    }
+}
+
+void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const double *qxy, int nq,
+                   const double *iso_thick, const double *iso_index, int n_iso, double dx, double dy, double z_foc,
+                   double NA, hipStream_t st) {
+    if (n_iso > kMaxIsoLayers) throw std::runtime_error("too many isotropic layers (max 16)");
+    hipLaunchKernelGGL(k_filter, dim3(blocks_for(g.Nfull, 256), nslots * nq), dim3(256), 0, st, E, g, sp, qxy, nq,
+                       iso_thick, iso_index, n_iso, dx, dy, z_foc, NA);
+    NMK_LAUNCH_CHECK();
+}
+
+// explicit instantiations
+#define NMK_INST(T)                                                                                                   \
+    template void launch_coefs<T>(const double *, const Geo &, const SlotParams *, int, C2<T> *, hipStream_t);       \
+    template void launch_init_fields<T>(const double *, const uint8_t *, const Geo &, const SlotParams *,             \
+                                        const PropParams *, int, int, double, double, double, C2<T> *, hipStream_t);  \
+    template void launch_pack_fields<T>(const c64 *, const Geo &, const PropParams *, int, int, C2<T> *, hipStream_t); \
+    template void launch_shift<T>(const C2<T> *, C2<T> *, const Geo &, const PropParams *, int, hipStream_t);        \
+    template void launch_phase<T>(C2<T> *, const C2<T> *, const Geo &, const PropParams *, int, hipStream_t);        \
+    template void launch_explicit<T>(int, const C2<T> *, C2<T> *, const C2<T> *, const Geo &, const PropParams *,    \
+                                     int, hipStream_t);                                                               \
+    template void launch_line_solve<T>(int, int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const Geo &,        \
+                                       const PropParams *, int, hipStream_t);                                        \
+    template void launch_output<T>(const C2<T> *, const double *, const uint8_t *, const Geo &, const SlotParams *,  \
+                                   const PropParams *, int, int, c64 *, hipStream_t);
+NMK_INST(float)
+NMK_INST(double)
+
+}  // namespace nmk

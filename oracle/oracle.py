@@ -84,13 +84,14 @@ def _dp(a):
 # arithmetic evaluator for index expressions (subset of muParser syntax)
 # ---------------------------------------------------------------------------
 _BIN = {ast.Add: operator.add, ast.Sub: operator.sub, ast.Mult: operator.mul,
-        ast.Div: operator.truediv, ast.Pow: operator.pow, ast.BitXor: operator.pow}
+        ast.Div: operator.truediv, ast.Pow: operator.pow}
 _FUN = {"sqrt": math.sqrt, "exp": math.exp, "sin": math.sin, "cos": math.cos, "tan": math.tan,
         "log": math.log, "ln": math.log, "log10": math.log10, "abs": abs}
 
 
 def eval_expr(expr, **variables):
-    node = ast.parse(str(expr).replace("lambda", "lam_"), mode="eval").body
+    # muParser's ^ is exponentiation binding tighter than unary minus: Python's **
+    node = ast.parse(str(expr).replace("lambda", "lam_").replace("^", "**"), mode="eval").body
     env = {("lam_" if k == "lambda" else k): v for k, v in variables.items()}
 
     def ev(n):

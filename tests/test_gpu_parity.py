@@ -1,0 +1,201 @@
+"""GPU parity of the HIP path against the FP64 CPU oracle, through the C ABI.
+
+Tolerances (relative L2 per output (wavelength, wavevector, polarisation)):
+  FP64 path: 1e-10      FP32 path: 1e-4
+Batched (all wavelengths at once) and hipGraph-replayed runs must equal the
+plain run bit for bit.
+"""
+import json
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from nemaktis_amd import bpm_backend, fields
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+TOL32 = 1e-4
+
+
+def _gpu(s, lc, mask=None, formula="", E_in=None, **kw):
+    print("[gpu case]", s["Physics settings"]["Initial conditions"]["Mesh dimensions"], kw, flush=True)
+    return bpm_backend.propagate(json.dumps(s), lc, mask_vals=mask, mask_formula=formula, E_in=E_in, **kw)
+
+
+def _rel(a, b):
+    """max over outputs of the relative L2 error of a against b"""
+    a = a.reshape(a.shape[0], a.shape[1], 2, -1)
+    b = b.reshape(b.shape[0], b.shape[1], 2, -1)
+    num = np.linalg.norm(a - b, axis=-1)
+    den = np.linalg.norm(b, axis=-1)
+    return float((num / den).max())
+
+
+def _formula_weights(s, formula_fn, mask):
+    """Interface weights by bisection (UtilitiesLib/VectorField.cpp:64-113)."""
+    Nx, Ny, Nz = s["Physics settings"]["Initial conditions"]["Mesh dimensions"]
+    dx, dy, dz = s["Physics settings"]["Initial conditions"]["Mesh spacings"]
+    w = np.full((Nz - 1, Ny, Nx, 2), 0.5)
+    m = (mask >= 0)
+    failed = False
+    for iy in range(Ny):
+        for ix in range(Nx):
+            x, y = (ix - 0.5 * (Nx - 1)) * dx, (iy - 0.5 * (Ny - 1)) * dy
+            f = lambda z: formula_fn(x, y, z)
+            for iz in range(Nz - 1):
+                zi = (iz - 0.5 * (Nz - 1)) * dz
+                zm, zp = zi, (iz + 1 - 0.5 * (Nz - 1)) * dz
+                if f(zm) * f(zp) <= 0:
+                    if m[iz, iy, ix] == m[iz + 1, iy, ix]:
+                        failed = True
+                    z = zm
+                    while abs(zp - zm) >= 1e-3 * dz:
+                        z = zm
+                        fm = f(z)
+                        z = 0.5 * (zp + zm)
+                        fz = f(z)
+                        if fz == 0:
+                            break
+                        if fz * fm < 0:
+                            zp = z
+                        else:
+                            zm = z
+                    w[iz, iy, ix, 0] = (z - zi) / dz
+                    w[iz, iy, ix, 1] = 1 - w[iz, iy, ix, 0]
+    return None if failed else w
+
+
+def _base(dims, sp=(0.1, 0.1, 0.1), **kw):
+    return fields.make_settings(dims, sp, kw.pop("wl", [0.6]), **kw)
+
+
+def test_twisted_nematic_fp64():
+    s = _base((16, 16, 6), no="1.5", ne="1.7", nhost="1.5")
+    lc = fields.twisted_nematic(16, 16, 6)
+    err = _rel(_gpu(s, lc), oracle.propagate(s, lc))
+    assert err < TOL64, err
+
+
+def _random_case(dims=(20, 18, 6), seed=3):
+    s = _base(dims, wl=[0.55, 0.62], wavevectors=[(0.0, 0.0), (0.15, -0.1)], no="1.5",
+              ne="1.7+0.01/lambda^2", ne_imag="0.01", nin="1.2", nout="1.1", na_objective=0.7)
+    lc = fields.random_smooth(dims[0], dims[1], dims[2], (0.1, 0.1, 0.1), seed=seed, corr_length=0.3)
+    return s, lc
+
+
+def test_random_director_walkoff_shift_fp64():
+    s, lc = _random_case()
+    err = _rel(_gpu(s, lc), oracle.propagate(s, lc))
+    assert err < TOL64, err
+
+
+@pytest.mark.parametrize("nw", [0, 1, 4])
+def test_neumann_depth_fp64(nw):
+    s, lc = _random_case((14, 12, 4))
+    s["Algorithm settings"]["Beam propagation"]["N Woodbury steps"] = nw
+    err = _rel(_gpu(s, lc), oracle.propagate(s, lc))
+    assert err < TOL64, err
+
+
+def test_droplet_mask_gaussian_beam_fp64():
+    dims, sp = (24, 22, 8), (0.25, 0.25, 0.25)
+    lc, mask, _ = fields.droplet(*dims, sp, 2.2)
+    s = _base(dims, sp, wavevectors=[(0.0, 0.0), (0.1, 0.05)], no="1.5", ne="1.7", nhost="1.51",
+              beam_profile="GaussianBeam(1.5)", na_objective=0.5)
+    err = _rel(_gpu(s, lc, mask=mask), oracle.propagate(s, lc, mask=mask))
+    assert err < TOL64, err
+
+
+def test_droplet_mask_formula_weights_fp64():
+    dims, sp = (20, 20, 9), (0.25, 0.25, 0.25)
+    R = 1.9
+    lc, mask, formula = fields.droplet(*dims, sp, R)
+    s = _base(dims, sp, no="1.5", ne="1.7", nhost="1.51", na_objective=0.5)
+    w = _formula_weights(s, lambda x, y, z: R * R - x * x - y * y - z * z, mask)
+    assert w is not None
+    want = oracle.propagate(s, lc, mask=mask, weights=w)
+    err = _rel(_gpu(s, lc, mask=mask, formula=formula), want)
+    assert err < TOL64, err
+    # the weights matter: without them the result differs
+    assert _rel(oracle.propagate(s, lc, mask=mask), want) > 1e-6
+
+
+def test_qtensor_field_fp64():
+    dims = (16, 14, 5)
+    n = fields.random_smooth(*dims, (0.1, 0.1, 0.1), seed=9, corr_length=0.3)
+    Q = 0.6 * (np.einsum("...i,...j->...ij", n, n) - np.eye(3) / 3)
+    lc = np.stack([Q[..., 0, 0], Q[..., 1, 1], Q[..., 2, 2], Q[..., 0, 1], Q[..., 0, 2], Q[..., 1, 2]], -1)
+    s = _base(dims, no="1.5", ne="1.7", field_type="Q-tensor", wavevectors=[(0.05, 0.1)])
+    err = _rel(_gpu(s, lc), oracle.propagate(s, lc))
+    assert err < TOL64, err
+
+
+def test_iso_layers_focus_and_caller_fields_fp64():
+    s, lc = _random_case((16, 16, 5), seed=5)
+    s["Postprocessor settings"]["Screen output"].update({
+        "Isotropic layer thicknesses": [1.0, 0.5], "Isotropic layer refractive indices": [1.45, 1.3],
+        "Focalisation z-shift": 0.7})
+    s["Physics settings"]["Initial conditions"]["Beam profile"] = "None"
+    rng = np.random.default_rng(1)
+    E_in = rng.standard_normal(fields.output_shape(s)) + 1j * rng.standard_normal(fields.output_shape(s))
+    err = _rel(_gpu(s, lc, E_in=E_in), oracle.propagate(s, lc, E=E_in))
+    assert err < TOL64, err
+
+
+@pytest.mark.parametrize("dims", [(15, 12, 5), (3, 20, 4), (18, 1, 4), (21, 10, 3)])
+def test_odd_and_degenerate_meshes_fp64(dims):
+    s = _base(dims, wavevectors=[(0.08, -0.12)], no="1.5", ne="1.7", na_objective=0.6)
+    lc = fields.random_smooth(*dims, (0.1, 0.1, 0.1), seed=11, corr_length=0.3)
+    err = _rel(_gpu(s, lc), oracle.propagate(s, lc))
+    assert err < TOL64, err
+
+
+def test_full_twisted_nematic_case_fp64():
+    s, lc, _, _ = fields.workload("tn64")
+    err = _rel(_gpu(s, lc), oracle.propagate(s, lc))
+    assert err < TOL64, err
+
+
+def test_fp32_path_error_bound():
+    s, lc = _random_case((48, 40, 12), seed=7)
+    want = oracle.propagate(s, lc)
+    err = _rel(_gpu(s, lc, precision=32), want)
+    print("fp32 rel L2", err)
+    assert err < TOL32
+
+
+def test_batched_and_graph_equal_plain_bitwise():
+    s, lc = _random_case((18, 16, 5), seed=13)
+    plain = _gpu(s, lc)
+    batched = _gpu(s, lc, wl_batch=0)
+    assert np.array_equal(plain, batched)
+    js = json.dumps(s)
+    n_E = int(np.prod(fields.output_shape(s)))
+    with bpm_backend.Context(js, lc, n_E, wl_batch=0, use_graph=True) as ctx:
+        for _ in range(3):
+            ctx.propagate()
+            out = ctx.download(np.empty(fields.output_shape(s), np.complex128))
+            assert np.array_equal(out, plain)
+    with bpm_backend.Context(js, lc, n_E, precision=32, wl_batch=0, use_graph=True) as ctx:
+        outs = []
+        for _ in range(3):
+            ctx.propagate()
+            outs.append(ctx.download(np.empty(fields.output_shape(s), np.complex128)))
+        assert all(np.array_equal(o, outs[0]) for o in outs)
+        err = _rel(outs[0], plain)
+    assert err < TOL32, err
+
+
+def test_random512_reduced_depth_fp32_vs_fp64():
+    """512x512 slabs (the benchmark's transverse size, 8 slabs deep): FP32 vs FP64."""
+    s, _, _, _ = fields.workload("tn64")
+    s = _base((512, 512, 9), no="1.52", ne="1.74", wl=[0.55], na_objective=0.4)
+    lc = fields.random_smooth(512, 512, 9, (0.1, 0.1, 0.1), seed=2, corr_length=1.0)
+    e64 = _gpu(s, lc)
+    e32 = _gpu(s, lc, precision=32)
+    assert np.isfinite(e64).all()
+    err = _rel(e32, e64)
+    assert err < TOL32, err
