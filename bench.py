@@ -1,0 +1,193 @@
+#!/usr/bin/env python
+"""Benchmark of the bpm-solver propagation step on one MI355X.
+
+Workload (default): 512x512x100 mesh, 0.1 um spacing, seeded smoothed random
+director (1 um correlation), no = 1.52, ne = 1.74, wavelength 0.55 um, normal
+incidence, 2 input polarisations.  One step = one full propagation of that
+job (permittivity and coefficient precompute, input/output Fresnel, the
+99-slab z-march and the FFT screen filter) with the director already
+resident in HBM.  `value` is the FP32 path (complex64 march, FP64
+coefficients, hipGraph-replayed z-march); the FP64 plain path (complex128
+march, one wavelength at a time, eager launches) is timed beside it, as is
+the FP64 CPU oracle on a bounded sample of the same workload.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FAMILIES = ["material", "coefs", "shift", "phase", "explicit", "solve_x", "solve_y", "output", "filter", "setup"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(fam, c, nprops, nslots, NP, Nfull, fd, nw):
+    """Unique HBM bytes one launch of a kernel family must move: every array
+    element it reads or writes counted once (c = bytes per complex value)."""
+    if fam == "shift":
+        return 4 * nprops * NP * c
+    if fam == "phase":
+        return 4 * nprops * NP * c + 3 * nslots * NP * c
+    if fam == "explicit":
+        return 4 * nprops * NP * c + 20 * nslots * NP * c
+    if fam in ("solve_x", "solve_y"):
+        init = (6 * nprops + 6 * nslots) * NP * c
+        neu = (8 * nprops + 20 * nslots) * NP * c
+        return (init + nw * neu) / (1 + nw)
+    if fam == "coefs":
+        return 15 * 8 * nslots * Nfull + 43 * nslots * NP * c
+    if fam == "material":
+        return 2 * fd * 8 * Nfull + 15 * 8 * nslots * Nfull
+    return None
+
+
+def rel_errors(a, b):
+    a = a.reshape(a.shape[0], a.shape[1], 2, -1)
+    b = b.reshape(b.shape[0], b.shape[1], 2, -1)
+    e = np.linalg.norm(a - b, axis=-1) / np.linalg.norm(b, axis=-1)
+    return e.reshape(-1)
+
+
+def run_gpu_leg(js, lc, n_E, shape, precision, wl_batch, use_graph, steps, warmup):
+    from nemaktis_amd import bpm_backend
+    with bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, use_graph=use_graph) as ctx:
+        for _ in range(warmup):
+            ctx.propagate()
+        dev_ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = ctx.propagate()  # returns once the device work of the step has completed
+            dev_ms.append(st["total_ms"])
+        wall = time.perf_counter() - t0
+        out = ctx.download(np.empty(shape, np.complex128))
+        return wall, dev_ms, st, out
+
+
+def run_profile_leg(js, lc, n_E, precision, wl_batch, steps):
+    from nemaktis_amd import bpm_backend
+    fam_ms = {f: 0.0 for f in FAMILIES}
+    fam_n = {f: 0 for f in FAMILIES}
+    with bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, profile=True) as ctx:
+        ctx.propagate()  # warm-up
+        for _ in range(steps):
+            st = ctx.propagate()
+            for f in FAMILIES:
+                fam_ms[f] += st["family_ms"][f]
+                fam_n[f] += st["family_launches"][f]
+    return fam_ms, fam_n
+
+
+def cpu_baseline(settings_fn, lc_full, nz_sample):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from nemaktis_amd import fields
+    s = settings_fn(nz_sample)
+    lc = np.ascontiguousarray(lc_full[:nz_sample])
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    oracle.propagate(s, lc, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return fields.updates_per_run(s) / dt, threads, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-nz", type=int, default=100, help="planes of the CPU oracle sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the bpm march shards by independent (wavelength, wavevector, "
+              "polarisation) propagations and this build runs on a single MI355X; --gpus %d is not supported"
+              % max(args.gpus, world), file=sys.stderr)
+        sys.exit(2)
+
+    from nemaktis_amd import _lib, fields
+    _lib.load()
+    Nx = Ny = 512
+    Nz = 100
+    sp = (0.1, 0.1, 0.1)
+    log("generating the seeded 512x512x100 director field ...")
+    lc = fields.random_smooth(Nx, Ny, Nz, sp, seed=2, corr_length=1.0)
+
+    def settings_for(nz):
+        return fields.make_settings((Nx, Ny, nz), sp, [0.55], no="1.52", ne="1.74", na_objective=0.4)
+
+    s = settings_for(Nz)
+    js = json.dumps(s)
+    shape = fields.output_shape(s)
+    n_E = int(np.prod(shape))
+    updates = fields.updates_per_run(s)
+
+    log("FP32 batched path (graph) ...")
+    wall32, dev32, st32, out32 = run_gpu_leg(js, lc, n_E, shape, 32, 0, True, args.steps, args.warmup)
+    ms32 = 1e3 * wall32 / args.steps
+    value = updates * args.steps / wall32
+
+    fp64 = None
+    if not args.no_fp64:
+        log("FP64 plain path ...")
+        k64 = max(1, args.steps // 2)
+        wall64, dev64, st64, out64 = run_gpu_leg(js, lc, n_E, shape, 64, 1, False, k64, 1)
+        errs = rel_errors(out32, out64)
+        fp64 = {"value": updates * k64 / wall64, "ms_per_step": 1e3 * wall64 / k64,
+                "fp32_speedup": (updates * args.steps / wall32) / (updates * k64 / wall64),
+                "fp32_rel_l2_max": float(errs.max()), "fp32_rel_l2_median": float(np.median(errs))}
+
+    log("profiled pass (kernel-family event timing) ...")
+    fam_ms, fam_n = run_profile_leg(js, lc, n_E, 32, 0, max(1, min(args.steps, 3)))
+    march_fams = [f for f in FAMILIES if f not in ("setup", "output", "filter") and fam_n[f]]
+    dom = max(march_fams, key=lambda f: fam_ms[f])
+    avg_ms = fam_ms[dom] / fam_n[dom]
+    Mx, My = Nx - 1, Ny - 1
+    algo = algorithmic_bytes(dom, 8, 2, 1, Mx * My, Nx * Ny, 3, 2)
+    achieved = algo / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "k_line_solve" if dom.startswith("solve") else "k_" + dom,
+                "family": dom, "avg_launch_ms": avg_ms, "launches": fam_n[dom],
+                "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "algorithmic_bytes_per_launch": algo,
+                "family_share": {f: round(fam_ms[f] / max(1e-9, sum(fam_ms.values())), 4) for f in FAMILIES}}
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU oracle sample ...")
+        v, threads, dt = cpu_baseline(settings_for, lc, args.cpu_nz)
+        cpu = {"value": v, "unit": "voxel-slice updates/s", "cores": threads, "kind": "port",
+               "sample": "FP64 oracle, 512x512x%d slice of the same director (%d slabs, 2 polarisations), "
+                         "%.1f s wall" % (args.cpu_nz, args.cpu_nz - 1, dt)}
+
+    line = {
+        "metric": "voxel-slice updates/s at 512x512x100 (FP32 path; FP64 plain path beside it)",
+        "value": value, "unit": "voxel-slice updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 (FP64 coefficient precompute)", "data": "synthetic",
+        "config": {"workload": "random-smooth director 512x512x100, dx=0.1um, lambda=0.55um, no=1.52, "
+                               "ne=1.74, normal incidence, 2 polarisations", "updates_per_step": updates,
+                   "device_ms_per_step_median": float(np.median(dev32))},
+        "fp64_plain": fp64,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["vs_cpu"] = value / cpu["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

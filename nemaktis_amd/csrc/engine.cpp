@@ -151,6 +151,10 @@ public:
             (void)hipGraphDestroy(kv.second.graph);
         }
         for (auto &e : evs_) (void)hipEventDestroy(e);
+        for (auto &pr : prof_pool_) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
         if (own_stream_) (void)hipStreamDestroy(st_);
     }
 
@@ -190,6 +194,7 @@ public:
             output_batch(w0, nslots, nprops, state);
             HIPCHK(hipEventRecord(d, st_));
             HIPCHK(hipEventSynchronize(d));
+            resolve_profile();
             float t;
             HIPCHK(hipEventElapsedTime(&t, a, b));
             setup_ms += t;
@@ -249,22 +254,38 @@ private:
         return evs_[i];
     }
 
-    // kernel-family timing in profile mode
+    // kernel-family timing in profile mode: an event pair around every launch,
+    // resolved once the propagation has finished (no host sync in between)
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pool_;
+    std::vector<int> prof_fam_;
     template <class F>
     void timed(int fam, F &&launch) {
         if (!o_.profile || !stats_) {
             launch();
             return;
         }
-        hipEvent_t a = ev(6), b = ev(7);
-        HIPCHK(hipEventRecord(a, st_));
+        const size_t k = prof_fam_.size();
+        while (prof_pool_.size() <= k) {
+            hipEvent_t a, b;
+            HIPCHK(hipEventCreate(&a));
+            HIPCHK(hipEventCreate(&b));
+            prof_pool_.emplace_back(a, b);
+        }
+        HIPCHK(hipEventRecord(prof_pool_[k].first, st_));
         launch();
-        HIPCHK(hipEventRecord(b, st_));
-        HIPCHK(hipEventSynchronize(b));
-        float t;
-        HIPCHK(hipEventElapsedTime(&t, a, b));
-        stats_->family_ms[fam] += t;
-        stats_->family_launches[fam] += 1;
+        HIPCHK(hipEventRecord(prof_pool_[k].second, st_));
+        prof_fam_.push_back(fam);
+    }
+    void resolve_profile() {
+        if (!stats_) return;
+        for (size_t k = 0; k < prof_fam_.size(); k++) {
+            HIPCHK(hipEventSynchronize(prof_pool_[k].second));
+            float t;
+            HIPCHK(hipEventElapsedTime(&t, prof_pool_[k].first, prof_pool_[k].second));
+            stats_->family_ms[prof_fam_[k]] += t;
+            stats_->family_launches[prof_fam_[k]] += 1;
+        }
+        prof_fam_.clear();
     }
 
     void setup_batch(int w0, int nslots, int nprops) {
