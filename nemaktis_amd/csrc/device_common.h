@@ -95,10 +95,35 @@ struct PropParams {
     int slot, iq, pol, pad;
     int ox, oy;           // first tap offset of the Catmull-Rom shift (floor(s)-1)
     double wx[4], wy[4];  // Catmull-Rom weights along x and y
+    int o1x, o1y, o2x, o2y;      // first tap of one shift / of two composed shifts
+    double w1x[7], w1y[7];       // one shift as a 7-tap filter (last 3 taps zero)
+    double w2x[7], w2y[7];       // two shifts composed: w * w
     double qx, qy;
     double ex, ey;        // input polarisation (cos, sin of pol*pi/2)
     double ph_re, ph_im;  // accumulated global phase of all shift half-steps
 };
+
+// chunking of one periodic line of M unknowns into P chunks (P a power of
+// two) of lengths base+1 (the first rem chunks) or base; lmax bounds the
+// chunk length and sets the register arrays of the chunk solver
+struct LinePlan {
+    int M, P, base, rem, lmax;
+};
+__host__ __device__ __forceinline__ int chunk_start(const LinePlan &lp, int p) {
+    return p * lp.base + (p < lp.rem ? p : lp.rem);
+}
+__host__ __device__ __forceinline__ int chunk_len(const LinePlan &lp, int p) { return lp.base + (p < lp.rem ? 1 : 0); }
+// chunk p and offset i of line element x
+__host__ __device__ __forceinline__ void chunk_of(const LinePlan &lp, int x, int &p, int &i) {
+    const int big = lp.rem * (lp.base + 1);
+    if (x < big) {
+        p = x / (lp.base + 1);
+        i = x - p * (lp.base + 1);
+    } else {
+        p = lp.rem + (x - big) / lp.base;
+        i = x - big - (p - lp.rem) * lp.base;
+    }
+}
 
 // coefficient plane slots (per interior point, stored as mu*R and P)
 enum : int {

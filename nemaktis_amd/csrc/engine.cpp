@@ -50,6 +50,16 @@ void cr_weights(double nu, double *w) {
     w[3] = -nu * nu * (1 - nu) / 2;
 }
 
+// one shift as a 7-tap filter and two successive shifts composed (w * w)
+void compose_taps(const double *w, int o, double *w1, int &o1, double *w2, int &o2) {
+    for (int t = 0; t < 7; t++) w1[t] = w2[t] = 0;
+    for (int t = 0; t < 4; t++) w1[t] = w[t];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) w2[i + j] += w[i] * w[j];
+    o1 = o;
+    o2 = 2 * o;
+}
+
 struct Sizes {
     int nwl, nq, batch, nslots_max, nprops_max;
     int64_t Nfull, NP;
@@ -69,6 +79,12 @@ Sizes sizes_of(const Problem &p, const Options &o) {
 }
 
 constexpr int kNrefBlocks = 1024;
+
+int pow2_at_least(int v) {
+    int p = 1;
+    while (p < v) p *= 2;
+    return p;
+}
 
 template <class T>
 class RunnerT final : public Runner {
@@ -139,7 +155,9 @@ public:
         pp_.alloc(sz_.nprops_max);
         partials_.alloc((size_t)kNrefBlocks * sz_.nslots_max);
         sm_.alloc((size_t)sz_.nslots_max * SM_COUNT * sz_.Nfull);
-        cf_.alloc((size_t)sz_.nslots_max * CF_COUNT * sz_.NP);
+        cf_.alloc(2 * (size_t)sz_.nslots_max * CF_COUNT * sz_.NP);
+        if (!o.profile) HIPCHK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
+        set_tuning(o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, g_.Mx, g_.My));
         for (auto &b : fb_) b.alloc((size_t)sz_.nprops_max * 2 * sz_.NP);
         E_.alloc((size_t)sz_.nwl * sz_.nq * 4 * sz_.Nfull);
         HIPCHK(hipStreamSynchronize(st_));
@@ -155,6 +173,8 @@ public:
             (void)hipEventDestroy(pr.first);
             (void)hipEventDestroy(pr.second);
         }
+        for (auto &x : sync_evs_) (void)hipEventDestroy(x);
+        if (st2_) (void)hipStreamDestroy(st2_);
         if (own_stream_) (void)hipStreamDestroy(st_);
     }
 
@@ -223,8 +243,9 @@ private:
     Options o_;
     Sizes sz_;
     Geo g_;
-    hipStream_t st_ = nullptr;
+    hipStream_t st_ = nullptr, st2_ = nullptr;  // march stream, coefficient stream
     bool own_stream_ = false;
+    std::vector<hipEvent_t> sync_evs_;
     FftPlan plan_x_, plan_y_;
     DevBuf<double> lc_, weights_, q_, iso_t_, iso_n_, partials_, sm_;
     DevBuf<uint8_t> mask_;
@@ -232,8 +253,10 @@ private:
     std::vector<c64> htwx_, htwy_;
     DevBuf<SlotParams> sp_;
     DevBuf<PropParams> pp_;
-    DevBuf<C2<T>> cf_;
+    DevBuf<C2<T>> cf_, xt_;
     DevBuf<C2<T>> fb_[4];
+    LinePlan lpx_{}, lpy_{};
+    Tuning tune_{};
     std::vector<SlotParams> hsp_;
     std::vector<PropParams> hpp_;
     std::vector<hipEvent_t> evs_;
@@ -352,6 +375,8 @@ private:
                     } else {
                         pp.wy[1] = 1;
                     }
+                    compose_taps(pp.wx, pp.ox, pp.w1x, pp.o1x, pp.w2x, pp.o2x);
+                    compose_taps(pp.wy, pp.oy, pp.w1y, pp.o1y, pp.w2y, pp.o2y);
                     const double phi = -sp.dZ * (qx * qx + qy * qy) / (4 * nref) * 2.0 * (p.Nz - 1);
                     pp.ph_re = std::cos(phi);
                     pp.ph_im = std::sin(phi);
@@ -397,49 +422,112 @@ private:
         return march_launches(nslots, nprops);
     }
 
-    C2<T> *march_launches(int nslots, int nprops) {
-        C2<T> *F = fb_[0].ptr, *T1 = fb_[1].ptr, *T2 = fb_[2].ptr, *T3 = fb_[3].ptr;
-        const int nw = p_.n_woodbury;
-        for (int iz = 0; iz < p_.Nz - 1; iz++) {
-            timed(KF_MATERIAL, [&] {
-                launch_slab_material(lc_.ptr, mask_.ptr, weights_.ptr, g_, sp_.ptr, nslots, iz, sm_.ptr, st_);
-            });
-            timed(KF_COEFS, [&] { launch_coefs<T>(sm_.ptr, g_, sp_.ptr, nslots, cf_.ptr, st_); });
-            // Sh, P
-            timed(KF_SHIFT, [&] { launch_shift<T>(F, T1, g_, pp_.ptr, nprops, st_); });
-            timed(KF_PHASE, [&] { launch_phase<T>(T1, cf_.ptr, g_, pp_.ptr, nprops, st_); });
-            // forward y half-step
-            timed(KF_EXPLICIT, [&] { launch_explicit<T>(1, T1, T2, cf_.ptr, g_, pp_.ptr, nprops, st_); });
-            // backward x half-step with its Neumann series
-            if (g_.px) {
-                timed(KF_SOLVE_X, [&] { launch_line_solve<T>(0, 0, T2, T1, F, cf_.ptr, g_, pp_.ptr, nprops, st_); });
-                C2<T> *a = T1, *b = T3;
-                for (int it = 0; it < nw; it++) {
-                    timed(KF_SOLVE_X, [&] { launch_line_solve<T>(0, 1, a, b, F, cf_.ptr, g_, pp_.ptr, nprops, st_); });
-                    std::swap(a, b);
-                }
-            } else {
-                std::swap(F, T2);
-            }
-            // forward x half-step
-            timed(KF_EXPLICIT, [&] { launch_explicit<T>(0, F, T2, cf_.ptr, g_, pp_.ptr, nprops, st_); });
-            // backward y half-step with its Neumann series
-            if (g_.py) {
-                timed(KF_SOLVE_Y, [&] { launch_line_solve<T>(1, 0, T2, T1, F, cf_.ptr, g_, pp_.ptr, nprops, st_); });
-                C2<T> *a = T1, *b = T3;
-                for (int it = 0; it < nw; it++) {
-                    timed(KF_SOLVE_Y, [&] { launch_line_solve<T>(1, 1, a, b, F, cf_.ptr, g_, pp_.ptr, nprops, st_); });
-                    std::swap(a, b);
-                }
-            } else {
-                std::swap(F, T2);
-            }
-            // P, Sh
-            timed(KF_PHASE, [&] { launch_phase<T>(F, cf_.ptr, g_, pp_.ptr, nprops, st_); });
-            timed(KF_SHIFT, [&] { launch_shift<T>(F, T1, g_, pp_.ptr, nprops, st_); });
-            std::swap(F, T1);
+    hipEvent_t sync_ev(size_t i) {
+        while (sync_evs_.size() <= i) {
+            hipEvent_t x;
+            HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+            sync_evs_.push_back(x);
         }
-        return F;
+        return sync_evs_[i];
+    }
+
+    // The z-march.  Coefficient planes of slab k+1 are computed on a second
+    // stream (double-buffered) while slab k marches; the slab boundary
+    // kernel joins the two streams.
+    C2<T> *march_launches(int nslots, int nprops) {
+        const int nsl = p_.Nz - 1, nw = p_.n_woodbury;
+        const size_t cfsz = (size_t)sz_.nslots_max * CF_COUNT * sz_.NP;
+        const size_t xtsz = xt_.count / 2;
+        auto CF = [&](int k) { return cf_.ptr + (size_t)(k % 2) * cfsz; };
+        auto XT = [&](int k) { return xt_.ptr ? xt_.ptr + (size_t)(k % 2) * xtsz : nullptr; };
+        const bool overlap = st2_ != nullptr;
+        hipStream_t cs = overlap ? st2_ : st_;
+        size_t evn = 0;
+        auto record = [&](hipStream_t s) {
+            hipEvent_t x = sync_ev(evn++);
+            HIPCHK(hipEventRecord(x, s));
+            return x;
+        };
+        auto coef = [&](int k) {
+            timed(KF_MATERIAL, [&] {
+                launch_slab_material(lc_.ptr, mask_.ptr, weights_.ptr, g_, sp_.ptr, nslots, k, sm_.ptr, cs);
+            });
+            timed(KF_COEFS, [&] { launch_coefs<T>(sm_.ptr, g_, sp_.ptr, nslots, CF(k), XT(k), lpx_, cs); });
+            return record(cs);
+        };
+        if (overlap) HIPCHK(hipStreamWaitEvent(cs, record(st_), 0));
+        std::vector<hipEvent_t> ready(nsl, nullptr);
+        ready[0] = coef(0);
+        if (nsl > 1) ready[1] = coef(1);
+
+        C2<T> *cur = fb_[1].ptr, *other = fb_[0].ptr, *A = fb_[2].ptr, *B = fb_[3].ptr;
+        HIPCHK(hipStreamWaitEvent(st_, ready[0], 0));
+        timed(KF_SHIFT, [&] { launch_boundary<T>(other, cur, nullptr, CF(0), g_, pp_.ptr, nprops, 1, st_); });
+        for (int k = 0; k < nsl; k++) {
+            // x half-steps: other = (I - mu A_x)^-1 (I + mu A_y) cur by its Neumann series
+            if (g_.px) {
+                timed(KF_SOLVE_X, [&] {
+                    launch_xsolve<T>(0, cur, A, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, st_);
+                });
+                C2<T> *a = A, *b = B;
+                for (int it = 0; it < nw; it++) {
+                    timed(KF_SOLVE_X, [&] {
+                        launch_xsolve<T>(1, a, b, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, st_);
+                    });
+                    std::swap(a, b);
+                }
+            } else {
+                timed(KF_EXPLICIT, [&] { launch_explicit<T>(1, cur, other, CF(k), g_, pp_.ptr, nprops, st_); });
+            }
+            // y half-steps: cur = (I - mu A_y)^-1 (I + mu A_x) other by its Neumann series
+            if (g_.py) {
+                timed(KF_SOLVE_Y, [&] {
+                    launch_ysolve<T>(0, other, A, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, st_);
+                });
+                C2<T> *a = A, *b = B;
+                for (int it = 0; it < nw; it++) {
+                    timed(KF_SOLVE_Y, [&] {
+                        launch_ysolve<T>(1, a, b, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, st_);
+                    });
+                    std::swap(a, b);
+                }
+            } else {
+                timed(KF_EXPLICIT, [&] { launch_explicit<T>(0, other, cur, CF(k), g_, pp_.ptr, nprops, st_); });
+            }
+            // P_k, Sh | Sh, P_{k+1}
+            if (k + 1 < nsl) {
+                HIPCHK(hipStreamWaitEvent(st_, ready[k + 1], 0));
+                timed(KF_SHIFT, [&] { launch_boundary<T>(cur, other, CF(k), CF(k + 1), g_, pp_.ptr, nprops, 2, st_); });
+            } else {
+                timed(KF_SHIFT, [&] { launch_boundary<T>(cur, other, CF(k), nullptr, g_, pp_.ptr, nprops, 1, st_); });
+            }
+            std::swap(cur, other);
+            if (k + 2 < nsl) {
+                // CF(k) and XT(k) are free once the boundary above has run
+                if (overlap) HIPCHK(hipStreamWaitEvent(cs, record(st_), 0));
+                ready[k + 2] = coef(k + 2);
+            }
+        }
+        if (overlap) HIPCHK(hipStreamWaitEvent(st_, record(cs), 0));
+        return cur;
+    }
+
+    void set_tuning(const Tuning &t) {
+        tune_ = t;
+        if (g_.px) lpx_ = make_line_plan(g_.Mx, t.x_chunks);
+        if (g_.py) lpy_ = make_line_plan(g_.My, t.y_chunks);
+        if (g_.px) {
+            if (lpx_.P * t.x_rows > 512 || xsolve_lds_bytes<T>(lpx_, t.x_rows) > 64 * 1024)
+                throw std::runtime_error("x line solver tuning exceeds the block limits");
+            xt_.alloc(2 * (size_t)sz_.nslots_max * 2 * 3 * g_.My * lpx_.lmax * lpx_.P);
+        }
+        if (g_.py && (lpy_.P * t.y_cols > 512 || ysolve_lds_bytes<T>(lpy_, t.y_cols) > 64 * 1024))
+            throw std::runtime_error("y line solver tuning exceeds the block limits");
+        for (auto &kv : graphs_) {
+            (void)hipGraphExecDestroy(kv.second.exec);
+            (void)hipGraphDestroy(kv.second.graph);
+        }
+        graphs_.clear();
     }
 
     void output_batch(int w0, int nslots, int nprops, const C2<T> *state) {
@@ -458,6 +546,25 @@ private:
 };
 
 }  // namespace
+
+Tuning default_tuning(int precision, int Mx, int My) {
+    Tuning t;
+    const size_t c = precision == 32 ? 8 : 16, budget = 64 * 1024;
+    auto lds = [&](int M, int chunks, int lines) {
+        LinePlan lp = make_line_plan(M, chunks);
+        return (size_t)lines * ((size_t)lp.lmax * lp.P * c + (size_t)lp.P * 7 * c);
+    };
+    // chunks of at most 8 points (16 for long lines)
+    t.x_chunks = std::max(64, pow2_at_least((Mx + 15) / 16));
+    t.x_rows = precision == 32 ? 4 : 2;
+    if (Mx > 3)
+        while (t.x_rows > 1 && (t.x_chunks * t.x_rows > 512 || lds(Mx, t.x_chunks, t.x_rows) > budget)) t.x_rows /= 2;
+    t.y_chunks = std::max(64, pow2_at_least((My + 15) / 16));
+    t.y_cols = precision == 32 ? 8 : 4;
+    if (My > 3)
+        while (t.y_cols > 1 && (t.y_chunks * t.y_cols > 512 || lds(My, t.y_chunks, t.y_cols) > budget)) t.y_cols /= 2;
+    return t;
+}
 
 int64_t estimate_footprint(const Problem &p, const Options &o) {
     Sizes s = sizes_of(p, o);

@@ -13,12 +13,20 @@
 
 namespace nmk {
 
+// launch shapes of the line solvers: chunks per line and lines per block
+struct Tuning {
+    int x_chunks = 0, x_rows = 0;  // x lines: chunks per row, rows per block
+    int y_chunks = 0, y_cols = 0;  // y lines: chunks per column, columns per block
+};
+Tuning default_tuning(int precision, int Mx, int My);
+
 struct Options {
     int precision = 64;  // 64: complex128 march; 32: complex64 march, FP64 coefficients
     int wl_batch = 1;    // wavelengths marched together (<= 0: all)
     int use_graph = 0;   // capture the z-march as one hipGraph and replay it
     int device = -1;     // HIP device (-1: current)
     int profile = 0;     // time every kernel family with events (no graph)
+    Tuning tuning;       // x_chunks == 0: default shapes
 };
 
 // per-kernel-family device time of one propagate() (profile mode)

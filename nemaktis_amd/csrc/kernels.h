@@ -16,8 +16,10 @@ void launch_slab_material(const double *lc, const uint8_t *mask, const double *w
                           const SlotParams *sp, int nslots, int iz, double *sm, hipStream_t st);
 
 // mu*R stencil coefficients and the phase operator on the march grid
+// (xt: the x-line tridiagonal rows in the chunk-interleaved layout of lpx)
 template <class T>
-void launch_coefs(const double *sm, const Geo &g, const SlotParams *sp, int nslots, C2<T> *cf, hipStream_t st);
+void launch_coefs(const double *sm, const Geo &g, const SlotParams *sp, int nslots, C2<T> *cf, C2<T> *xt,
+                  const LinePlan &lpx, hipStream_t st);
 
 // input beam + input Fresnel (beam != None) or packing of caller fields
 template <class T>
@@ -32,15 +34,30 @@ template <class T>
 void launch_shift(const C2<T> *in, C2<T> *out, const Geo &g, const PropParams *pp, int nprops, hipStream_t st);
 template <class T>
 void launch_phase(C2<T> *f, const C2<T> *cf, const Geo &g, const PropParams *pp, int nprops, hipStream_t st);
+// slab boundary out = Pn Sh^nshift Pp in (cf_prev / cf_next may be null)
+template <class T>
+void launch_boundary(const C2<T> *in, C2<T> *out, const C2<T> *cf_prev, const C2<T> *cf_next, const Geo &g,
+                     const PropParams *pp, int nprops, int nshift, hipStream_t st);
 // out = in + mu*A_y in (dir 1) or in + mu*A_x in (dir 0)
 template <class T>
 void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, const Geo &g, const PropParams *pp,
                      int nprops, hipStream_t st);
-// cyclic block-tridiagonal line solves along x (dir 0) or y (dir 1):
-// neumann == 0: dst = M src, acc = dst;  neumann == 1: dst = M (mu O src), acc += dst
+// cyclic block-tridiagonal line solves of the implicit half-steps:
+//   x lines, neumann == 0: dst = M_x (I + mu A_y) src, acc = dst
+//   x lines, neumann == 1: dst = M_x (mu O_x src),     acc += dst
+//   y lines, neumann == 0: dst = M_y (I + mu A_x) src, acc = dst
+//   y lines, neumann == 1: dst = M_y (mu O_y src),     acc += dst
+LinePlan make_line_plan(int M, int chunks);
 template <class T>
-void launch_line_solve(int dir, int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf,
-                       const Geo &g, const PropParams *pp, int nprops, hipStream_t st);
+size_t xsolve_lds_bytes(const LinePlan &lp, int R);
+template <class T>
+size_t ysolve_lds_bytes(const LinePlan &lp, int W);
+template <class T>
+void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
+                   const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, hipStream_t st);
+template <class T>
+void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
+                   const PropParams *pp, int nprops, const LinePlan &lp, int W, hipStream_t st);
 // interior march state -> full-grid complex128 fields with the global phase
 // and the output Fresnel interface
 template <class T>

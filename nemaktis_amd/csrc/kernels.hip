@@ -202,7 +202,7 @@ void launch_slab_material(const double *lc, const uint8_t *mask, const double *w
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void k_coefs(const double *__restrict__ sm, Geo g, const SlotParams *__restrict__ sp,
-                                               C2<T> *__restrict__ cf) {
+                                               C2<T> *__restrict__ cf, C2<T> *__restrict__ xt, LinePlan lpx) {
     const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= g.NP) return;
     const int slot = blockIdx.y;
@@ -333,11 +333,26 @@ __global__ __launch_bounds__(256) void k_coefs(const double *__restrict__ sm, Ge
     o[(int64_t)(CF_P + 0) * g.NP] = C2<T>{(T)at(SM_PXX_RE, ix, iy), (T)at(SM_PXX_IM, ix, iy)};
     o[(int64_t)(CF_P + 1) * g.NP] = C2<T>{(T)at(SM_PYY_RE, ix, iy), (T)at(SM_PYY_IM, ix, iy)};
     o[(int64_t)(CF_P + 2) * g.NP] = C2<T>{(T)at(SM_PXY_RE, ix, iy), (T)at(SM_PXY_IM, ix, iy)};
+    // (I - mu T) rows of the x-line systems (T = DX00, DX11) in the chunk-interleaved layout
+    if (g.px && xt) {
+        int pc, i;
+        chunk_of(lpx, ix, pc, i);
+        const int64_t SL = (int64_t)lpx.lmax * lpx.P;
+        for (int c2 = 0; c2 < 2; c2++) {
+            const int k0 = CF_DX + (c2 == 0 ? 0 : 3) * 3;
+            const int64_t base = ((((int64_t)slot * 2 + c2) * 3) * g.My + iy) * SL + (int64_t)i * lpx.P + pc;
+            const int64_t kst = (int64_t)g.My * SL;
+            xt[base] = C2<T>{(T)(hz * im[k0 + 0]), (T)(-hz * re[k0 + 0])};
+            xt[base + kst] = C2<T>{(T)(1.0 + hz * im[k0 + 1]), (T)(-hz * re[k0 + 1])};
+            xt[base + 2 * kst] = C2<T>{(T)(hz * im[k0 + 2]), (T)(-hz * re[k0 + 2])};
+        }
+    }
 }
 
 template <class T>
-void launch_coefs(const double *sm, const Geo &g, const SlotParams *sp, int nslots, C2<T> *cf, hipStream_t st) {
-    hipLaunchKernelGGL(k_coefs<T>, dim3(blocks_for(g.NP, 256), nslots), dim3(256), 0, st, sm, g, sp, cf);
+void launch_coefs(const double *sm, const Geo &g, const SlotParams *sp, int nslots, C2<T> *cf, C2<T> *xt,
+                  const LinePlan &lpx, hipStream_t st) {
+    hipLaunchKernelGGL(k_coefs<T>, dim3(blocks_for(g.NP, 256), nslots), dim3(256), 0, st, sm, g, sp, cf, xt, lpx);
     NMK_LAUNCH_CHECK();
 }
 
@@ -459,6 +474,103 @@ void launch_shift(const C2<T> *in, C2<T> *out, const Geo &g, const PropParams *p
 }
 
 // ---------------------------------------------------------------------------
+// slab boundary: out = Pn . Sh^s . Pp (in).  Between slabs k and k+1 the
+// reference applies P_k, Sh (end of slab k), Sh, P_{k+1} (start of slab k+1)
+// (BPMIteration.cpp:135-144); the two Catmull-Rom shifts are composed into
+// one 7-tap filter per axis (the x and y passes commute).  Pp / Pn are the
+// phase operators of the previous / next slab (either may be absent), and
+// the first and last boundaries carry a single shift.  One block computes a
+// TW x TH output tile from an LDS copy of its (TW+6) x (TH+6) input window.
+// ---------------------------------------------------------------------------
+constexpr int kBTW = 64, kBTH = 4;
+
+template <class T>
+__global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
+                                                  const C2<T> *__restrict__ cf_prev, const C2<T> *__restrict__ cf_next,
+                                                  Geo g, const PropParams *__restrict__ pp, int nshift) {
+    using C = C2<T>;
+    __shared__ C tile[2][kBTH + 6][kBTW + 7];
+    __shared__ C tmp[2][kBTH + 6][kBTW + 1];
+    const int prop = blockIdx.z;
+    const PropParams p = pp[prop];
+    const int64_t NP = g.NP;
+    const int x0 = blockIdx.x * kBTW, y0 = blockIdx.y * kBTH;
+    const double *wxs = nshift == 2 ? p.w2x : p.w1x, *wys = nshift == 2 ? p.w2y : p.w1y;
+    const int ox = g.px ? (nshift == 2 ? p.o2x : p.o1x) : 0;
+    const int oy = g.py ? (nshift == 2 ? p.o2y : p.o1y) : 0;
+    const C *f0 = in + ((int64_t)prop * 2) * NP, *f1 = f0 + NP;
+    const C *cp = cf_prev ? cf_prev + ((int64_t)p.slot * CF_COUNT + CF_P) * NP : nullptr;
+    auto wrapx = [&](int i) { return g.px ? wrapm(i, g.Mx) : min(max(i, 0), g.Mx - 1); };
+    auto wrapy = [&](int i) { return g.py ? wrapm(i, g.My) : min(max(i, 0), g.My - 1); };
+    // input window with the previous slab's phase operator applied
+    for (int e = threadIdx.x; e < (kBTH + 6) * (kBTW + 6); e += blockDim.x) {
+        const int r = e / (kBTW + 6), c = e - r * (kBTW + 6);
+        const int64_t q = wrapx(x0 + ox + c) + (int64_t)g.Mx * wrapy(y0 + oy + r);
+        C a = f0[q], b = f1[q];
+        if (cp) {
+            const C pxx = cp[q], pyy = cp[NP + q], pxy = cp[2 * NP + q];
+            const C na = pxx * a + pxy * b, nb = pxy * a + pyy * b;
+            a = na;
+            b = nb;
+        }
+        tile[0][r][c] = a;
+        tile[1][r][c] = b;
+    }
+    __syncthreads();
+    T wx[7], wy[7];
+#pragma unroll
+    for (int t = 0; t < 7; t++) {
+        wx[t] = g.px ? (T)wxs[t] : (T)(t == 0 ? 1 : 0);
+        wy[t] = g.py ? (T)wys[t] : (T)(t == 0 ? 1 : 0);
+    }
+    // x pass
+    for (int e = threadIdx.x; e < (kBTH + 6) * kBTW; e += blockDim.x) {
+        const int r = e / kBTW, c = e - r * kBTW;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            C a = wx[0] * tile[k][r][c];
+#pragma unroll
+            for (int t = 1; t < 7; t++) a = a + wx[t] * tile[k][r][c + t];
+            tmp[k][r][c] = a;
+        }
+    }
+    __syncthreads();
+    // y pass and the next slab's phase operator
+    for (int e = threadIdx.x; e < kBTH * kBTW; e += blockDim.x) {
+        const int r = e / kBTW, c = e - r * kBTW;
+        const int x = x0 + c, y = y0 + r;
+        if (x >= g.Mx || y >= g.My) continue;
+        C v[2];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            C a = wy[0] * tmp[k][r][c];
+#pragma unroll
+            for (int t = 1; t < 7; t++) a = a + wy[t] * tmp[k][r + t][c];
+            v[k] = a;
+        }
+        const int64_t q = x + (int64_t)g.Mx * y;
+        if (cf_next) {
+            const C *cn = cf_next + ((int64_t)p.slot * CF_COUNT + CF_P) * NP;
+            const C pxx = cn[q], pyy = cn[NP + q], pxy = cn[2 * NP + q];
+            const C a = pxx * v[0] + pxy * v[1], b = pxy * v[0] + pyy * v[1];
+            v[0] = a;
+            v[1] = b;
+        }
+        C *o0 = out + ((int64_t)prop * 2) * NP;
+        o0[q] = v[0];
+        o0[NP + q] = v[1];
+    }
+}
+
+template <class T>
+void launch_boundary(const C2<T> *in, C2<T> *out, const C2<T> *cf_prev, const C2<T> *cf_next, const Geo &g,
+                     const PropParams *pp, int nprops, int nshift, hipStream_t st) {
+    dim3 grid((g.Mx + kBTW - 1) / kBTW, (g.My + kBTH - 1) / kBTH, nprops);
+    hipLaunchKernelGGL(k_boundary<T>, grid, dim3(256), 0, st, in, out, cf_prev, cf_next, g, pp, nshift);
+    NMK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
 // pointwise 2x2 phase operator (PhaseEvolutionOperator.cpp:14-32)
 // ---------------------------------------------------------------------------
 template <class T>
@@ -573,260 +685,510 @@ void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, cons
 }
 
 // ---------------------------------------------------------------------------
-// cyclic line solves (I - mu T) x = d along x (T = DX00 / DX11) or y
-// (T = DY00 / DY11), the implicit halves of the ADI step with their Neumann
-// series (ParaxialPrimaryEvolutionOperator.cpp:162-179, 190-207; line solver
-// BlockSparseMatrix.cpp:194-258, 385-450).
-//
-// One workgroup solves one periodic line of length M for one component of
-// one propagation.  The line is cut into P chunks (P a power of two); each
-// thread runs a Thomas elimination on its chunk's interior for the right-hand
-// side and for the two coupling responses, the P chunk-boundary unknowns form
-// a cyclic tridiagonal system solved by parallel cyclic reduction in LDS,
-// and each thread then rebuilds its chunk.
-// In Neumann mode the right-hand side is mu*O*src, evaluated on the fly:
-//   x lines: O_x = [XY00  DX01+XY01; DX10  0]
-//   y lines: O_y = [0  DY01; DY10+XY10  XY11]
+// right-hand sides of the implicit half-steps, evaluated on the fly
+// (ParaxialPrimaryEvolutionOperator.cpp:153-207):
+//   x lines, first solve:  (I + mu A_y) X,   A_y = [DY00 DY01; DY10+XY10 DY11+XY11]
+//   x lines, Neumann:      mu O_x S,         O_x = [XY00  DX01+XY01; DX10  0]
+//   y lines, first solve:  (I + mu A_x) S,   A_x = [DX00+XY00 DX01+XY01; DX10 DX11]
+//   y lines, Neumann:      mu O_y S,         O_y = [0  DY01; DY10+XY10  XY11]
+// (mu is folded into the stored coefficients)
 // ---------------------------------------------------------------------------
-constexpr int kMaxLinePar = 256;
-
-template <class T, int DIR>
-__device__ __forceinline__ C2<T> neumann_rhs(int comp, int jx, int jy, const C2<T> *__restrict__ cfs,
-                                             const C2<T> *__restrict__ s0, const C2<T> *__restrict__ s1,
-                                             const Geo &g) {
-    const int64_t NP = g.NP;
-    const int64_t ip = jx + (int64_t)g.Mx * jy;
+template <class T>
+__device__ __forceinline__ C2<T> rhs_yplus(int comp, int jx, int jy, const C2<T> *__restrict__ cfs,
+                                           const C2<T> *__restrict__ f0, const C2<T> *__restrict__ f1, const Geo &g) {
+    const int64_t NP = g.NP, ip = jx + (int64_t)g.Mx * jy;
     const C2<T> *c = cfs + ip;
     const Nbr n = neighbours(g, jx, jy);
     C2<T> a{0, 0};
-    if (DIR == 0) {
-        const int64_t row = (int64_t)g.Mx * jy;
-        if (comp == 0) {
-            a = sx3(a, c, NP, CF_DX + 1 * 3, s1, row, n, jx);
-            a = sxy4(a, c, NP, CF_XY + 1 * 4, s1, g.Mx, n);
-            a = sxy4(a, c, NP, CF_XY + 0 * 4, s0, g.Mx, n);
-        } else {
-            a = sx3(a, c, NP, CF_DX + 2 * 3, s0, row, n, jx);
-        }
+    if (comp == 0) {
+        a = sy3(a, c, NP, CF_DY + 0 * 3, f0, g.Mx, n, jx, jy);
+        a = sy3(a, c, NP, CF_DY + 1 * 3, f1, g.Mx, n, jx, jy);
+        return f0[ip] + a;
+    }
+    a = sy3(a, c, NP, CF_DY + 3 * 3, f1, g.Mx, n, jx, jy);
+    a = sxy4(a, c, NP, CF_XY + 3 * 4, f1, g.Mx, n);
+    a = sy3(a, c, NP, CF_DY + 2 * 3, f0, g.Mx, n, jx, jy);
+    a = sxy4(a, c, NP, CF_XY + 2 * 4, f0, g.Mx, n);
+    return f1[ip] + a;
+}
+
+template <class T>
+__device__ __forceinline__ C2<T> rhs_xplus(int comp, int jx, int jy, const C2<T> *__restrict__ cfs,
+                                           const C2<T> *__restrict__ f0, const C2<T> *__restrict__ f1, const Geo &g) {
+    const int64_t NP = g.NP, ip = jx + (int64_t)g.Mx * jy, row = (int64_t)g.Mx * jy;
+    const C2<T> *c = cfs + ip;
+    const Nbr n = neighbours(g, jx, jy);
+    C2<T> a{0, 0};
+    if (comp == 0) {
+        a = sx3(a, c, NP, CF_DX + 0 * 3, f0, row, n, jx);
+        a = sxy4(a, c, NP, CF_XY + 0 * 4, f0, g.Mx, n);
+        a = sx3(a, c, NP, CF_DX + 1 * 3, f1, row, n, jx);
+        a = sxy4(a, c, NP, CF_XY + 1 * 4, f1, g.Mx, n);
+        return f0[ip] + a;
+    }
+    a = sx3(a, c, NP, CF_DX + 3 * 3, f1, row, n, jx);
+    a = sx3(a, c, NP, CF_DX + 2 * 3, f0, row, n, jx);
+    return f1[ip] + a;
+}
+
+template <class T>
+__device__ __forceinline__ C2<T> rhs_ox(int comp, int jx, int jy, const C2<T> *__restrict__ cfs,
+                                        const C2<T> *__restrict__ s0, const C2<T> *__restrict__ s1, const Geo &g) {
+    const int64_t NP = g.NP, ip = jx + (int64_t)g.Mx * jy, row = (int64_t)g.Mx * jy;
+    const C2<T> *c = cfs + ip;
+    const Nbr n = neighbours(g, jx, jy);
+    C2<T> a{0, 0};
+    if (comp == 0) {
+        a = sx3(a, c, NP, CF_DX + 1 * 3, s1, row, n, jx);
+        a = sxy4(a, c, NP, CF_XY + 1 * 4, s1, g.Mx, n);
+        a = sxy4(a, c, NP, CF_XY + 0 * 4, s0, g.Mx, n);
     } else {
-        if (comp == 0) {
-            a = sy3(a, c, NP, CF_DY + 1 * 3, s1, g.Mx, n, jx, jy);
-        } else {
-            a = sy3(a, c, NP, CF_DY + 2 * 3, s0, g.Mx, n, jx, jy);
-            a = sxy4(a, c, NP, CF_XY + 2 * 4, s0, g.Mx, n);
-            a = sxy4(a, c, NP, CF_XY + 3 * 4, s1, g.Mx, n);
-        }
+        a = sx3(a, c, NP, CF_DX + 2 * 3, s0, row, n, jx);
     }
     return a;
 }
 
-template <class T, int LMAX, int DIR, int NEU>
-__global__ __launch_bounds__(256) void k_line_solve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
-                                                    C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf, Geo g,
-                                                    const PropParams *__restrict__ pp, int P) {
+template <class T>
+__device__ __forceinline__ C2<T> rhs_oy(int comp, int jx, int jy, const C2<T> *__restrict__ cfs,
+                                        const C2<T> *__restrict__ s0, const C2<T> *__restrict__ s1, const Geo &g) {
+    const int64_t NP = g.NP, ip = jx + (int64_t)g.Mx * jy;
+    const C2<T> *c = cfs + ip;
+    const Nbr n = neighbours(g, jx, jy);
+    C2<T> a{0, 0};
+    if (comp == 0) {
+        a = sy3(a, c, NP, CF_DY + 1 * 3, s1, g.Mx, n, jx, jy);
+    } else {
+        a = sy3(a, c, NP, CF_DY + 2 * 3, s0, g.Mx, n, jx, jy);
+        a = sxy4(a, c, NP, CF_XY + 2 * 4, s0, g.Mx, n);
+        a = sxy4(a, c, NP, CF_XY + 3 * 4, s1, g.Mx, n);
+    }
+    return a;
+}
+
+// ---------------------------------------------------------------------------
+// cyclic line solves (I - mu T) x = d, T = DX00/DX11 along x or DY00/DY11
+// along y (line solver of BlockSparseMatrix.cpp:194-258, 385-450).
+//
+// A line of M unknowns is cut into P chunks (P a power of two).  One thread
+// owns one chunk: a Thomas elimination of the chunk interior for the
+// right-hand side and for the two coupling responses, then the P
+// chunk-boundary unknowns form a cyclic tridiagonal system solved by
+// parallel cyclic reduction in LDS, and the thread rebuilds its chunk.
+// ---------------------------------------------------------------------------
+template <class T, int LMAX>
+struct Chunk {
+    C2<T> gam[LMAX], yu[LMAX], yg[LMAX];
+    C2<T> hl, ul, gl, uf, gf, hf;
+};
+
+// forward elimination + first backward pass (boundary values of the chunk
+// interior responses); load(i, a, b, c, d) returns row i of the interior
+template <class T, int LMAX, class F>
+__device__ __forceinline__ void chunk_eliminate(Chunk<T, LMAX> &ck, int n, F &&load) {
     using C = C2<T>;
-    const int line = blockIdx.x, comp = blockIdx.y, prop = blockIdx.z;
-    const int slot = pp[prop].slot;
-    const int M = DIR == 0 ? g.Mx : g.My;
-    const int64_t NP = g.NP;
-    const C *cfs = cf + (int64_t)slot * CF_COUNT * NP;
-    const int fam = DIR == 0 ? CF_DX : CF_DY;
-    const int blk = comp == 0 ? 0 : 3;
-    const C *ca = cfs + (int64_t)(fam + blk * 3 + 0) * NP;
-    const C *cb = cfs + (int64_t)(fam + blk * 3 + 1) * NP;
-    const C *cc = cfs + (int64_t)(fam + blk * 3 + 2) * NP;
-    const C *s0 = src + ((int64_t)prop * 2) * NP;
-    const C *s1 = s0 + NP;
-    C *d_out = dst + ((int64_t)prop * 2 + comp) * NP;
-    C *a_out = acc + ((int64_t)prop * 2 + comp) * NP;
-
-    auto pt = [&](int j) -> int64_t { return DIR == 0 ? (int64_t)j + (int64_t)g.Mx * line : (int64_t)line + (int64_t)g.Mx * j; };
-    auto rhs = [&](int j) -> C {
-        if (NEU == 0) return (comp == 0 ? s0 : s1)[pt(j)];
-        return DIR == 0 ? neumann_rhs<T, 0>(comp, j, line, cfs, s0, s1, g) : neumann_rhs<T, 1>(comp, line, j, cfs, s0, s1, g);
-    };
-    // (I - mu T): a_j = -muT(minus), b_j = 1 - muT(centre), c_j = -muT(plus)
-    auto coef_a = [&](int j) { C v = ca[pt(j)]; return C{-v.x, -v.y}; };
-    auto coef_b = [&](int j) { C v = cb[pt(j)]; return C{(T)1 - v.x, -v.y}; };
-    auto coef_c = [&](int j) { C v = cc[pt(j)]; return C{-v.x, -v.y}; };
-
-    __shared__ C s_uf[kMaxLinePar], s_gf[kMaxLinePar], s_hf[kMaxLinePar];
-    __shared__ C s_A[kMaxLinePar], s_B[kMaxLinePar], s_C[kMaxLinePar], s_D[kMaxLinePar], s_X[kMaxLinePar];
-
-    const int p = threadIdx.x;
-    const bool active = p < P;
-    const int base = M / P, rem = M % P;
-    const int Lp = base + (p < rem ? 1 : 0);
-    const int s = p * base + (p < rem ? p : rem);
-    const int n = Lp - 1;  // interior unknowns of the chunk (>= 1)
-
-    C gam[LMAX], yu[LMAX], yg[LMAX];
     C binv{0, 0}, cprev{0, 0};
-    C ul{0, 0}, gl{0, 0}, hl{0, 0}, uf{0, 0}, gf{0, 0}, hf{0, 0};
-    if (active) {
 #pragma unroll
-        for (int i = 0; i < LMAX; i++) {
-            if (i < n) {
-                const int j = s + i;
-                const C a = coef_a(j), b = coef_b(j), c = coef_c(j), d = rhs(j);
-                if (i == 0) {
-                    const C bi = crcp(b);
-                    yu[0] = d * bi;
-                    yg[0] = -(a * bi);
-                    binv = bi;
-                } else {
-                    const C gm = cprev * binv;
-                    gam[i - 1] = gm;
-                    const C bi = crcp(b - a * gm);
-                    yu[i] = (d - a * yu[i - 1]) * bi;
-                    yg[i] = -(a * yg[i - 1]) * bi;
-                    binv = bi;
-                }
-                cprev = c;
+    for (int i = 0; i < LMAX; i++) {
+        if (i < n) {
+            C a, b, c, d;
+            load(i, a, b, c, d);
+            if (i == 0) {
+                const C bi = crcp(b);
+                ck.yu[0] = d * bi;
+                ck.yg[0] = -(a * bi);
+                binv = bi;
+            } else {
+                const C gm = cprev * binv;
+                ck.gam[i - 1] = gm;
+                const C bi = crcp(b - a * gm);
+                ck.yu[i] = (d - a * ck.yu[i - 1]) * bi;
+                ck.yg[i] = -(a * ck.yg[i - 1]) * bi;
+                binv = bi;
             }
+            cprev = c;
         }
-        hl = -(cprev * binv);
-        C u{0, 0}, gg{0, 0}, h{0, 0};
+    }
+    ck.hl = -(cprev * binv);
+    C u{0, 0}, gg{0, 0}, h{0, 0};
 #pragma unroll
-        for (int k = 0; k < LMAX; k++) {
-            const int i = LMAX - 1 - k;
-            if (i == n - 1) {
-                u = yu[i];
-                gg = yg[i];
-                h = hl;
-                ul = u;
-                gl = gg;
-            } else if (i < n - 1) {
-                u = yu[i] - gam[i] * u;
-                gg = yg[i] - gam[i] * gg;
-                h = -(gam[i] * h);
-            }
+    for (int k = 0; k < LMAX; k++) {
+        const int i = LMAX - 1 - k;
+        if (i == n - 1) {
+            u = ck.yu[i];
+            gg = ck.yg[i];
+            h = ck.hl;
+            ck.ul = u;
+            ck.gl = gg;
+        } else if (i < n - 1) {
+            u = ck.yu[i] - ck.gam[i] * u;
+            gg = ck.yg[i] - ck.gam[i] * gg;
+            h = -(ck.gam[i] * h);
         }
-        uf = u;
-        gf = gg;
-        hf = h;
-        s_uf[p] = uf;
-        s_gf[p] = gf;
-        s_hf[p] = hf;
     }
-    __syncthreads();
-    C xi{0, 0};
-    if (active) {
-        const int e = s + n;
-        const int nx = (p + 1) & (P - 1);
-        const C ae = coef_a(e), be = coef_b(e), ce = coef_c(e), de = rhs(e);
-        s_A[p] = ae * gl;
-        s_B[p] = be + ae * hl + ce * s_gf[nx];
-        s_C[p] = ce * s_hf[nx];
-        s_D[p] = de - ae * ul - ce * s_uf[nx];
-    }
-    __syncthreads();
-    for (int st = 1; 2 * st < P; st <<= 1) {
-        C nA{0, 0}, nB{0, 0}, nC{0, 0}, nD{0, 0};
-        if (active) {
-            const int im = (p - st) & (P - 1), ipp = (p + st) & (P - 1);
-            const C k1 = cdiv(s_A[p], s_B[im]), k2 = cdiv(s_C[p], s_B[ipp]);
-            nA = -(k1 * s_A[im]);
-            nC = -(k2 * s_C[ipp]);
-            nB = s_B[p] - k1 * s_C[im] - k2 * s_A[ipp];
-            nD = s_D[p] - k1 * s_D[im] - k2 * s_D[ipp];
-        }
-        __syncthreads();
-        if (active) {
-            s_A[p] = nA;
-            s_B[p] = nB;
-            s_C[p] = nC;
-            s_D[p] = nD;
-        }
-        __syncthreads();
-    }
-    if (active) {
-        if (P == 1) {
-            xi = cdiv(s_D[0], s_A[0] + s_B[0] + s_C[0]);
-        } else {
-            const int q = (p + P / 2) & (P - 1);
-            const C apc = s_A[p] + s_C[p], aqc = s_A[q] + s_C[q];
-            xi = cdiv(s_D[p] * s_B[q] - apc * s_D[q], s_B[p] * s_B[q] - apc * aqc);
-        }
-        s_X[p] = xi;
-    }
-    __syncthreads();
-    if (active) {
-        const C xprev = s_X[(p - 1) & (P - 1)];
-        C u{0, 0}, gg{0, 0}, h{0, 0};
+    ck.uf = u;
+    ck.gf = gg;
+    ck.hf = h;
+}
+
+// x_i = u_i + xprev g_i + xi h_i for the chunk interior; store(i, x_i)
+template <class T, int LMAX, class S>
+__device__ __forceinline__ void chunk_rebuild(const Chunk<T, LMAX> &ck, int n, C2<T> xprev, C2<T> xi, S &&store) {
+    C2<T> u{0, 0}, gg{0, 0}, h{0, 0};
 #pragma unroll
-        for (int k = 0; k < LMAX; k++) {
-            const int i = LMAX - 1 - k;
-            bool live = false;
-            if (i == n - 1) {
-                u = yu[i];
-                gg = yg[i];
-                h = hl;
-                live = true;
-            } else if (i < n - 1) {
-                u = yu[i] - gam[i] * u;
-                gg = yg[i] - gam[i] * gg;
-                h = -(gam[i] * h);
-                live = true;
-            }
-            if (live) {
-                const C x = u + xprev * gg + xi * h;
-                const int64_t q = pt(s + i);
-                d_out[q] = x;
-                if (NEU) a_out[q] = a_out[q] + x;
-                else a_out[q] = x;
-            }
+    for (int k = 0; k < LMAX; k++) {
+        const int i = LMAX - 1 - k;
+        bool live = false;
+        if (i == n - 1) {
+            u = ck.yu[i];
+            gg = ck.yg[i];
+            h = ck.hl;
+            live = true;
+        } else if (i < n - 1) {
+            u = ck.yu[i] - ck.gam[i] * u;
+            gg = ck.yg[i] - ck.gam[i] * gg;
+            h = -(ck.gam[i] * h);
+            live = true;
         }
-        const int64_t qe = pt(s + n);
-        d_out[qe] = xi;
-        if (NEU) a_out[qe] = a_out[qe] + xi;
-        else a_out[qe] = xi;
+        if (live) store(i, u + xprev * gg + xi * h);
     }
 }
 
-template <class T, int DIR, int NEU>
-static void line_solve_dispatch(int lmax, dim3 grid, int bs, const C2<T> *src, C2<T> *dst, C2<T> *acc,
-                                const C2<T> *cf, const Geo &g, const PropParams *pp, int P, hipStream_t st) {
-    switch (lmax) {
-        case 4: hipLaunchKernelGGL((k_line_solve<T, 4, DIR, NEU>), grid, dim3(bs), 0, st, src, dst, acc, cf, g, pp, P); break;
-        case 8: hipLaunchKernelGGL((k_line_solve<T, 8, DIR, NEU>), grid, dim3(bs), 0, st, src, dst, acc, cf, g, pp, P); break;
-        case 16: hipLaunchKernelGGL((k_line_solve<T, 16, DIR, NEU>), grid, dim3(bs), 0, st, src, dst, acc, cf, g, pp, P); break;
-        case 32: hipLaunchKernelGGL((k_line_solve<T, 32, DIR, NEU>), grid, dim3(bs), 0, st, src, dst, acc, cf, g, pp, P); break;
-        default: throw std::runtime_error("line solver: unsupported chunk length");
+// interface equation of chunk p: A xi_{p-1} + B xi_p + C xi_{p+1} = D, from
+// the chunk's own responses and the first interior values of chunk p+1
+template <class T, int LMAX>
+__device__ __forceinline__ void interface_row(const Chunk<T, LMAX> &ck, C2<T> ae, C2<T> be, C2<T> ce, C2<T> de,
+                                              C2<T> uf_next, C2<T> gf_next, C2<T> hf_next, C2<T> &A, C2<T> &B,
+                                              C2<T> &Cv, C2<T> &D) {
+    A = ae * ck.gl;
+    B = be + ae * ck.hl + ce * gf_next;
+    Cv = ce * hf_next;
+    D = de - ae * ck.ul - ce * uf_next;
+}
+
+// cyclic tridiagonal system over the P chunk boundaries of one line, held in
+// LDS at slots base + q*stride; every thread of the block must call it.
+// Parallel cyclic reduction until the couplings reach P/2, then 2x2 pairs.
+template <class T>
+__device__ __forceinline__ void reduced_pcr(C2<T> *sA, C2<T> *sB, C2<T> *sC, C2<T> *sD, C2<T> *sX, int P, int p,
+                                            int base, int stride, bool active) {
+    using C = C2<T>;
+    for (int st = 1; 2 * st < P; st <<= 1) {
+        C nA{0, 0}, nB{0, 0}, nC{0, 0}, nD{0, 0};
+        if (active) {
+            const int me = base + p * stride;
+            const int im = base + ((p - st) & (P - 1)) * stride, iq = base + ((p + st) & (P - 1)) * stride;
+            const C k1 = cdiv(sA[me], sB[im]), k2 = cdiv(sC[me], sB[iq]);
+            nA = -(k1 * sA[im]);
+            nC = -(k2 * sC[iq]);
+            nB = sB[me] - k1 * sC[im] - k2 * sA[iq];
+            nD = sD[me] - k1 * sD[im] - k2 * sD[iq];
+        }
+        __syncthreads();
+        if (active) {
+            const int me = base + p * stride;
+            sA[me] = nA;
+            sB[me] = nB;
+            sC[me] = nC;
+            sD[me] = nD;
+        }
+        __syncthreads();
+    }
+    if (active) {
+        const int me = base + p * stride;
+        C xi;
+        if (P == 1) {
+            xi = cdiv(sD[me], sA[me] + sB[me] + sC[me]);
+        } else {
+            const int q = base + ((p + P / 2) & (P - 1)) * stride;
+            const C apc = sA[me] + sC[me], aqc = sA[q] + sC[q];
+            xi = cdiv(sD[me] * sB[q] - apc * sD[q], sB[me] * sB[q] - apc * aqc);
+        }
+        sX[me] = xi;
+    }
+    __syncthreads();
+}
+
+// LDS bytes of the reduced-system arrays (first values + A, B, C, D) per line
+template <class T>
+__host__ __device__ constexpr int reduced_lds_per_chunk() {
+    return 7 * (int)sizeof(C2<T>);
+}
+
+// ---- x lines: R rows per block, P chunks per row; the right-hand sides are
+// staged through LDS in the chunk-interleaved order (element (p, i) of a row
+// at i*P + p) so that global loads and stores stay coalesced; the tridiagonal
+// coefficients come pre-interleaved from the coefficient kernel -------------
+template <class T, int LMAX, int NEU>
+__global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
+                                                 C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf,
+                                                 const C2<T> *__restrict__ xt, Geo g,
+                                                 const PropParams *__restrict__ pp, LinePlan lp, int R) {
+    using C = C2<T>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int P = lp.P, SL = LMAX * P, M = g.Mx;
+    C *sd = reinterpret_cast<C *>(smem_raw);
+    C *sUF = sd + R * SL, *sGF = sUF + R * P, *sHF = sGF + R * P;
+    C *sA = sHF + R * P, *sB = sA + R * P, *sC = sB + R * P, *sD = sC + R * P;
+    C *sX = sUF;
+    const int row0 = blockIdx.x * R, comp = blockIdx.y, prop = blockIdx.z;
+    const int slot = pp[prop].slot;
+    const int64_t NP = g.NP;
+    const C *cfs = cf + (int64_t)slot * CF_COUNT * NP;
+    const C *s0 = src + ((int64_t)prop * 2) * NP, *s1 = s0 + NP;
+
+    {
+        const int ntile = R * M, bs = blockDim.x;
+        for (int e0 = threadIdx.x; e0 < ntile; e0 += 4 * bs) {
+            C d[4];
+            int slotidx[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = e0 + u * bs;
+                slotidx[u] = -1;
+                if (e < ntile) {
+                    const int r = e / M, x = e - r * M, y = row0 + r;
+                    if (y < g.My) {
+                        d[u] = NEU ? rhs_ox<T>(comp, x, y, cfs, s0, s1, g) : rhs_yplus<T>(comp, x, y, cfs, s0, s1, g);
+                        int p, i;
+                        chunk_of(lp, x, p, i);
+                        slotidx[u] = r * SL + i * P + p;
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (slotidx[u] >= 0) sd[slotidx[u]] = d[u];
+        }
+    }
+    __syncthreads();
+
+    const int r = threadIdx.x / P, p = threadIdx.x - r * P, y = row0 + r;
+    const bool active = r < R && y < g.My;
+    const int n = chunk_len(lp, p) - 1;
+    const int64_t xrow = ((((int64_t)slot * 2 + comp) * 3) * g.My + y) * SL;
+    const int64_t xstride = (int64_t)g.My * SL;
+    const C *xa = xt + xrow, *xb = xa + xstride, *xc = xb + xstride;
+    Chunk<T, LMAX> ck;
+    if (active) {
+        chunk_eliminate(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
+            const int q = i * P + p;
+            a = xa[q];
+            b = xb[q];
+            c = xc[q];
+            d = sd[r * SL + q];
+        });
+        sUF[r * P + p] = ck.uf;
+        sGF[r * P + p] = ck.gf;
+        sHF[r * P + p] = ck.hf;
+    }
+    __syncthreads();
+    C A{0, 0}, B{0, 0}, Cv{0, 0}, D{0, 0};
+    if (active) {
+        const int nx = r * P + ((p + 1) & (P - 1)), q = n * P + p;
+        interface_row(ck, xa[q], xb[q], xc[q], sd[r * SL + q], sUF[nx], sGF[nx], sHF[nx], A, B, Cv, D);
+    }
+    __syncthreads();
+    if (active) {
+        sA[r * P + p] = A;
+        sB[r * P + p] = B;
+        sC[r * P + p] = Cv;
+        sD[r * P + p] = D;
+    }
+    __syncthreads();
+    reduced_pcr(sA, sB, sC, sD, sX, P, p, r * P, 1, active);
+    if (active) {
+        const C xi = sX[r * P + p], xprev = sX[r * P + ((p - 1) & (P - 1))];
+        chunk_rebuild(ck, n, xprev, xi, [&](int i, C v) { sd[r * SL + i * P + p] = v; });
+        sd[r * SL + n * P + p] = xi;
+    }
+    __syncthreads();
+
+    C *dd = dst + ((int64_t)prop * 2 + comp) * NP, *aa = acc + ((int64_t)prop * 2 + comp) * NP;
+    for (int e = threadIdx.x; e < R * M; e += blockDim.x) {
+        const int rr = e / M, x = e - rr * M, yy = row0 + rr;
+        if (yy < g.My) {
+            int pc, i;
+            chunk_of(lp, x, pc, i);
+            const C v = sd[rr * SL + i * P + pc];
+            const int64_t q = x + (int64_t)M * yy;
+            dd[q] = v;
+            aa[q] = NEU ? aa[q] + v : v;
+        }
+    }
+}
+
+// ---- y lines: W adjacent columns x P chunks per block.  The right-hand
+// sides of the W x M tile are computed by all threads (consecutive threads on
+// consecutive columns) and staged in LDS in the chunk-interleaved order
+// ((i*P + p)*W + column); the chunk threads then read the tridiagonal rows
+// from global memory one W-wide row segment at a time.
+template <class T, int LMAX, int NEU>
+__global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
+                                                 C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf, Geo g,
+                                                 const PropParams *__restrict__ pp, LinePlan lp, int W) {
+    using C = C2<T>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int P = lp.P, M = g.My;
+    C *sd = reinterpret_cast<C *>(smem_raw);  // LMAX * P * W
+    C *sUF = sd + LMAX * P * W;
+    C *sGF = sUF + W * P, *sHF = sGF + W * P;
+    C *sA = sHF + W * P, *sB = sA + W * P, *sC = sB + W * P, *sD = sC + W * P;
+    C *sX = sUF;
+    const int x0 = blockIdx.x * W, comp = blockIdx.y, prop = blockIdx.z;
+    const int slot = pp[prop].slot;
+    const int64_t NP = g.NP;
+    const C *cfs = cf + (int64_t)slot * CF_COUNT * NP;
+    const C *s0 = src + ((int64_t)prop * 2) * NP, *s1 = s0 + NP;
+
+    // right-hand sides of the tile, four per thread per round
+    const int ntile = M * W, bs = blockDim.x;
+    for (int e0 = threadIdx.x; e0 < ntile; e0 += 4 * bs) {
+        C d[4];
+        int slotidx[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int e = e0 + u * bs;
+            slotidx[u] = -1;
+            if (e < ntile) {
+                const int j = e / W, c = e - j * W, x = x0 + c;
+                if (x < g.Mx) {
+                    d[u] = NEU ? rhs_oy<T>(comp, x, j, cfs, s0, s1, g) : rhs_xplus<T>(comp, x, j, cfs, s0, s1, g);
+                    int p, i;
+                    chunk_of(lp, j, p, i);
+                    slotidx[u] = (i * P + p) * W + c;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (slotidx[u] >= 0) sd[slotidx[u]] = d[u];
+    }
+    __syncthreads();
+
+    const int cofs = threadIdx.x % W, p = threadIdx.x / W, x = x0 + cofs;
+    const bool active = x < g.Mx && p < P;
+    const int blk = comp == 0 ? 0 : 3;
+    const C *ca = cfs + (int64_t)(CF_DY + blk * 3 + 0) * NP;
+    const C *cb = cfs + (int64_t)(CF_DY + blk * 3 + 1) * NP;
+    const C *cc = cfs + (int64_t)(CF_DY + blk * 3 + 2) * NP;
+    const int s = chunk_start(lp, p), n = chunk_len(lp, p) - 1;
+    auto tri = [&](int j, C &a, C &b, C &c) {
+        const int64_t q = x + (int64_t)g.Mx * j;
+        const C va = ca[q], vb = cb[q], vc = cc[q];
+        a = C{-va.x, -va.y};
+        b = C{(T)1 - vb.x, -vb.y};
+        c = C{-vc.x, -vc.y};
+    };
+    const int me = p * W + cofs, nx = ((p + 1) & (P - 1)) * W + cofs;
+    Chunk<T, LMAX> ck;
+    if (active) {
+        chunk_eliminate(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
+            tri(s + i, a, b, c);
+            d = sd[(i * P + p) * W + cofs];
+        });
+        sUF[me] = ck.uf;
+        sGF[me] = ck.gf;
+        sHF[me] = ck.hf;
+    }
+    __syncthreads();
+    C A{0, 0}, B{0, 0}, Cv{0, 0}, D{0, 0};
+    if (active) {
+        C ae, be, ce;
+        tri(s + n, ae, be, ce);
+        interface_row(ck, ae, be, ce, sd[(n * P + p) * W + cofs], sUF[nx], sGF[nx], sHF[nx], A, B, Cv, D);
+    }
+    __syncthreads();
+    if (active) {
+        sA[me] = A;
+        sB[me] = B;
+        sC[me] = Cv;
+        sD[me] = D;
+    }
+    __syncthreads();
+    reduced_pcr(sA, sB, sC, sD, sX, P, p, cofs, W, active);
+    if (active) {
+        const C xi = sX[me], xprev = sX[((p - 1) & (P - 1)) * W + cofs];
+        C *dd = dst + ((int64_t)prop * 2 + comp) * NP, *aa = acc + ((int64_t)prop * 2 + comp) * NP;
+        auto put = [&](int j, C v) {
+            const int64_t q = x + (int64_t)g.Mx * j;
+            dd[q] = v;
+            aa[q] = NEU ? aa[q] + v : v;
+        };
+        chunk_rebuild(ck, n, xprev, xi, [&](int i, C v) { put(s + i, v); });
+        put(s + n, xi);
+    }
+}
+
+LinePlan make_line_plan(int M, int chunks) {
+    LinePlan lp{};
+    lp.M = M;
+    int cap = std::max(1, M / 2);
+    int P = 1;
+    while (P * 2 <= std::min(cap, chunks)) P *= 2;
+    lp.P = P;
+    lp.base = M / P;
+    lp.rem = M % P;
+    int need = lp.base + (lp.rem ? 1 : 0);
+    lp.lmax = need <= 4 ? 4 : need <= 8 ? 8 : need <= 16 ? 16 : need <= 32 ? 32 : 0;
+    if (lp.lmax == 0) throw std::runtime_error("line solver: chunks longer than 32 points (too few chunks)");
+    return lp;
+}
+
+template <class T>
+size_t xsolve_lds_bytes(const LinePlan &lp, int R) {
+    return (size_t)R * ((size_t)lp.lmax * lp.P * sizeof(C2<T>) + (size_t)lp.P * reduced_lds_per_chunk<T>());
+}
+template <class T>
+size_t ysolve_lds_bytes(const LinePlan &lp, int W) {
+    return (size_t)W * ((size_t)lp.lmax * lp.P * sizeof(C2<T>) + (size_t)lp.P * reduced_lds_per_chunk<T>());
+}
+
+template <class T, int NEU>
+static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
+                            const C2<T> *cf, const C2<T> *xt, const Geo &g, const PropParams *pp, hipStream_t st) {
+    const dim3 block(lp.P * R);
+    const size_t lds = xsolve_lds_bytes<T>(lp, R);
+    switch (lp.lmax) {
+        case 4: hipLaunchKernelGGL((k_xsolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R); break;
+        case 8: hipLaunchKernelGGL((k_xsolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R); break;
+        case 16: hipLaunchKernelGGL((k_xsolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R); break;
+        case 32: hipLaunchKernelGGL((k_xsolve<T, 32, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R); break;
+        default: throw std::runtime_error("x line solver: unsupported chunk length");
     }
     NMK_LAUNCH_CHECK();
 }
 
-// chunking of a periodic line of M unknowns: P (power of two) chunks of at
-// least 2 points, at most kMaxLinePar chunks, chunk length bound LMAX
-static void line_plan(int M, int &P, int &lmax, int &bs) {
-    int cap = std::min(kMaxLinePar, std::max(1, M / 2));
-    int pw = 1;
-    while (pw * 2 <= cap) pw *= 2;
-    // prefer one wavefront per line while chunks stay short
-    P = std::min(pw, 64);
-    if ((M + P - 1) / P > 16) P = pw;
-    int need = (M + P - 1) / P;
-    lmax = need <= 4 ? 4 : need <= 8 ? 8 : need <= 16 ? 16 : need <= 32 ? 32 : 0;
-    if (lmax == 0) throw std::runtime_error("line solver: mesh dimension too large (max 8192 points per axis)");
-    bs = P <= 64 ? 64 : P;
+template <class T, int NEU>
+static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
+                            const C2<T> *cf, const Geo &g, const PropParams *pp, hipStream_t st) {
+    const dim3 block(lp.P * W);
+    const size_t lds = ysolve_lds_bytes<T>(lp, W);
+    switch (lp.lmax) {
+        case 4: hipLaunchKernelGGL((k_ysolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
+        case 8: hipLaunchKernelGGL((k_ysolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
+        case 16: hipLaunchKernelGGL((k_ysolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
+        case 32: hipLaunchKernelGGL((k_ysolve<T, 32, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
+        default: throw std::runtime_error("y line solver: unsupported chunk length");
+    }
+    NMK_LAUNCH_CHECK();
 }
 
 template <class T>
-void launch_line_solve(int dir, int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf,
-                       const Geo &g, const PropParams *pp, int nprops, hipStream_t st) {
-    const int M = dir == 0 ? g.Mx : g.My;
-    const int lines = dir == 0 ? g.My : g.Mx;
-    int P, lmax, bs;
-    line_plan(M, P, lmax, bs);
-    dim3 grid(lines, 2, nprops);
-    if (dir == 0) {
-        if (neumann) line_solve_dispatch<T, 0, 1>(lmax, grid, bs, src, dst, acc, cf, g, pp, P, st);
-        else line_solve_dispatch<T, 0, 0>(lmax, grid, bs, src, dst, acc, cf, g, pp, P, st);
-    } else {
-        if (neumann) line_solve_dispatch<T, 1, 1>(lmax, grid, bs, src, dst, acc, cf, g, pp, P, st);
-        else line_solve_dispatch<T, 1, 0>(lmax, grid, bs, src, dst, acc, cf, g, pp, P, st);
-    }
+void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
+                   const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, hipStream_t st) {
+    if (lp.P * R > 512 || lp.M != g.Mx) throw std::runtime_error("x line solver: bad launch shape");
+    dim3 grid((g.My + R - 1) / R, 2, nprops);
+    if (neumann) xsolve_dispatch<T, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, st);
+    else xsolve_dispatch<T, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, st);
+}
+
+template <class T>
+void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
+                   const PropParams *pp, int nprops, const LinePlan &lp, int W, hipStream_t st) {
+    if (lp.P * W > 512 || lp.M != g.My) throw std::runtime_error("y line solver: bad launch shape");
+    dim3 grid((g.Mx + W - 1) / W, 2, nprops);
+    if (neumann) ysolve_dispatch<T, 1>(lp, W, grid, src, dst, acc, cf, g, pp, st);
+    else ysolve_dispatch<T, 0>(lp, W, grid, src, dst, acc, cf, g, pp, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1072,16 +1434,23 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
 
 // explicit instantiations
 #define NMK_INST(T)                                                                                                   \
-    template void launch_coefs<T>(const double *, const Geo &, const SlotParams *, int, C2<T> *, hipStream_t);       \
+    template void launch_coefs<T>(const double *, const Geo &, const SlotParams *, int, C2<T> *, C2<T> *,           \
+                                  const LinePlan &, hipStream_t);                                                     \
     template void launch_init_fields<T>(const double *, const uint8_t *, const Geo &, const SlotParams *,             \
                                         const PropParams *, int, int, double, double, double, C2<T> *, hipStream_t);  \
     template void launch_pack_fields<T>(const c64 *, const Geo &, const PropParams *, int, int, C2<T> *, hipStream_t); \
     template void launch_shift<T>(const C2<T> *, C2<T> *, const Geo &, const PropParams *, int, hipStream_t);        \
     template void launch_phase<T>(C2<T> *, const C2<T> *, const Geo &, const PropParams *, int, hipStream_t);        \
+    template void launch_boundary<T>(const C2<T> *, C2<T> *, const C2<T> *, const C2<T> *, const Geo &,             \
+                                     const PropParams *, int, int, hipStream_t);                                      \
     template void launch_explicit<T>(int, const C2<T> *, C2<T> *, const C2<T> *, const Geo &, const PropParams *,    \
                                      int, hipStream_t);                                                               \
-    template void launch_line_solve<T>(int, int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const Geo &,        \
-                                       const PropParams *, int, hipStream_t);                                        \
+    template void launch_xsolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const C2<T> *, const Geo &, \
+                                   const PropParams *, int, const LinePlan &, int, hipStream_t);                      \
+    template void launch_ysolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const Geo &,                  \
+                                   const PropParams *, int, const LinePlan &, int, hipStream_t);                      \
+    template size_t xsolve_lds_bytes<T>(const LinePlan &, int);                                                       \
+    template size_t ysolve_lds_bytes<T>(const LinePlan &, int);                                                       \
     template void launch_output<T>(const C2<T> *, const double *, const uint8_t *, const Geo &, const SlotParams *,  \
                                    const PropParams *, int, int, c64 *, hipStream_t);
 NMK_INST(float)

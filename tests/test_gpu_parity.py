@@ -145,7 +145,7 @@ def test_iso_layers_focus_and_caller_fields_fp64():
     assert err < TOL64, err
 
 
-@pytest.mark.parametrize("dims", [(15, 12, 5), (3, 20, 4), (18, 1, 4), (21, 10, 3)])
+@pytest.mark.parametrize("dims", [(15, 12, 5), (3, 20, 4), (18, 1, 4), (21, 10, 3), (12, 10, 2), (2, 2, 3)])
 def test_odd_and_degenerate_meshes_fp64(dims):
     s = _base(dims, wavevectors=[(0.08, -0.12)], no="1.5", ne="1.7", na_objective=0.6)
     lc = fields.random_smooth(*dims, (0.1, 0.1, 0.1), seed=11, corr_length=0.3)
