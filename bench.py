@@ -25,7 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FAMILIES = ["material", "coefs", "shift", "phase", "explicit", "solve_x", "solve_y", "output", "filter", "setup"]
+FAMILIES = ["material", "coefs", "boundary", "reserved", "explicit", "solve_x", "solve_y", "output", "filter", "setup"]
 
 
 def log(*a):
@@ -35,10 +35,8 @@ def log(*a):
 def algorithmic_bytes(fam, c, nprops, nslots, NP, Nfull, fd, nw):
     """Unique HBM bytes one launch of a kernel family must move: every array
     element it reads or writes counted once (c = bytes per complex value)."""
-    if fam == "shift":
-        return 4 * nprops * NP * c
-    if fam == "phase":
-        return 4 * nprops * NP * c + 3 * nslots * NP * c
+    if fam == "boundary":
+        return 4 * nprops * NP * c + 6 * nslots * NP * c
     if fam == "explicit":
         return 4 * nprops * NP * c + 20 * nslots * NP * c
     if fam in ("solve_x", "solve_y"):
@@ -59,9 +57,10 @@ def rel_errors(a, b):
     return e.reshape(-1)
 
 
-def run_gpu_leg(js, lc, n_E, shape, precision, wl_batch, use_graph, steps, warmup):
+def run_gpu_leg(js, lc, n_E, shape, precision, wl_batch, use_graph, steps, warmup, tuning=None):
     from nemaktis_amd import bpm_backend
-    with bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, use_graph=use_graph) as ctx:
+    with bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, use_graph=use_graph,
+                             tuning=tuning) as ctx:
         for _ in range(warmup):
             ctx.propagate()
         dev_ms = []
@@ -74,11 +73,12 @@ def run_gpu_leg(js, lc, n_E, shape, precision, wl_batch, use_graph, steps, warmu
         return wall, dev_ms, st, out
 
 
-def run_profile_leg(js, lc, n_E, precision, wl_batch, steps):
+def run_profile_leg(js, lc, n_E, precision, wl_batch, steps, tuning=None):
     from nemaktis_amd import bpm_backend
     fam_ms = {f: 0.0 for f in FAMILIES}
     fam_n = {f: 0 for f in FAMILIES}
-    with bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, profile=True) as ctx:
+    with bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, profile=True,
+                             tuning=tuning) as ctx:
         ctx.propagate()  # warm-up
         for _ in range(steps):
             st = ctx.propagate()
@@ -109,6 +109,7 @@ def main():
     ap.add_argument("--cpu-nz", type=int, default=100, help="planes of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="built-in line-solver shapes instead of the autotuner")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -134,8 +135,14 @@ def main():
     n_E = int(np.prod(shape))
     updates = fields.updates_per_run(s)
 
+    tuning = None
+    if not args.no_tune:
+        from nemaktis_amd import autotune
+        log("autotuning the line-solver shapes (FP32, shallow copy of the job) ...")
+        tuning, _ = autotune.tune(s, lc, precision=32, wl_batch=0, use_cache=False, log=log)
+
     log("FP32 batched path (graph) ...")
-    wall32, dev32, st32, out32 = run_gpu_leg(js, lc, n_E, shape, 32, 0, True, args.steps, args.warmup)
+    wall32, dev32, st32, out32 = run_gpu_leg(js, lc, n_E, shape, 32, 0, True, args.steps, args.warmup, tuning)
     ms32 = 1e3 * wall32 / args.steps
     value = updates * args.steps / wall32
 
@@ -150,7 +157,7 @@ def main():
                 "fp32_rel_l2_max": float(errs.max()), "fp32_rel_l2_median": float(np.median(errs))}
 
     log("profiled pass (kernel-family event timing) ...")
-    fam_ms, fam_n = run_profile_leg(js, lc, n_E, 32, 0, max(1, min(args.steps, 3)))
+    fam_ms, fam_n = run_profile_leg(js, lc, n_E, 32, 0, max(1, min(args.steps, 3)), tuning)
     march_fams = [f for f in FAMILIES if f not in ("setup", "output", "filter") and fam_n[f]]
     dom = max(march_fams, key=lambda f: fam_ms[f])
     avg_ms = fam_ms[dom] / fam_n[dom]
@@ -179,7 +186,7 @@ def main():
         "vs_baseline": None, "dtype": "c64 (FP64 coefficient precompute)", "data": "synthetic",
         "config": {"workload": "random-smooth director 512x512x100, dx=0.1um, lambda=0.55um, no=1.52, "
                                "ne=1.74, normal incidence, 2 polarisations", "updates_per_step": updates,
-                   "device_ms_per_step_median": float(np.median(dev32))},
+                   "device_ms_per_step_median": float(np.median(dev32)), "line_solver_shapes": tuning},
         "fp64_plain": fp64,
         "roofline": roofline,
         "cpu_baseline": cpu,

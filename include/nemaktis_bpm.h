@@ -57,7 +57,12 @@ typedef struct {
     int32_t use_graph;   /* 1: capture the z-march as a hipGraph and replay it */
     int32_t device;      /* HIP device ordinal, -1: current device */
     int32_t profile;     /* 1: time every kernel family (disables the graph) */
-    int32_t reserved[3];
+    int32_t x_chunks;    /* line-solver launch shapes; 0: built-in defaults      */
+    int32_t x_rows;      /*   x lines: chunks per row, rows per block            */
+    int32_t y_chunks;    /*   y lines: chunks per column, columns per block      */
+    int32_t y_cols;
+    int32_t y_stage;     /*   1: y right-hand sides staged through LDS          */
+    int32_t reserved[2];
 } nmk_bpm_options;
 
 typedef struct {
@@ -65,10 +70,14 @@ typedef struct {
     double updates;                                  /* voxel-slice updates of one run */
     double updates_per_s;
     int64_t footprint_bytes;                         /* HBM held by the run */
-    double family_ms[10];                            /* profile mode: per kernel family */
+    /* profile mode: device time per kernel family, indexed 0 material,
+       1 coefficients, 2 slab boundary, 3 reserved, 4 explicit half-step,
+       5 x-line solves, 6 y-line solves, 7 output, 8 screen filter, 9 setup */
+    double family_ms[10];
     int64_t family_launches[10];
 } nmk_bpm_stats;
 
+/* fills the defaults: FP64, one wavelength at a time, eager launches, built-in shapes */
 void nmk_bpm_default_options(nmk_bpm_options *opts);
 
 /* One-shot run with options (mask_vals == NULL selects the unmasked form). */

@@ -18,7 +18,9 @@ c_double_p = ctypes.POINTER(ctypes.c_double)
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("wl_batch", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("device", ctypes.c_int32), ("profile", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("device", ctypes.c_int32), ("profile", ctypes.c_int32), ("x_chunks", ctypes.c_int32),
+                ("x_rows", ctypes.c_int32), ("y_chunks", ctypes.c_int32), ("y_cols", ctypes.c_int32),
+                ("y_stage", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 class Stats(ctypes.Structure):
@@ -28,7 +30,7 @@ class Stats(ctypes.Structure):
                 ("family_launches", ctypes.c_int64 * 10)]
 
     def as_dict(self):
-        names = ["material", "coefs", "shift", "phase", "explicit", "solve_x", "solve_y", "output", "filter", "setup"]
+        names = ["material", "coefs", "boundary", "reserved", "explicit", "solve_x", "solve_y", "output", "filter", "setup"]
         return {
             "setup_ms": self.setup_ms, "march_ms": self.march_ms, "output_ms": self.output_ms,
             "total_ms": self.total_ms, "updates": self.updates, "updates_per_s": self.updates_per_s,
@@ -77,7 +79,8 @@ def dptr(a):
     return None if a is None else a.ctypes.data_as(c_double_p)
 
 
-def options(precision=64, wl_batch=1, use_graph=False, device=-1, profile=False):
+def options(precision=64, wl_batch=1, use_graph=False, device=-1, profile=False, tuning=None):
+    """tuning: None (built-in shapes) or dict(x_chunks, x_rows, y_chunks, y_cols[, y_stage])"""
     o = Options()
     load().nmk_bpm_default_options(ctypes.byref(o))
     o.precision = int(precision)
@@ -85,6 +88,10 @@ def options(precision=64, wl_batch=1, use_graph=False, device=-1, profile=False)
     o.use_graph = 1 if use_graph else 0
     o.device = int(device)
     o.profile = 1 if profile else 0
+    if tuning:
+        o.x_chunks, o.x_rows = int(tuning["x_chunks"]), int(tuning["x_rows"])
+        o.y_chunks, o.y_cols = int(tuning["y_chunks"]), int(tuning["y_cols"])
+        o.y_stage = int(tuning.get("y_stage", 1))
     return o
 
 

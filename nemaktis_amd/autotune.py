@@ -1,0 +1,156 @@
+"""Autotuner of the line-solver launch shapes.
+
+The two cyclic line solvers (x lines and y lines) cut each periodic line into
+P chunks and put several lines in one block (rows for x lines, adjacent
+columns for y lines).  The best shapes depend on the mesh, the precision and
+how many propagations march together, so they are measured: every candidate
+shape that fits the block limits (<= 512 threads, <= 64 KiB LDS, chunks of at
+most 16 points) is timed on a shallow copy of the job (same transverse
+mesh, a few slabs), by coordinate descent (x shape first, then y shape), and
+the winner is cached per (device, precision, mesh, propagation count).
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import bpm_backend, fields
+
+_CACHE_ENV = "NEMAKTIS_AMD_TUNING_CACHE"
+
+
+def _pow2s(lo, hi):
+    v = lo
+    while v <= hi:
+        yield v
+        v *= 2
+
+
+def _plan(M, chunks):
+    cap = max(1, M // 2)
+    P = 1
+    while P * 2 <= min(cap, chunks):
+        P *= 2
+    base, rem = M // P, M % P
+    need = base + (1 if rem else 0)
+    lmax = 4 if need <= 4 else 8 if need <= 8 else 16 if need <= 16 else 32 if need <= 32 else 0
+    return P, lmax
+
+
+def _lds(M, chunks, lines, c, staged=True):
+    P, lmax = _plan(M, chunks)
+    return lines * ((lmax * P * c if staged else 0) + 7 * P * c) + M * 4
+
+
+def x_candidates(Mx, precision):
+    c = 8 if precision == 32 else 16
+    out = []
+    for chunks in _pow2s(16, 512):
+        P, lmax = _plan(Mx, chunks)
+        if lmax == 0 or lmax > 16 or P != chunks:
+            continue
+        for rows in (1, 2, 4, 8):
+            if P * rows <= 512 and _lds(Mx, chunks, rows, c) <= 64 * 1024:
+                out.append((chunks, rows))
+    return out
+
+
+def y_candidates(My, precision):
+    c = 8 if precision == 32 else 16
+    out = []
+    for chunks in _pow2s(16, 512):
+        P, lmax = _plan(My, chunks)
+        if lmax == 0 or lmax > 16 or P != chunks:
+            continue
+        for cols in (1, 2, 4, 8, 16):
+            for stage in (1, 0):
+                if P * cols <= 512 and _lds(My, chunks, cols, c, stage) <= 64 * 1024:
+                    out.append((chunks, cols, stage))
+    return out
+
+
+def _cache_path():
+    p = os.environ.get(_CACHE_ENV)
+    if p:
+        return p
+    return os.path.join(os.path.expanduser("~"), ".cache", "nemaktis_amd", "tuning.json")
+
+
+def _load_cache():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except Exception:
+        return {}
+
+
+def _save_cache(c):
+    p = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        with open(p, "w") as f:
+            json.dump(c, f, indent=1, sort_keys=True)
+    except OSError:
+        pass
+
+
+def cache_key(precision, Nx, Ny, nprops):
+    return "gfx950|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
+
+
+def _time(js, lc, n_E, precision, wl_batch, tuning, reps):
+    with bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, use_graph=True,
+                             tuning=tuning) as ctx:
+        ctx.propagate()
+        t = []
+        for _ in range(reps):
+            t.append(ctx.propagate()["march_ms"])
+    return float(np.median(t))
+
+
+def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache=True, log=None):
+    """Returns the best tuning dict for this job's transverse mesh, measured
+    on a `depth`-plane copy of it, plus the timing table."""
+    ic = settings["Physics settings"]["Initial conditions"]
+    Nx, Ny, Nz = ic["Mesh dimensions"]
+    co = settings["Physics settings"]["Coefficients"]
+    nwl = len(co["Wavelengths"])
+    nprops = (nwl if wl_batch <= 0 else min(wl_batch, nwl)) * (len(co["Wavevectors"]) // 2) * 2
+    key = cache_key(precision, Nx, Ny, nprops)
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        return cache[key]["tuning"], cache[key].get("table", [])
+    depth = max(2, min(depth, Nz))
+    s = json.loads(json.dumps(settings))
+    s["Physics settings"]["Initial conditions"]["Mesh dimensions"] = [Nx, Ny, depth]
+    lc = np.ascontiguousarray(np.asarray(lc_vals).reshape(Nz, Ny, Nx, -1)[:depth])
+    js = json.dumps(s)
+    n_E = int(np.prod(fields.output_shape(s)))
+    Mx, My = (Nx - 1 if Nx > 3 else Nx), (Ny - 1 if Ny > 3 else Ny)
+    xs, ys = x_candidates(Mx, precision), y_candidates(My, precision)
+    if not xs or not ys:
+        return None, []
+    best = {"x_chunks": xs[0][0], "x_rows": xs[0][1], "y_chunks": ys[0][0], "y_cols": ys[0][1], "y_stage": ys[0][2]}
+    table = []
+    t0 = time.time()
+    best_ms = None
+    for (xc, xr) in xs:
+        cand = dict(best, x_chunks=xc, x_rows=xr)
+        ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+        table.append((cand, ms))
+        if best_ms is None or ms < best_ms:
+            best, best_ms = cand, ms
+    for (yc, yw, ystage) in ys:
+        cand = dict(best, y_chunks=yc, y_cols=yw, y_stage=ystage)
+        ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+        table.append((cand, ms))
+        if ms < best_ms:
+            best, best_ms = cand, ms
+    if log:
+        log("autotune %s: %d candidates in %.1f s, best %s (%.3f ms for %d slabs)"
+            % (key, len(table), time.time() - t0, best, best_ms, depth - 1))
+    if use_cache:
+        cache[key] = {"tuning": best, "table": [[c, m] for c, m in table]}
+        _save_cache(cache)
+    return best, table

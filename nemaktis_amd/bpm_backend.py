@@ -65,29 +65,30 @@ def _report(rc, err, strict):
 
 
 def run_backend_without_mask(json_str, lc_field_vals, E_field_vals, *, precision=64, wl_batch=1,
-                             use_graph=False, strict=None, stats=None):
+                             use_graph=False, strict=None, stats=None, tuning=None):
     """Propagates the fields of every (wavelength, wavevector, polarisation)
     through the LC sample and applies the screen filter; E_field_vals
     (N_wavelengths, N_wavevectors, 4, Ny, Nx) complex128 is written in place."""
     _check_E(E_field_vals)
     lc = _lib.as_lc(lc_field_vals)
-    return _run(json_str, None, lc, None, E_field_vals, precision, wl_batch, use_graph, strict, stats)
+    return _run(json_str, None, lc, None, E_field_vals, precision, wl_batch, use_graph, strict, stats, tuning)
 
 
 def run_backend_with_mask(json_str, mask_formula, lc_field_vals, mask_vals, E_field_vals, *, precision=64,
-                          wl_batch=1, use_graph=False, strict=None, stats=None):
+                          wl_batch=1, use_graph=False, strict=None, stats=None, tuning=None):
     """As run_backend_without_mask, with a domain mask (mask_vals >= 0 is LC,
     elsewhere the isotropic host) and, if mask_formula is not empty, the
     interface interpolation weights derived from the formula."""
     _check_E(E_field_vals)
     lc = _lib.as_lc(lc_field_vals)
     mask = _lib.as_lc(mask_vals)
-    return _run(json_str, mask_formula or "", lc, mask, E_field_vals, precision, wl_batch, use_graph, strict, stats)
+    return _run(json_str, mask_formula or "", lc, mask, E_field_vals, precision, wl_batch, use_graph, strict, stats,
+                tuning)
 
 
-def _run(json_str, formula, lc, mask, E, precision, wl_batch, use_graph, strict, stats):
+def _run(json_str, formula, lc, mask, E, precision, wl_batch, use_graph, strict, stats, tuning=None):
     lib = _lib.load()
-    opts = _lib.options(precision=precision, wl_batch=wl_batch, use_graph=use_graph)
+    opts = _lib.options(precision=precision, wl_batch=wl_batch, use_graph=use_graph, tuning=tuning)
     st = _lib.Stats()
     err = _lib.err_buf()
     Ef = E.reshape(-1).view(np.float64)
@@ -121,13 +122,13 @@ class Context:
     loop)."""
 
     def __init__(self, json_str, lc_field_vals, n_E, mask_vals=None, mask_formula="", precision=64, wl_batch=1,
-                 use_graph=False, profile=False, device=-1):
+                 use_graph=False, profile=False, device=-1, tuning=None):
         self.lib = _lib.load()
         lc = _lib.as_lc(lc_field_vals)
         mask = None if mask_vals is None else _lib.as_lc(mask_vals)
         self.n_E = int(n_E)
         opts = _lib.options(precision=precision, wl_batch=wl_batch, use_graph=use_graph, profile=profile,
-                            device=device)
+                            device=device, tuning=tuning)
         err = _lib.err_buf()
         self.h = self.lib.nmk_bpm_create(json_str.encode(), None if mask is None else (mask_formula or "").encode(),
                                          _lib.dptr(lc), lc.size, _lib.dptr(mask), 0 if mask is None else mask.size,

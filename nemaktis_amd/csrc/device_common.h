@@ -48,13 +48,21 @@ template <class T>
 __host__ __device__ __forceinline__ C2<T> conj(C2<T> a) {
     return {a.x, -a.y};
 }
-// 1/a with scaling against overflow
+// 1/a = conj(a)/|a|^2 with one real reciprocal (the line-solver pivots are
+// O(1), far from the overflow range); FP32 uses the hardware reciprocal
+// (1 ulp), FP64 an IEEE division
+__host__ __device__ __forceinline__ float real_rcp(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(v);
+#else
+    return 1.0f / v;
+#endif
+}
+__host__ __device__ __forceinline__ double real_rcp(double v) { return 1.0 / v; }
 template <class T>
 __host__ __device__ __forceinline__ C2<T> crcp(C2<T> a) {
-    T s = (a.x < 0 ? -a.x : a.x) + (a.y < 0 ? -a.y : a.y);
-    T ar = a.x / s, ai = a.y / s;
-    T d = s * (ar * ar + ai * ai);
-    return {ar / d, -ai / d};
+    const T r = real_rcp(a.x * a.x + a.y * a.y);
+    return {a.x * r, -a.y * r};
 }
 template <class T>
 __host__ __device__ __forceinline__ C2<T> cdiv(C2<T> a, C2<T> b) {

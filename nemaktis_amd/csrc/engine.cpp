@@ -462,7 +462,7 @@ private:
 
         C2<T> *cur = fb_[1].ptr, *other = fb_[0].ptr, *A = fb_[2].ptr, *B = fb_[3].ptr;
         HIPCHK(hipStreamWaitEvent(st_, ready[0], 0));
-        timed(KF_SHIFT, [&] { launch_boundary<T>(other, cur, nullptr, CF(0), g_, pp_.ptr, nprops, 1, st_); });
+        timed(KF_BOUNDARY, [&] { launch_boundary<T>(other, cur, nullptr, CF(0), g_, pp_.ptr, nprops, 1, st_); });
         for (int k = 0; k < nsl; k++) {
             // x half-steps: other = (I - mu A_x)^-1 (I + mu A_y) cur by its Neumann series
             if (g_.px) {
@@ -482,12 +482,12 @@ private:
             // y half-steps: cur = (I - mu A_y)^-1 (I + mu A_x) other by its Neumann series
             if (g_.py) {
                 timed(KF_SOLVE_Y, [&] {
-                    launch_ysolve<T>(0, other, A, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, st_);
+                    launch_ysolve<T>(0, other, A, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, tune_.y_stage, st_);
                 });
                 C2<T> *a = A, *b = B;
                 for (int it = 0; it < nw; it++) {
                     timed(KF_SOLVE_Y, [&] {
-                        launch_ysolve<T>(1, a, b, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, st_);
+                        launch_ysolve<T>(1, a, b, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, tune_.y_stage, st_);
                     });
                     std::swap(a, b);
                 }
@@ -497,9 +497,9 @@ private:
             // P_k, Sh | Sh, P_{k+1}
             if (k + 1 < nsl) {
                 HIPCHK(hipStreamWaitEvent(st_, ready[k + 1], 0));
-                timed(KF_SHIFT, [&] { launch_boundary<T>(cur, other, CF(k), CF(k + 1), g_, pp_.ptr, nprops, 2, st_); });
+                timed(KF_BOUNDARY, [&] { launch_boundary<T>(cur, other, CF(k), CF(k + 1), g_, pp_.ptr, nprops, 2, st_); });
             } else {
-                timed(KF_SHIFT, [&] { launch_boundary<T>(cur, other, CF(k), nullptr, g_, pp_.ptr, nprops, 1, st_); });
+                timed(KF_BOUNDARY, [&] { launch_boundary<T>(cur, other, CF(k), nullptr, g_, pp_.ptr, nprops, 1, st_); });
             }
             std::swap(cur, other);
             if (k + 2 < nsl) {
@@ -521,7 +521,7 @@ private:
                 throw std::runtime_error("x line solver tuning exceeds the block limits");
             xt_.alloc(2 * (size_t)sz_.nslots_max * 2 * 3 * g_.My * lpx_.lmax * lpx_.P);
         }
-        if (g_.py && (lpy_.P * t.y_cols > 512 || ysolve_lds_bytes<T>(lpy_, t.y_cols) > 64 * 1024))
+        if (g_.py && (lpy_.P * t.y_cols > 512 || ysolve_lds_bytes<T>(lpy_, t.y_cols, t.y_stage) > 64 * 1024))
             throw std::runtime_error("y line solver tuning exceeds the block limits");
         for (auto &kv : graphs_) {
             (void)hipGraphExecDestroy(kv.second.exec);
@@ -552,7 +552,7 @@ Tuning default_tuning(int precision, int Mx, int My) {
     const size_t c = precision == 32 ? 8 : 16, budget = 64 * 1024;
     auto lds = [&](int M, int chunks, int lines) {
         LinePlan lp = make_line_plan(M, chunks);
-        return (size_t)lines * ((size_t)lp.lmax * lp.P * c + (size_t)lp.P * 7 * c);
+        return (size_t)lines * ((size_t)lp.lmax * lp.P * c + (size_t)lp.P * 7 * c) + (size_t)M * 4;
     };
     // chunks of at most 8 points (16 for long lines)
     t.x_chunks = std::max(64, pow2_at_least((Mx + 15) / 16));
@@ -561,6 +561,7 @@ Tuning default_tuning(int precision, int Mx, int My) {
         while (t.x_rows > 1 && (t.x_chunks * t.x_rows > 512 || lds(Mx, t.x_chunks, t.x_rows) > budget)) t.x_rows /= 2;
     t.y_chunks = std::max(64, pow2_at_least((My + 15) / 16));
     t.y_cols = precision == 32 ? 8 : 4;
+    t.y_stage = 1;
     if (My > 3)
         while (t.y_cols > 1 && (t.y_chunks * t.y_cols > 512 || lds(My, t.y_chunks, t.y_cols) > budget)) t.y_cols /= 2;
     return t;

@@ -17,6 +17,7 @@ namespace nmk {
 struct Tuning {
     int x_chunks = 0, x_rows = 0;  // x lines: chunks per row, rows per block
     int y_chunks = 0, y_cols = 0;  // y lines: chunks per column, columns per block
+    int y_stage = 1;               // y right-hand sides staged through LDS (1) or not (0)
 };
 Tuning default_tuning(int precision, int Mx, int My);
 
@@ -31,7 +32,7 @@ struct Options {
 
 // per-kernel-family device time of one propagate() (profile mode)
 enum KernelFamily {
-    KF_MATERIAL = 0, KF_COEFS, KF_SHIFT, KF_PHASE, KF_EXPLICIT, KF_SOLVE_X, KF_SOLVE_Y, KF_OUTPUT, KF_FILTER,
+    KF_MATERIAL = 0, KF_COEFS, KF_BOUNDARY, KF_RESERVED, KF_EXPLICIT, KF_SOLVE_X, KF_SOLVE_Y, KF_OUTPUT, KF_FILTER,
     KF_SETUP, KF_COUNT
 };
 

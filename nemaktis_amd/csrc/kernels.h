@@ -30,10 +30,6 @@ template <class T>
 void launch_pack_fields(const c64 *E, const Geo &g, const PropParams *pp, int nprops, int nq, C2<T> *F,
                         hipStream_t st);
 
-template <class T>
-void launch_shift(const C2<T> *in, C2<T> *out, const Geo &g, const PropParams *pp, int nprops, hipStream_t st);
-template <class T>
-void launch_phase(C2<T> *f, const C2<T> *cf, const Geo &g, const PropParams *pp, int nprops, hipStream_t st);
 // slab boundary out = Pn Sh^nshift Pp in (cf_prev / cf_next may be null)
 template <class T>
 void launch_boundary(const C2<T> *in, C2<T> *out, const C2<T> *cf_prev, const C2<T> *cf_next, const Geo &g,
@@ -51,13 +47,14 @@ LinePlan make_line_plan(int M, int chunks);
 template <class T>
 size_t xsolve_lds_bytes(const LinePlan &lp, int R);
 template <class T>
-size_t ysolve_lds_bytes(const LinePlan &lp, int W);
+size_t ysolve_lds_bytes(const LinePlan &lp, int W, int stage);
 template <class T>
 void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
                    const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, hipStream_t st);
+// stage: 1 stages the y right-hand sides through LDS, 0 lets the chunk threads evaluate them
 template <class T>
 void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
-                   const PropParams *pp, int nprops, const LinePlan &lp, int W, hipStream_t st);
+                   const PropParams *pp, int nprops, const LinePlan &lp, int W, int stage, hipStream_t st);
 // interior march state -> full-grid complex128 fields with the global phase
 // and the output Fresnel interface
 template <class T>

@@ -415,62 +415,9 @@ void launch_pack_fields(const c64 *E, const Geo &g, const PropParams *pp, int np
     NMK_LAUNCH_CHECK();
 }
 
-// ---------------------------------------------------------------------------
-// Catmull-Rom half-step shift (BeamPropagationLib/SimpleShiftOperator.cpp:52-85):
-// 4 taps along x, then 4 taps along y, on the periodic march grid.  The
-// global phase factor of each half-step is accumulated on the host in FP64
-// and applied once at the output.
-// ---------------------------------------------------------------------------
 __device__ __forceinline__ int wrapm(int i, int M) {
     int r = i % M;
     return r < 0 ? r + M : r;
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void k_shift(const C2<T> *__restrict__ in, C2<T> *__restrict__ out, Geo g,
-                                               const PropParams *__restrict__ pp) {
-    const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ip >= g.NP) return;
-    const int prop = blockIdx.y;
-    const PropParams p = pp[prop];
-    const int ix = (int)(ip % g.Mx), iy = (int)(ip / g.Mx);
-    int cx[4], cy[4];
-    T wx[4], wy[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        cx[k] = wrapm(ix + p.ox + k, g.Mx);
-        cy[k] = wrapm(iy + p.oy + k, g.My);
-        wx[k] = (T)p.wx[k];
-        wy[k] = (T)p.wy[k];
-    }
-    for (int c = 0; c < 2; c++) {
-        const C2<T> *f = in + ((int64_t)prop * 2 + c) * g.NP;
-        auto xpass = [&](int row) -> C2<T> {
-            const C2<T> *r = f + (int64_t)g.Mx * row;
-            if (!g.px) return r[ix];
-            C2<T> a = wx[0] * r[cx[0]];
-            a = a + wx[1] * r[cx[1]];
-            a = a + wx[2] * r[cx[2]];
-            a = a + wx[3] * r[cx[3]];
-            return a;
-        };
-        C2<T> acc;
-        if (g.py) {
-            acc = wy[0] * xpass(cy[0]);
-            acc = acc + wy[1] * xpass(cy[1]);
-            acc = acc + wy[2] * xpass(cy[2]);
-            acc = acc + wy[3] * xpass(cy[3]);
-        } else {
-            acc = xpass(iy);
-        }
-        out[((int64_t)prop * 2 + c) * g.NP + ip] = acc;
-    }
-}
-
-template <class T>
-void launch_shift(const C2<T> *in, C2<T> *out, const Geo &g, const PropParams *pp, int nprops, hipStream_t st) {
-    hipLaunchKernelGGL(k_shift<T>, dim3(blocks_for(g.NP, 256), nprops), dim3(256), 0, st, in, out, g, pp);
-    NMK_LAUNCH_CHECK();
 }
 
 // ---------------------------------------------------------------------------
@@ -567,31 +514,6 @@ void launch_boundary(const C2<T> *in, C2<T> *out, const C2<T> *cf_prev, const C2
                      const PropParams *pp, int nprops, int nshift, hipStream_t st) {
     dim3 grid((g.Mx + kBTW - 1) / kBTW, (g.My + kBTH - 1) / kBTH, nprops);
     hipLaunchKernelGGL(k_boundary<T>, grid, dim3(256), 0, st, in, out, cf_prev, cf_next, g, pp, nshift);
-    NMK_LAUNCH_CHECK();
-}
-
-// ---------------------------------------------------------------------------
-// pointwise 2x2 phase operator (PhaseEvolutionOperator.cpp:14-32)
-// ---------------------------------------------------------------------------
-template <class T>
-__global__ __launch_bounds__(256) void k_phase(C2<T> *__restrict__ f, const C2<T> *__restrict__ cf, Geo g,
-                                               const PropParams *__restrict__ pp) {
-    const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ip >= g.NP) return;
-    const int prop = blockIdx.y;
-    const int slot = pp[prop].slot;
-    const C2<T> *c = cf + ((int64_t)slot * CF_COUNT + CF_P) * g.NP + ip;
-    const C2<T> pxx = c[0], pyy = c[g.NP], pxy = c[2 * g.NP];
-    C2<T> *a = f + ((int64_t)prop * 2) * g.NP + ip;
-    C2<T> *b = a + g.NP;
-    const C2<T> x = *a, y = *b;
-    *a = pxx * x + pxy * y;
-    *b = pxy * x + pyy * y;
-}
-
-template <class T>
-void launch_phase(C2<T> *f, const C2<T> *cf, const Geo &g, const PropParams *pp, int nprops, hipStream_t st) {
-    hipLaunchKernelGGL(k_phase<T>, dim3(blocks_for(g.NP, 256), nprops), dim3(256), 0, st, f, cf, g, pp);
     NMK_LAUNCH_CHECK();
 }
 
@@ -787,11 +709,27 @@ template <class T, int LMAX, class F>
 __device__ __forceinline__ void chunk_eliminate(Chunk<T, LMAX> &ck, int n, F &&load) {
     using C = C2<T>;
     C binv{0, 0}, cprev{0, 0};
+    // FP32: every row of the chunk is fetched before the dependent elimination
+    // chain starts, so its loads overlap; FP64 keeps the registers for the chain
+    constexpr bool kPrefetch = sizeof(T) == 4;
+    C pa[kPrefetch ? LMAX : 1], pb[kPrefetch ? LMAX : 1], pc[kPrefetch ? LMAX : 1], pd[kPrefetch ? LMAX : 1];
+    if (kPrefetch) {
+#pragma unroll
+        for (int i = 0; i < LMAX; i++)
+            if (i < n) load(i, pa[i], pb[i], pc[i], pd[i]);
+    }
 #pragma unroll
     for (int i = 0; i < LMAX; i++) {
         if (i < n) {
             C a, b, c, d;
-            load(i, a, b, c, d);
+            if (kPrefetch) {
+                a = pa[i];
+                b = pb[i];
+                c = pc[i];
+                d = pd[i];
+            } else {
+                load(i, a, b, c, d);
+            }
             if (i == 0) {
                 const C bi = crcp(b);
                 ck.yu[0] = d * bi;
@@ -930,34 +868,37 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     C *sUF = sd + R * SL, *sGF = sUF + R * P, *sHF = sGF + R * P;
     C *sA = sHF + R * P, *sB = sA + R * P, *sC = sB + R * P, *sD = sC + R * P;
     C *sX = sUF;
+    int *smap = reinterpret_cast<int *>(sD + R * P);  // element x -> i*P + p
     const int row0 = blockIdx.x * R, comp = blockIdx.y, prop = blockIdx.z;
+    if ((int)threadIdx.x < P) {
+        const int pc = threadIdx.x, s0c = chunk_start(lp, pc), lc = chunk_len(lp, pc);
+        for (int i = 0; i < lc; i++) smap[s0c + i] = i * P + pc;
+    }
+    __syncthreads();
     const int slot = pp[prop].slot;
     const int64_t NP = g.NP;
     const C *cfs = cf + (int64_t)slot * CF_COUNT * NP;
     const C *s0 = src + ((int64_t)prop * 2) * NP, *s1 = s0 + NP;
 
     {
-        const int ntile = R * M, bs = blockDim.x;
-        for (int e0 = threadIdx.x; e0 < ntile; e0 += 4 * bs) {
-            C d[4];
-            int slotidx[4];
+        const int bs = blockDim.x;
+        for (int r = 0; r < R; r++) {
+            const int y = row0 + r;
+            if (y >= g.My) break;
+            for (int x0 = threadIdx.x; x0 < M; x0 += 4 * bs) {
+                C d[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int e = e0 + u * bs;
-                slotidx[u] = -1;
-                if (e < ntile) {
-                    const int r = e / M, x = e - r * M, y = row0 + r;
-                    if (y < g.My) {
+                for (int u = 0; u < 4; u++) {
+                    const int x = x0 + u * bs;
+                    if (x < M)
                         d[u] = NEU ? rhs_ox<T>(comp, x, y, cfs, s0, s1, g) : rhs_yplus<T>(comp, x, y, cfs, s0, s1, g);
-                        int p, i;
-                        chunk_of(lp, x, p, i);
-                        slotidx[u] = r * SL + i * P + p;
-                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int x = x0 + u * bs;
+                    if (x < M) sd[r * SL + smap[x]] = d[u];
                 }
             }
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (slotidx[u] >= 0) sd[slotidx[u]] = d[u];
         }
     }
     __syncthreads();
@@ -1004,12 +945,11 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     __syncthreads();
 
     C *dd = dst + ((int64_t)prop * 2 + comp) * NP, *aa = acc + ((int64_t)prop * 2 + comp) * NP;
-    for (int e = threadIdx.x; e < R * M; e += blockDim.x) {
-        const int rr = e / M, x = e - rr * M, yy = row0 + rr;
-        if (yy < g.My) {
-            int pc, i;
-            chunk_of(lp, x, pc, i);
-            const C v = sd[rr * SL + i * P + pc];
+    for (int rr = 0; rr < R; rr++) {
+        const int yy = row0 + rr;
+        if (yy >= g.My) break;
+        for (int x = threadIdx.x; x < M; x += blockDim.x) {
+            const C v = sd[rr * SL + smap[x]];
             const int64_t q = x + (int64_t)M * yy;
             dd[q] = v;
             aa[q] = NEU ? aa[q] + v : v;
@@ -1022,15 +962,15 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
 // consecutive columns) and staged in LDS in the chunk-interleaved order
 // ((i*P + p)*W + column); the chunk threads then read the tridiagonal rows
 // from global memory one W-wide row segment at a time.
-template <class T, int LMAX, int NEU>
+template <class T, int LMAX, int NEU, int STAGE>
 __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
                                                  C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf, Geo g,
                                                  const PropParams *__restrict__ pp, LinePlan lp, int W) {
     using C = C2<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int P = lp.P, M = g.My;
-    C *sd = reinterpret_cast<C *>(smem_raw);  // LMAX * P * W
-    C *sUF = sd + LMAX * P * W;
+    C *sd = reinterpret_cast<C *>(smem_raw);  // LMAX * P * W when the right-hand sides are staged
+    C *sUF = sd + (STAGE ? LMAX * P * W : 0);
     C *sGF = sUF + W * P, *sHF = sGF + W * P;
     C *sA = sHF + W * P, *sB = sA + W * P, *sC = sB + W * P, *sD = sC + W * P;
     C *sX = sUF;
@@ -1040,8 +980,9 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     const C *cfs = cf + (int64_t)slot * CF_COUNT * NP;
     const C *s0 = src + ((int64_t)prop * 2) * NP, *s1 = s0 + NP;
 
-    // right-hand sides of the tile, four per thread per round
-    const int ntile = M * W, bs = blockDim.x;
+    // right-hand sides of the tile, four per thread per round (STAGE), or
+    // evaluated by the chunk threads themselves
+    const int ntile = STAGE ? M * W : 0, bs = blockDim.x;
     for (int e0 = threadIdx.x; e0 < ntile; e0 += 4 * bs) {
         C d[4];
         int slotidx[4];
@@ -1081,10 +1022,14 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     };
     const int me = p * W + cofs, nx = ((p + 1) & (P - 1)) * W + cofs;
     Chunk<T, LMAX> ck;
+    auto rhs = [&](int i) -> C {
+        if (STAGE) return sd[(i * P + p) * W + cofs];
+        return NEU ? rhs_oy<T>(comp, x, s + i, cfs, s0, s1, g) : rhs_xplus<T>(comp, x, s + i, cfs, s0, s1, g);
+    };
     if (active) {
         chunk_eliminate(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
             tri(s + i, a, b, c);
-            d = sd[(i * P + p) * W + cofs];
+            d = rhs(i);
         });
         sUF[me] = ck.uf;
         sGF[me] = ck.gf;
@@ -1095,7 +1040,7 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     if (active) {
         C ae, be, ce;
         tri(s + n, ae, be, ce);
-        interface_row(ck, ae, be, ce, sd[(n * P + p) * W + cofs], sUF[nx], sGF[nx], sHF[nx], A, B, Cv, D);
+        interface_row(ck, ae, be, ce, rhs(n), sUF[nx], sGF[nx], sHF[nx], A, B, Cv, D);
     }
     __syncthreads();
     if (active) {
@@ -1136,11 +1081,12 @@ LinePlan make_line_plan(int M, int chunks) {
 
 template <class T>
 size_t xsolve_lds_bytes(const LinePlan &lp, int R) {
-    return (size_t)R * ((size_t)lp.lmax * lp.P * sizeof(C2<T>) + (size_t)lp.P * reduced_lds_per_chunk<T>());
+    return (size_t)R * ((size_t)lp.lmax * lp.P * sizeof(C2<T>) + (size_t)lp.P * reduced_lds_per_chunk<T>()) +
+           (size_t)lp.M * sizeof(int);
 }
 template <class T>
-size_t ysolve_lds_bytes(const LinePlan &lp, int W) {
-    return (size_t)W * ((size_t)lp.lmax * lp.P * sizeof(C2<T>) + (size_t)lp.P * reduced_lds_per_chunk<T>());
+size_t ysolve_lds_bytes(const LinePlan &lp, int W, int stage) {
+    return (size_t)W * ((stage ? (size_t)lp.lmax * lp.P * sizeof(C2<T>) : 0) + (size_t)lp.P * reduced_lds_per_chunk<T>());
 }
 
 template <class T, int NEU>
@@ -1158,16 +1104,15 @@ static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *s
     NMK_LAUNCH_CHECK();
 }
 
-template <class T, int NEU>
+template <class T, int NEU, int STAGE>
 static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
                             const C2<T> *cf, const Geo &g, const PropParams *pp, hipStream_t st) {
     const dim3 block(lp.P * W);
-    const size_t lds = ysolve_lds_bytes<T>(lp, W);
+    const size_t lds = ysolve_lds_bytes<T>(lp, W, STAGE);
     switch (lp.lmax) {
-        case 4: hipLaunchKernelGGL((k_ysolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
-        case 8: hipLaunchKernelGGL((k_ysolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
-        case 16: hipLaunchKernelGGL((k_ysolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
-        case 32: hipLaunchKernelGGL((k_ysolve<T, 32, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
+        case 4: hipLaunchKernelGGL((k_ysolve<T, 4, NEU, STAGE>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
+        case 8: hipLaunchKernelGGL((k_ysolve<T, 8, NEU, STAGE>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
+        case 16: hipLaunchKernelGGL((k_ysolve<T, 16, NEU, STAGE>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
         default: throw std::runtime_error("y line solver: unsupported chunk length");
     }
     NMK_LAUNCH_CHECK();
@@ -1184,11 +1129,16 @@ void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const 
 
 template <class T>
 void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
-                   const PropParams *pp, int nprops, const LinePlan &lp, int W, hipStream_t st) {
+                   const PropParams *pp, int nprops, const LinePlan &lp, int W, int stage, hipStream_t st) {
     if (lp.P * W > 512 || lp.M != g.My) throw std::runtime_error("y line solver: bad launch shape");
     dim3 grid((g.Mx + W - 1) / W, 2, nprops);
-    if (neumann) ysolve_dispatch<T, 1>(lp, W, grid, src, dst, acc, cf, g, pp, st);
-    else ysolve_dispatch<T, 0>(lp, W, grid, src, dst, acc, cf, g, pp, st);
+    if (stage) {
+        if (neumann) ysolve_dispatch<T, 1, 1>(lp, W, grid, src, dst, acc, cf, g, pp, st);
+        else ysolve_dispatch<T, 0, 1>(lp, W, grid, src, dst, acc, cf, g, pp, st);
+    } else {
+        if (neumann) ysolve_dispatch<T, 1, 0>(lp, W, grid, src, dst, acc, cf, g, pp, st);
+        else ysolve_dispatch<T, 0, 0>(lp, W, grid, src, dst, acc, cf, g, pp, st);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1439,8 +1389,6 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
     template void launch_init_fields<T>(const double *, const uint8_t *, const Geo &, const SlotParams *,             \
                                         const PropParams *, int, int, double, double, double, C2<T> *, hipStream_t);  \
     template void launch_pack_fields<T>(const c64 *, const Geo &, const PropParams *, int, int, C2<T> *, hipStream_t); \
-    template void launch_shift<T>(const C2<T> *, C2<T> *, const Geo &, const PropParams *, int, hipStream_t);        \
-    template void launch_phase<T>(C2<T> *, const C2<T> *, const Geo &, const PropParams *, int, hipStream_t);        \
     template void launch_boundary<T>(const C2<T> *, C2<T> *, const C2<T> *, const C2<T> *, const Geo &,             \
                                      const PropParams *, int, int, hipStream_t);                                      \
     template void launch_explicit<T>(int, const C2<T> *, C2<T> *, const C2<T> *, const Geo &, const PropParams *,    \
@@ -1448,9 +1396,9 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
     template void launch_xsolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const C2<T> *, const Geo &, \
                                    const PropParams *, int, const LinePlan &, int, hipStream_t);                      \
     template void launch_ysolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const Geo &,                  \
-                                   const PropParams *, int, const LinePlan &, int, hipStream_t);                      \
+                                   const PropParams *, int, const LinePlan &, int, int, hipStream_t);                 \
     template size_t xsolve_lds_bytes<T>(const LinePlan &, int);                                                       \
-    template size_t ysolve_lds_bytes<T>(const LinePlan &, int);                                                       \
+    template size_t ysolve_lds_bytes<T>(const LinePlan &, int, int);                                                  \
     template void launch_output<T>(const C2<T> *, const double *, const uint8_t *, const Geo &, const SlotParams *,  \
                                    const PropParams *, int, int, c64 *, hipStream_t);
 NMK_INST(float)

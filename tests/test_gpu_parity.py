@@ -199,3 +199,16 @@ def test_random512_reduced_depth_fp32_vs_fp64():
     assert np.isfinite(e64).all()
     err = _rel(e32, e64)
     assert err < TOL32, err
+
+
+@pytest.mark.parametrize("tuning", [
+    {"x_chunks": 16, "x_rows": 8, "y_chunks": 16, "y_cols": 16, "y_stage": 1},
+    {"x_chunks": 32, "x_rows": 1, "y_chunks": 32, "y_cols": 1, "y_stage": 0},
+    {"x_chunks": 64, "x_rows": 2, "y_chunks": 64, "y_cols": 4, "y_stage": 0},
+])
+def test_line_solver_shapes_fp64(tuning):
+    """Every autotuner candidate shape must give the oracle's answer."""
+    s, lc = _random_case((70, 66, 5), seed=17)
+    want = oracle.propagate(s, lc)
+    err = _rel(_gpu(s, lc, tuning=tuning), want)
+    assert err < TOL64, err
