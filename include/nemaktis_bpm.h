@@ -61,7 +61,7 @@ typedef struct {
     int32_t x_rows;      /*   x lines: chunks per row, rows per block            */
     int32_t y_chunks;    /*   y lines: chunks per column, columns per block      */
     int32_t y_cols;
-    int32_t y_stage;     /*   1: y right-hand sides staged through LDS          */
+    int32_t reserved0;   /*   unused, set to 0                                   */
     int32_t reserved[2];
 } nmk_bpm_options;
 

@@ -34,13 +34,16 @@ def _plan(M, chunks):
         P *= 2
     base, rem = M // P, M % P
     need = base + (1 if rem else 0)
-    lmax = 4 if need <= 4 else 8 if need <= 8 else 16 if need <= 16 else 32 if need <= 32 else 0
+    lmax = 4 if need <= 4 else 8 if need <= 8 else 16 if need <= 16 else 0
     return P, lmax
 
 
-def _lds(M, chunks, lines, c, staged=True):
+def _lds(M, chunks, lines, c, pols, smap):
+    """LDS bytes of one line-solver block: the staged right-hand sides and the
+    seven reduced-system arrays of each polarisation held by the block
+    (x blocks hold both, y blocks one), and the x solver's slot map."""
     P, lmax = _plan(M, chunks)
-    return lines * ((lmax * P * c if staged else 0) + 7 * P * c) + M * 4
+    return pols * lines * (lmax * P + 7 * P) * c + (M * 4 if smap else 0)
 
 
 def x_candidates(Mx, precision):
@@ -51,7 +54,7 @@ def x_candidates(Mx, precision):
         if lmax == 0 or lmax > 16 or P != chunks:
             continue
         for rows in (1, 2, 4, 8):
-            if P * rows <= 512 and _lds(Mx, chunks, rows, c) <= 64 * 1024:
+            if 2 * P * rows <= 512 and _lds(Mx, chunks, rows, c, 2, True) <= 64 * 1024:
                 out.append((chunks, rows))
     return out
 
@@ -64,9 +67,8 @@ def y_candidates(My, precision):
         if lmax == 0 or lmax > 16 or P != chunks:
             continue
         for cols in (1, 2, 4, 8, 16):
-            for stage in (1, 0):
-                if P * cols <= 512 and _lds(My, chunks, cols, c, stage) <= 64 * 1024:
-                    out.append((chunks, cols, stage))
+            if P * cols <= 512 and _lds(My, chunks, cols, c, 1, False) <= 64 * 1024:
+                out.append((chunks, cols))
     return out
 
 
@@ -96,7 +98,7 @@ def _save_cache(c):
 
 
 def cache_key(precision, Nx, Ny, nprops):
-    return "gfx950|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
+    return "gfx950|pair-solver|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
 
 
 def _time(js, lc, n_E, precision, wl_batch, tuning, reps):
@@ -131,7 +133,7 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
     xs, ys = x_candidates(Mx, precision), y_candidates(My, precision)
     if not xs or not ys:
         return None, []
-    best = {"x_chunks": xs[0][0], "x_rows": xs[0][1], "y_chunks": ys[0][0], "y_cols": ys[0][1], "y_stage": ys[0][2]}
+    best = {"x_chunks": xs[0][0], "x_rows": xs[0][1], "y_chunks": ys[0][0], "y_cols": ys[0][1]}
     table = []
     t0 = time.time()
     best_ms = None
@@ -141,8 +143,8 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
         table.append((cand, ms))
         if best_ms is None or ms < best_ms:
             best, best_ms = cand, ms
-    for (yc, yw, ystage) in ys:
-        cand = dict(best, y_chunks=yc, y_cols=yw, y_stage=ystage)
+    for (yc, yw) in ys:
+        cand = dict(best, y_chunks=yc, y_cols=yw)
         ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
         table.append((cand, ms))
         if ms < best_ms:

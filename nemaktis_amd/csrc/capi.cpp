@@ -35,7 +35,6 @@ Options to_options(const nmk_bpm_options *o) {
             r.tuning.x_rows = o->x_rows;
             r.tuning.y_chunks = o->y_chunks;
             r.tuning.y_cols = o->y_cols;
-            r.tuning.y_stage = o->y_stage;
         }
     }
     return r;
@@ -109,7 +108,6 @@ void nmk_bpm_default_options(nmk_bpm_options *o) {
     o->wl_batch = 1;
     o->use_graph = 0;
     o->device = -1;
-    o->y_stage = 1;
 }
 
 int nmk_bpm_run(const char *json, const char *formula, const double *lc, int64_t n_lc, const double *mask,

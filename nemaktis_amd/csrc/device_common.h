@@ -141,6 +141,19 @@ enum : int {
     CF_P = 40,     // op_xx, op_yy, op_xy
     CF_COUNT = 43
 };
+// Coefficient planes are stored in tiles of 64 consecutive march-grid
+// points: element (point p, plane k) of a slot sits at
+// ((p / 64) * CF_COUNT + k) * 64 + p % 64, so a wave reads every tap of its
+// points as contiguous segments at offsets fixed at compile time.
+constexpr int kCfTile = 64;
+__host__ __device__ __forceinline__ int64_t cf_slot_size(int64_t NP) {
+    return (NP + kCfTile - 1) / kCfTile * CF_COUNT * kCfTile;
+}
+// offset of plane 0 of point p inside its slot (p < 2^31 / CF_COUNT)
+__host__ __device__ __forceinline__ unsigned cf_point(unsigned p) {
+    return (p / kCfTile) * (unsigned)(CF_COUNT * kCfTile) + (p % kCfTile);
+}
+
 // slab-material plane slots (per full-grid point, FP64)
 enum : int {
     SM_EINV = 0, SM_KXX, SM_KYY, SM_KXY, SM_IXX, SM_IYY, SM_IXY, SM_VX, SM_VY,

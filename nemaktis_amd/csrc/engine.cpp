@@ -155,7 +155,7 @@ public:
         pp_.alloc(sz_.nprops_max);
         partials_.alloc((size_t)kNrefBlocks * sz_.nslots_max);
         sm_.alloc((size_t)sz_.nslots_max * SM_COUNT * sz_.Nfull);
-        cf_.alloc(2 * (size_t)sz_.nslots_max * CF_COUNT * sz_.NP);
+        cf_.alloc(2 * (size_t)sz_.nslots_max * cf_slot_size(sz_.NP));
         if (!o.profile) HIPCHK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
         set_tuning(o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, g_.Mx, g_.My));
         for (auto &b : fb_) b.alloc((size_t)sz_.nprops_max * 2 * sz_.NP);
@@ -436,7 +436,7 @@ private:
     // kernel joins the two streams.
     C2<T> *march_launches(int nslots, int nprops) {
         const int nsl = p_.Nz - 1, nw = p_.n_woodbury;
-        const size_t cfsz = (size_t)sz_.nslots_max * CF_COUNT * sz_.NP;
+        const size_t cfsz = (size_t)sz_.nslots_max * cf_slot_size(sz_.NP);
         const size_t xtsz = xt_.count / 2;
         auto CF = [&](int k) { return cf_.ptr + (size_t)(k % 2) * cfsz; };
         auto XT = [&](int k) { return xt_.ptr ? xt_.ptr + (size_t)(k % 2) * xtsz : nullptr; };
@@ -482,12 +482,12 @@ private:
             // y half-steps: cur = (I - mu A_y)^-1 (I + mu A_x) other by its Neumann series
             if (g_.py) {
                 timed(KF_SOLVE_Y, [&] {
-                    launch_ysolve<T>(0, other, A, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, tune_.y_stage, st_);
+                    launch_ysolve<T>(0, other, A, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, st_);
                 });
                 C2<T> *a = A, *b = B;
                 for (int it = 0; it < nw; it++) {
                     timed(KF_SOLVE_Y, [&] {
-                        launch_ysolve<T>(1, a, b, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, tune_.y_stage, st_);
+                        launch_ysolve<T>(1, a, b, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, st_);
                     });
                     std::swap(a, b);
                 }
@@ -517,11 +517,11 @@ private:
         if (g_.px) lpx_ = make_line_plan(g_.Mx, t.x_chunks);
         if (g_.py) lpy_ = make_line_plan(g_.My, t.y_chunks);
         if (g_.px) {
-            if (lpx_.P * t.x_rows > 512 || xsolve_lds_bytes<T>(lpx_, t.x_rows) > 64 * 1024)
+            if (2 * lpx_.P * t.x_rows > 512 || xsolve_lds_bytes<T>(lpx_, t.x_rows) > 64 * 1024)
                 throw std::runtime_error("x line solver tuning exceeds the block limits");
             xt_.alloc(2 * (size_t)sz_.nslots_max * 2 * 3 * g_.My * lpx_.lmax * lpx_.P);
         }
-        if (g_.py && (lpy_.P * t.y_cols > 512 || ysolve_lds_bytes<T>(lpy_, t.y_cols, t.y_stage) > 64 * 1024))
+        if (g_.py && (lpy_.P * t.y_cols > 512 || ysolve_lds_bytes<T>(lpy_, t.y_cols) > 64 * 1024))
             throw std::runtime_error("y line solver tuning exceeds the block limits");
         for (auto &kv : graphs_) {
             (void)hipGraphExecDestroy(kv.second.exec);
@@ -549,21 +549,24 @@ private:
 
 Tuning default_tuning(int precision, int Mx, int My) {
     Tuning t;
-    const size_t c = precision == 32 ? 8 : 16, budget = 64 * 1024;
-    auto lds = [&](int M, int chunks, int lines) {
-        LinePlan lp = make_line_plan(M, chunks);
-        return (size_t)lines * ((size_t)lp.lmax * lp.P * c + (size_t)lp.P * 7 * c) + (size_t)M * 4;
+    const size_t budget = 64 * 1024;
+    auto xlds = [&](int chunks, int rows) {
+        const LinePlan lp = make_line_plan(Mx, chunks);
+        return precision == 32 ? xsolve_lds_bytes<float>(lp, rows) : xsolve_lds_bytes<double>(lp, rows);
     };
-    // chunks of at most 8 points (16 for long lines)
-    t.x_chunks = std::max(64, pow2_at_least((Mx + 15) / 16));
-    t.x_rows = precision == 32 ? 4 : 2;
+    auto ylds = [&](int chunks, int cols) {
+        const LinePlan lp = make_line_plan(My, chunks);
+        return precision == 32 ? ysolve_lds_bytes<float>(lp, cols) : ysolve_lds_bytes<double>(lp, cols);
+    };
+    // chunks of at most 8 points (16 for lines longer than 1024)
+    t.x_chunks = std::min(128, std::max(16, pow2_at_least((Mx + 7) / 8)));
+    t.x_rows = 4;
     if (Mx > 3)
-        while (t.x_rows > 1 && (t.x_chunks * t.x_rows > 512 || lds(Mx, t.x_chunks, t.x_rows) > budget)) t.x_rows /= 2;
-    t.y_chunks = std::max(64, pow2_at_least((My + 15) / 16));
-    t.y_cols = precision == 32 ? 8 : 4;
-    t.y_stage = 1;
+        while (t.x_rows > 1 && (2 * t.x_chunks * t.x_rows > 512 || xlds(t.x_chunks, t.x_rows) > budget)) t.x_rows /= 2;
+    t.y_chunks = std::min(128, std::max(16, pow2_at_least((My + 7) / 8)));
+    t.y_cols = 8;
     if (My > 3)
-        while (t.y_cols > 1 && (t.y_chunks * t.y_cols > 512 || lds(My, t.y_chunks, t.y_cols) > budget)) t.y_cols /= 2;
+        while (t.y_cols > 1 && (t.y_chunks * t.y_cols > 512 || ylds(t.y_chunks, t.y_cols) > budget)) t.y_cols /= 2;
     return t;
 }
 
@@ -574,7 +577,8 @@ int64_t estimate_footprint(const Problem &p, const Options &o) {
     if (!p.mask.empty()) b += p.voxels();
     b += (int64_t)p.weights.size() * 8;
     b += (int64_t)s.nslots_max * SM_COUNT * s.Nfull * 8;
-    b += (int64_t)s.nslots_max * CF_COUNT * s.NP * fsz;
+    b += 2 * (int64_t)s.nslots_max * cf_slot_size(s.NP) * fsz;                 // double-buffered coefficients
+    b += 2 * (int64_t)s.nslots_max * 6 * (p.Ny) * 2 * ((int64_t)p.Nx + 64) * fsz;  // x-line rows (upper bound)
     b += 4 * (int64_t)s.nprops_max * 2 * s.NP * fsz;
     b += (int64_t)s.nwl * s.nq * 4 * s.Nfull * 16;
     return b;

@@ -17,7 +17,6 @@ namespace nmk {
 struct Tuning {
     int x_chunks = 0, x_rows = 0;  // x lines: chunks per row, rows per block
     int y_chunks = 0, y_cols = 0;  // y lines: chunks per column, columns per block
-    int y_stage = 1;               // y right-hand sides staged through LDS (1) or not (0)
 };
 Tuning default_tuning(int precision, int Mx, int My);
 

@@ -38,7 +38,9 @@ void launch_boundary(const C2<T> *in, C2<T> *out, const C2<T> *cf_prev, const C2
 template <class T>
 void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, const Geo &g, const PropParams *pp,
                      int nprops, hipStream_t st);
-// cyclic block-tridiagonal line solves of the implicit half-steps:
+// cyclic block-tridiagonal line solves of the implicit half-steps, one
+// thread per chunk carrying both polarisations (nprops must be even, the
+// propagations ordered pol + 2 * (wavevector + n_wavevectors * slot)):
 //   x lines, neumann == 0: dst = M_x (I + mu A_y) src, acc = dst
 //   x lines, neumann == 1: dst = M_x (mu O_x src),     acc += dst
 //   y lines, neumann == 0: dst = M_y (I + mu A_x) src, acc = dst
@@ -47,14 +49,13 @@ LinePlan make_line_plan(int M, int chunks);
 template <class T>
 size_t xsolve_lds_bytes(const LinePlan &lp, int R);
 template <class T>
-size_t ysolve_lds_bytes(const LinePlan &lp, int W, int stage);
+size_t ysolve_lds_bytes(const LinePlan &lp, int W);
 template <class T>
 void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
                    const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, hipStream_t st);
-// stage: 1 stages the y right-hand sides through LDS, 0 lets the chunk threads evaluate them
 template <class T>
 void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
-                   const PropParams *pp, int nprops, const LinePlan &lp, int W, int stage, hipStream_t st);
+                   const PropParams *pp, int nprops, const LinePlan &lp, int W, hipStream_t st);
 // interior march state -> full-grid complex128 fields with the global phase
 // and the output Fresnel interface
 template <class T>

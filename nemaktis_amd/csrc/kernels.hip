@@ -3,7 +3,7 @@
 // Data layout in HBM
 //   director     lc[c + fd*(ix + Nx*(iy + Ny*iz))]   float64 (caller layout)
 //   slab material sm[(slot*SM_COUNT + c)*Nfull + m]   float64, full grid
-//   coefficients cf[(slot*CF_COUNT + c)*NP + p]       C2<T>, march grid
+//   coefficients cf[slot*cf_slot_size(NP) + cf_point(p) + c*64]  C2<T>, march grid
 //   march state  F[(prop*2 + comp)*NP + p]            C2<T>, march grid
 //   output       E[(2*prop + comp)*Nfull + m]         complex128, full grid
 // with m = ix + Nx*iy on the full grid and p = jx + Mx*jy on the march grid
@@ -327,12 +327,12 @@ __global__ __launch_bounds__(256) void k_coefs(const double *__restrict__ sm, Ge
     }
     // store mu*R with mu = i*dZ/2
     const double hz = 0.5 * s.dZ;
-    C2<T> *o = cf + (int64_t)slot * CF_COUNT * g.NP + ip;
+    C2<T> *o = cf + (size_t)slot * cf_slot_size(g.NP) + cf_point((unsigned)ip);
 #pragma unroll
-    for (int k = 0; k < 40; k++) o[(int64_t)k * g.NP] = C2<T>{(T)(-hz * im[k]), (T)(hz * re[k])};
-    o[(int64_t)(CF_P + 0) * g.NP] = C2<T>{(T)at(SM_PXX_RE, ix, iy), (T)at(SM_PXX_IM, ix, iy)};
-    o[(int64_t)(CF_P + 1) * g.NP] = C2<T>{(T)at(SM_PYY_RE, ix, iy), (T)at(SM_PYY_IM, ix, iy)};
-    o[(int64_t)(CF_P + 2) * g.NP] = C2<T>{(T)at(SM_PXY_RE, ix, iy), (T)at(SM_PXY_IM, ix, iy)};
+    for (int k = 0; k < 40; k++) o[k * kCfTile] = C2<T>{(T)(-hz * im[k]), (T)(hz * re[k])};
+    o[(CF_P + 0) * kCfTile] = C2<T>{(T)at(SM_PXX_RE, ix, iy), (T)at(SM_PXX_IM, ix, iy)};
+    o[(CF_P + 1) * kCfTile] = C2<T>{(T)at(SM_PYY_RE, ix, iy), (T)at(SM_PYY_IM, ix, iy)};
+    o[(CF_P + 2) * kCfTile] = C2<T>{(T)at(SM_PXY_RE, ix, iy), (T)at(SM_PXY_IM, ix, iy)};
     // (I - mu T) rows of the x-line systems (T = DX00, DX11) in the chunk-interleaved layout
     if (g.px && xt) {
         int pc, i;
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, 
     const int ox = g.px ? (nshift == 2 ? p.o2x : p.o1x) : 0;
     const int oy = g.py ? (nshift == 2 ? p.o2y : p.o1y) : 0;
     const C *f0 = in + ((int64_t)prop * 2) * NP, *f1 = f0 + NP;
-    const C *cp = cf_prev ? cf_prev + ((int64_t)p.slot * CF_COUNT + CF_P) * NP : nullptr;
+    const C *cp = cf_prev ? cf_prev + (size_t)p.slot * cf_slot_size(NP) + CF_P * kCfTile : nullptr;
     auto wrapx = [&](int i) { return g.px ? wrapm(i, g.Mx) : min(max(i, 0), g.Mx - 1); };
     auto wrapy = [&](int i) { return g.py ? wrapm(i, g.My) : min(max(i, 0), g.My - 1); };
     // input window with the previous slab's phase operator applied
@@ -455,7 +455,8 @@ __global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, 
         const int64_t q = wrapx(x0 + ox + c) + (int64_t)g.Mx * wrapy(y0 + oy + r);
         C a = f0[q], b = f1[q];
         if (cp) {
-            const C pxx = cp[q], pyy = cp[NP + q], pxy = cp[2 * NP + q];
+            const C *c3 = cp + cf_point((unsigned)q);
+            const C pxx = c3[0], pyy = c3[kCfTile], pxy = c3[2 * kCfTile];
             const C na = pxx * a + pxy * b, nb = pxy * a + pyy * b;
             a = na;
             b = nb;
@@ -497,8 +498,8 @@ __global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, 
         }
         const int64_t q = x + (int64_t)g.Mx * y;
         if (cf_next) {
-            const C *cn = cf_next + ((int64_t)p.slot * CF_COUNT + CF_P) * NP;
-            const C pxx = cn[q], pyy = cn[NP + q], pxy = cn[2 * NP + q];
+            const C *cn = cf_next + (size_t)p.slot * cf_slot_size(NP) + CF_P * kCfTile + cf_point((unsigned)q);
+            const C pxx = cn[0], pyy = cn[kCfTile], pxy = cn[2 * kCfTile];
             const C a = pxx * v[0] + pxy * v[1], b = pxy * v[0] + pyy * v[1];
             v[0] = a;
             v[1] = b;
@@ -518,83 +519,260 @@ void launch_boundary(const C2<T> *in, C2<T> *out, const C2<T> *cf_prev, const C2
 }
 
 // ---------------------------------------------------------------------------
-// explicit half-steps (ParaxialPrimaryEvolutionOperator.cpp:153-160, 181-188)
-//   dir 1: out = in + mu A_y in,  A_y = [DY00 DY01; DY10+XY10 DY11+XY11]
-//   dir 0: out = in + mu A_x in,  A_x = [DX00+XY00 DX01+XY01; DX10 DX11]
+// transverse operator taps (ParaxialPrimaryEvolutionOperator.cpp:102-210).
+// Every right-hand side of the z-march is a field value plus a sum of
+// (mu*R coefficient) x (neighbouring field value) taps:
+//   yplus  (I + mu A_y) X,  A_y = [DY00 DY01; DY10+XY10 DY11+XY11]
+//   xplus  (I + mu A_x) S,  A_x = [DX00+XY00 DX01+XY01; DX10 DX11]
+//   ox     mu O_x S,        O_x = [XY00  DX01+XY01; DX10  0]
+//   oy     mu O_y S,        O_y = [0  DY01; DY10+XY10  XY11]
+// (mu is folded into the stored coefficients).  A tap is (coefficient plane,
+// field component, dx, dy); D? stencils are {minus, centre, plus} along their
+// axis and XY stencils {(-,-), (+,-), (-,+), (+,+)}.
 // ---------------------------------------------------------------------------
-struct Nbr {
-    int xm, xp, ym, yp;
+enum RhsKind : int { RHS_YPLUS = 0, RHS_XPLUS = 1, RHS_OX = 2, RHS_OY = 3 };
+
+struct Tap {
+    int k, fc, dx, dy;
 };
-__device__ __forceinline__ Nbr neighbours(const Geo &g, int ix, int iy) {
+struct TapList {
+    int n, centre;  // centre: field component added with weight 1, or -1
+    Tap t[14];
+};
+
+__host__ __device__ constexpr void push_d(TapList &L, int base, int blk, int fc, bool along_x) {
+    for (int s = 0; s < 3; s++) {
+        L.t[L.n] = along_x ? Tap{base + blk * 3 + s, fc, s - 1, 0} : Tap{base + blk * 3 + s, fc, 0, s - 1};
+        L.n++;
+    }
+}
+__host__ __device__ constexpr void push_xy(TapList &L, int blk, int fc) {
+    constexpr int dx[4] = {-1, 1, -1, 1}, dy[4] = {-1, -1, 1, 1};
+    for (int s = 0; s < 4; s++) {
+        L.t[L.n] = Tap{CF_XY + blk * 4 + s, fc, dx[s], dy[s]};
+        L.n++;
+    }
+}
+__host__ __device__ constexpr TapList rhs_taps(int kind, int comp) {
+    TapList L{0, -1, {}};
+    if (kind == RHS_YPLUS) {
+        L.centre = comp;
+        if (comp == 0) {
+            push_d(L, CF_DY, 0, 0, false);
+            push_d(L, CF_DY, 1, 1, false);
+        } else {
+            push_d(L, CF_DY, 2, 0, false);
+            push_xy(L, 2, 0);
+            push_d(L, CF_DY, 3, 1, false);
+            push_xy(L, 3, 1);
+        }
+    } else if (kind == RHS_XPLUS) {
+        L.centre = comp;
+        if (comp == 0) {
+            push_d(L, CF_DX, 0, 0, true);
+            push_xy(L, 0, 0);
+            push_d(L, CF_DX, 1, 1, true);
+            push_xy(L, 1, 1);
+        } else {
+            push_d(L, CF_DX, 2, 0, true);
+            push_d(L, CF_DX, 3, 1, true);
+        }
+    } else if (kind == RHS_OX) {
+        if (comp == 0) {
+            push_xy(L, 0, 0);
+            push_d(L, CF_DX, 1, 1, true);
+            push_xy(L, 1, 1);
+        } else {
+            push_d(L, CF_DX, 2, 0, true);
+        }
+    } else {
+        if (comp == 0) {
+            push_d(L, CF_DY, 1, 1, false);
+        } else {
+            push_d(L, CF_DY, 2, 0, false);
+            push_xy(L, 2, 0);
+            push_xy(L, 3, 1);
+        }
+    }
+    return L;
+}
+
+struct Nbr {
+    unsigned xm, xp, ym, yp;
+};
+__device__ __forceinline__ Nbr neighbours(const Geo &g, unsigned ix, unsigned iy) {
     Nbr n;
     n.xm = g.px ? (ix == 0 ? g.Mx - 1 : ix - 1) : ix;
-    n.xp = g.px ? (ix == g.Mx - 1 ? 0 : ix + 1) : ix;
+    n.xp = g.px ? (ix == (unsigned)g.Mx - 1 ? 0 : ix + 1) : ix;
     n.ym = g.py ? (iy == 0 ? g.My - 1 : iy - 1) : iy;
-    n.yp = g.py ? (iy == g.My - 1 ? 0 : iy + 1) : iy;
+    n.yp = g.py ? (iy == (unsigned)g.My - 1 ? 0 : iy + 1) : iy;
     return n;
 }
 
-// sum over a 3-point x stencil of coefficient block `c0` applied to field f
+// Gathered operands of one right-hand side: every coefficient and field
+// value of the point is loaded before any arithmetic, so that the loads of a
+// point overlap in one memory round trip.  Indices are 32-bit element
+// offsets from uniform plane pointers.
+template <class T, int KIND, int COMP>
+struct RhsGather {
+    static constexpr int N = rhs_taps(KIND, COMP).n;
+    static constexpr int CTR = rhs_taps(KIND, COMP).centre;
+    C2<T> c[N], f[N], ctr;
+
+    // tap I and the following ones; the tap table is evaluated at compile time
+    template <int I>
+    __device__ __forceinline__ void load_taps(const C2<T> *__restrict__ pc, const C2<T> *__restrict__ f0,
+                                              const C2<T> *__restrict__ f1, const Nbr &n, unsigned jx, unsigned jy,
+                                              unsigned Mx) {
+        constexpr Tap tp = rhs_taps(KIND, COMP).t[I];
+        const unsigned col = tp.dx < 0 ? n.xm : tp.dx > 0 ? n.xp : jx;
+        const unsigned row = tp.dy < 0 ? n.ym : tp.dy > 0 ? n.yp : jy;
+        c[I] = pc[tp.k * kCfTile];
+        f[I] = (tp.fc == 0 ? f0 : f1)[col + Mx * row];
+        if constexpr (I + 1 < N) load_taps<I + 1>(pc, f0, f1, n, jx, jy, Mx);
+    }
+
+    __device__ __forceinline__ void load(unsigned jx, unsigned jy, const C2<T> *__restrict__ cfs,
+                                         const C2<T> *__restrict__ f0, const C2<T> *__restrict__ f1, const Geo &g,
+                                         unsigned NP) {
+        (void)NP;
+        const Nbr n = neighbours(g, jx, jy);
+        const unsigned Mx = g.Mx, ip = jx + Mx * jy;
+        load_taps<0>(cfs + cf_point(ip), f0, f1, n, jx, jy, Mx);
+        if constexpr (CTR >= 0) ctr = (CTR == 0 ? f0 : f1)[ip];
+    }
+    __device__ __forceinline__ C2<T> sum() const {
+        C2<T> a = C2<T>{0, 0};
+        if constexpr (CTR >= 0) a = ctr;
+#pragma unroll
+        for (int t = 0; t < N; t++) a = cmad(a, c[t], f[t]);
+        return a;
+    }
+};
+
+// Both polarisations of one (wavelength, wavevector) read the same
+// coefficients: one gather loads the coefficient taps once and the field
+// taps of both.  a0/a1: components of polarisation 0, b0/b1: polarisation 1.
+template <class T, int KIND, int COMP>
+struct RhsGather2 {
+    static constexpr int N = rhs_taps(KIND, COMP).n;
+    static constexpr int CTR = rhs_taps(KIND, COMP).centre;
+    C2<T> c[N], fa[N], fb[N], ca, cb;
+
+    template <int I>
+    __device__ __forceinline__ void load_taps(const C2<T> *__restrict__ pc, const C2<T> *__restrict__ a0,
+                                              const C2<T> *__restrict__ a1, const C2<T> *__restrict__ b0,
+                                              const C2<T> *__restrict__ b1, const Nbr &n, unsigned jx, unsigned jy,
+                                              unsigned Mx) {
+        constexpr Tap tp = rhs_taps(KIND, COMP).t[I];
+        const unsigned col = tp.dx < 0 ? n.xm : tp.dx > 0 ? n.xp : jx;
+        const unsigned row = tp.dy < 0 ? n.ym : tp.dy > 0 ? n.yp : jy;
+        const unsigned q = col + Mx * row;
+        c[I] = pc[tp.k * kCfTile];
+        fa[I] = (tp.fc == 0 ? a0 : a1)[q];
+        fb[I] = (tp.fc == 0 ? b0 : b1)[q];
+        if constexpr (I + 1 < N) load_taps<I + 1>(pc, a0, a1, b0, b1, n, jx, jy, Mx);
+    }
+    __device__ __forceinline__ void load(unsigned jx, unsigned jy, const C2<T> *__restrict__ cfs,
+                                         const C2<T> *__restrict__ a0, const C2<T> *__restrict__ a1,
+                                         const C2<T> *__restrict__ b0, const C2<T> *__restrict__ b1, const Geo &g) {
+        const Nbr n = neighbours(g, jx, jy);
+        const unsigned Mx = g.Mx, ip = jx + Mx * jy;
+        load_taps<0>(cfs + cf_point(ip), a0, a1, b0, b1, n, jx, jy, Mx);
+        if constexpr (CTR >= 0) {
+            ca = (CTR == 0 ? a0 : a1)[ip];
+            cb = (CTR == 0 ? b0 : b1)[ip];
+        }
+    }
+    __device__ __forceinline__ void sum(C2<T> &ra, C2<T> &rb) const {
+        C2<T> x{0, 0}, y{0, 0};
+        if constexpr (CTR >= 0) {
+            x = ca;
+            y = cb;
+        }
+#pragma unroll
+        for (int t = 0; t < N; t++) {
+            x = cmad(x, c[t], fa[t]);
+            y = cmad(y, c[t], fb[t]);
+        }
+        ra = x;
+        rb = y;
+    }
+};
+
+// field planes of one polarisation pair: prop0 = 2 * pair
 template <class T>
-__device__ __forceinline__ C2<T> sx3(C2<T> acc, const C2<T> *c, int64_t NP, int k0, const C2<T> *f, int64_t row,
-                                     const Nbr &n, int ix) {
-    acc = cmad(acc, c[(int64_t)(k0 + 0) * NP], f[row + n.xm]);
-    acc = cmad(acc, c[(int64_t)(k0 + 1) * NP], f[row + ix]);
-    acc = cmad(acc, c[(int64_t)(k0 + 2) * NP], f[row + n.xp]);
-    return acc;
-}
+struct PairFields {
+    const C2<T> *a0, *a1, *b0, *b1;
+    __device__ __forceinline__ PairFields(const C2<T> *src, int prop0, size_t NP) {
+        a0 = src + (size_t)prop0 * 2 * NP;
+        a1 = a0 + NP;
+        b0 = a1 + NP;
+        b1 = b0 + NP;
+    }
+};
+
+// points per thread per round of the right-hand-side gathers: two in FP32,
+// one in FP64 (register budget)
 template <class T>
-__device__ __forceinline__ C2<T> sy3(C2<T> acc, const C2<T> *c, int64_t NP, int k0, const C2<T> *f, int Mx,
-                                     const Nbr &n, int ix, int iy) {
-    acc = cmad(acc, c[(int64_t)(k0 + 0) * NP], f[ix + (int64_t)Mx * n.ym]);
-    acc = cmad(acc, c[(int64_t)(k0 + 1) * NP], f[ix + (int64_t)Mx * iy]);
-    acc = cmad(acc, c[(int64_t)(k0 + 2) * NP], f[ix + (int64_t)Mx * n.yp]);
-    return acc;
-}
-template <class T>
-__device__ __forceinline__ C2<T> sxy4(C2<T> acc, const C2<T> *c, int64_t NP, int k0, const C2<T> *f, int Mx,
-                                      const Nbr &n) {
-    acc = cmad(acc, c[(int64_t)(k0 + 0) * NP], f[n.xm + (int64_t)Mx * n.ym]);
-    acc = cmad(acc, c[(int64_t)(k0 + 1) * NP], f[n.xp + (int64_t)Mx * n.ym]);
-    acc = cmad(acc, c[(int64_t)(k0 + 2) * NP], f[n.xm + (int64_t)Mx * n.yp]);
-    acc = cmad(acc, c[(int64_t)(k0 + 3) * NP], f[n.xp + (int64_t)Mx * n.yp]);
-    return acc;
+constexpr int rhs_points() {
+    return sizeof(T) == 4 ? 2 : 1;
 }
 
+// right-hand sides of component `comp` (uniform per block), both
+// polarisations, at one point or (FP32) two; every load is issued before any sum
+template <class T, int KIND, int COMP>
+__device__ __forceinline__ void rhs_pair_comp(unsigned xa, unsigned ya, unsigned xb, unsigned yb, bool vb,
+                                              const C2<T> *__restrict__ cfs, const PairFields<T> &f, const Geo &g,
+                                              C2<T> (&ra)[2], C2<T> (&rb)[2]) {
+    RhsGather2<T, KIND, COMP> a;
+    a.load(xa, ya, cfs, f.a0, f.a1, f.b0, f.b1, g);
+    if constexpr (rhs_points<T>() == 2) {
+        RhsGather2<T, KIND, COMP> b;
+        if (vb) b.load(xb, yb, cfs, f.a0, f.a1, f.b0, f.b1, g);
+        __builtin_amdgcn_sched_barrier(0);
+        a.sum(ra[0], ra[1]);
+        if (vb) b.sum(rb[0], rb[1]);
+    } else {
+        (void)xb;
+        (void)yb;
+        (void)vb;
+        (void)rb;
+        __builtin_amdgcn_sched_barrier(0);
+        a.sum(ra[0], ra[1]);
+    }
+}
+template <class T, int KIND>
+__device__ __forceinline__ void rhs_pair2(int comp, unsigned xa, unsigned ya, unsigned xb, unsigned yb, bool vb,
+                                          const C2<T> *__restrict__ cfs, const PairFields<T> &f, const Geo &g,
+                                          C2<T> (&ra)[2], C2<T> (&rb)[2]) {
+    if (comp == 0) rhs_pair_comp<T, KIND, 0>(xa, ya, xb, yb, vb, cfs, f, g, ra, rb);
+    else rhs_pair_comp<T, KIND, 1>(xa, ya, xb, yb, vb, cfs, f, g, ra, rb);
+}
+
+// explicit half-steps, used along an axis too short for line solves
+// (ParaxialPrimaryEvolutionOperator.cpp:153-160, 181-188):
+//   dir 1: out = (I + mu A_y) in,   dir 0: out = (I + mu A_x) in
 template <class T, int DIR>
 __global__ __launch_bounds__(256) void k_explicit(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
                                                   const C2<T> *__restrict__ cf, Geo g,
                                                   const PropParams *__restrict__ pp) {
-    const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ip >= g.NP) return;
+    const unsigned ip = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= (unsigned)g.NP) return;
     const int prop = blockIdx.y;
     const int slot = pp[prop].slot;
-    const int ix = (int)(ip % g.Mx), iy = (int)(ip / g.Mx);
-    const Nbr n = neighbours(g, ix, iy);
-    const int64_t NP = g.NP;
-    const C2<T> *c = cf + (int64_t)slot * CF_COUNT * NP + ip;
-    const C2<T> *f0 = in + ((int64_t)prop * 2) * NP;
-    const C2<T> *f1 = f0 + NP;
-    const int64_t row = (int64_t)g.Mx * iy;
-    C2<T> a0{0, 0}, a1{0, 0};
-    if (DIR == 1) {
-        a0 = sy3(a0, c, NP, CF_DY + 0 * 3, f0, g.Mx, n, ix, iy);
-        a0 = sy3(a0, c, NP, CF_DY + 1 * 3, f1, g.Mx, n, ix, iy);
-        a1 = sy3(a1, c, NP, CF_DY + 3 * 3, f1, g.Mx, n, ix, iy);
-        a1 = sxy4(a1, c, NP, CF_XY + 3 * 4, f1, g.Mx, n);
-        a1 = sy3(a1, c, NP, CF_DY + 2 * 3, f0, g.Mx, n, ix, iy);
-        a1 = sxy4(a1, c, NP, CF_XY + 2 * 4, f0, g.Mx, n);
-    } else {
-        a0 = sx3(a0, c, NP, CF_DX + 0 * 3, f0, row, n, ix);
-        a0 = sxy4(a0, c, NP, CF_XY + 0 * 4, f0, g.Mx, n);
-        a0 = sx3(a0, c, NP, CF_DX + 1 * 3, f1, row, n, ix);
-        a0 = sxy4(a0, c, NP, CF_XY + 1 * 4, f1, g.Mx, n);
-        a1 = sx3(a1, c, NP, CF_DX + 3 * 3, f1, row, n, ix);
-        a1 = sx3(a1, c, NP, CF_DX + 2 * 3, f0, row, n, ix);
-    }
-    C2<T> *o0 = out + ((int64_t)prop * 2) * NP;
-    o0[ip] = f0[ip] + a0;
-    o0[NP + ip] = f1[ip] + a1;
+    const unsigned NP = (unsigned)g.NP, ix = ip % (unsigned)g.Mx, iy = ip / (unsigned)g.Mx;
+    const C2<T> *c = cf + (size_t)slot * cf_slot_size(NP);
+    const C2<T> *f0 = in + ((size_t)prop * 2) * NP, *f1 = f0 + NP;
+    constexpr int KIND = DIR == 1 ? RHS_YPLUS : RHS_XPLUS;
+    RhsGather<T, KIND, 0> a;
+    RhsGather<T, KIND, 1> b;
+    a.load(ix, iy, c, f0, f1, g, NP);
+    b.load(ix, iy, c, f0, f1, g, NP);
+    __builtin_amdgcn_sched_barrier(0);
+    C2<T> *o0 = out + ((size_t)prop * 2) * NP;
+    o0[ip] = a.sum();
+    o0[NP + ip] = b.sum();
 }
 
 template <class T>
@@ -607,92 +785,14 @@ void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, cons
 }
 
 // ---------------------------------------------------------------------------
-// right-hand sides of the implicit half-steps, evaluated on the fly
-// (ParaxialPrimaryEvolutionOperator.cpp:153-207):
-//   x lines, first solve:  (I + mu A_y) X,   A_y = [DY00 DY01; DY10+XY10 DY11+XY11]
-//   x lines, Neumann:      mu O_x S,         O_x = [XY00  DX01+XY01; DX10  0]
-//   y lines, first solve:  (I + mu A_x) S,   A_x = [DX00+XY00 DX01+XY01; DX10 DX11]
-//   y lines, Neumann:      mu O_y S,         O_y = [0  DY01; DY10+XY10  XY11]
-// (mu is folded into the stored coefficients)
-// ---------------------------------------------------------------------------
-template <class T>
-__device__ __forceinline__ C2<T> rhs_yplus(int comp, int jx, int jy, const C2<T> *__restrict__ cfs,
-                                           const C2<T> *__restrict__ f0, const C2<T> *__restrict__ f1, const Geo &g) {
-    const int64_t NP = g.NP, ip = jx + (int64_t)g.Mx * jy;
-    const C2<T> *c = cfs + ip;
-    const Nbr n = neighbours(g, jx, jy);
-    C2<T> a{0, 0};
-    if (comp == 0) {
-        a = sy3(a, c, NP, CF_DY + 0 * 3, f0, g.Mx, n, jx, jy);
-        a = sy3(a, c, NP, CF_DY + 1 * 3, f1, g.Mx, n, jx, jy);
-        return f0[ip] + a;
-    }
-    a = sy3(a, c, NP, CF_DY + 3 * 3, f1, g.Mx, n, jx, jy);
-    a = sxy4(a, c, NP, CF_XY + 3 * 4, f1, g.Mx, n);
-    a = sy3(a, c, NP, CF_DY + 2 * 3, f0, g.Mx, n, jx, jy);
-    a = sxy4(a, c, NP, CF_XY + 2 * 4, f0, g.Mx, n);
-    return f1[ip] + a;
-}
-
-template <class T>
-__device__ __forceinline__ C2<T> rhs_xplus(int comp, int jx, int jy, const C2<T> *__restrict__ cfs,
-                                           const C2<T> *__restrict__ f0, const C2<T> *__restrict__ f1, const Geo &g) {
-    const int64_t NP = g.NP, ip = jx + (int64_t)g.Mx * jy, row = (int64_t)g.Mx * jy;
-    const C2<T> *c = cfs + ip;
-    const Nbr n = neighbours(g, jx, jy);
-    C2<T> a{0, 0};
-    if (comp == 0) {
-        a = sx3(a, c, NP, CF_DX + 0 * 3, f0, row, n, jx);
-        a = sxy4(a, c, NP, CF_XY + 0 * 4, f0, g.Mx, n);
-        a = sx3(a, c, NP, CF_DX + 1 * 3, f1, row, n, jx);
-        a = sxy4(a, c, NP, CF_XY + 1 * 4, f1, g.Mx, n);
-        return f0[ip] + a;
-    }
-    a = sx3(a, c, NP, CF_DX + 3 * 3, f1, row, n, jx);
-    a = sx3(a, c, NP, CF_DX + 2 * 3, f0, row, n, jx);
-    return f1[ip] + a;
-}
-
-template <class T>
-__device__ __forceinline__ C2<T> rhs_ox(int comp, int jx, int jy, const C2<T> *__restrict__ cfs,
-                                        const C2<T> *__restrict__ s0, const C2<T> *__restrict__ s1, const Geo &g) {
-    const int64_t NP = g.NP, ip = jx + (int64_t)g.Mx * jy, row = (int64_t)g.Mx * jy;
-    const C2<T> *c = cfs + ip;
-    const Nbr n = neighbours(g, jx, jy);
-    C2<T> a{0, 0};
-    if (comp == 0) {
-        a = sx3(a, c, NP, CF_DX + 1 * 3, s1, row, n, jx);
-        a = sxy4(a, c, NP, CF_XY + 1 * 4, s1, g.Mx, n);
-        a = sxy4(a, c, NP, CF_XY + 0 * 4, s0, g.Mx, n);
-    } else {
-        a = sx3(a, c, NP, CF_DX + 2 * 3, s0, row, n, jx);
-    }
-    return a;
-}
-
-template <class T>
-__device__ __forceinline__ C2<T> rhs_oy(int comp, int jx, int jy, const C2<T> *__restrict__ cfs,
-                                        const C2<T> *__restrict__ s0, const C2<T> *__restrict__ s1, const Geo &g) {
-    const int64_t NP = g.NP, ip = jx + (int64_t)g.Mx * jy;
-    const C2<T> *c = cfs + ip;
-    const Nbr n = neighbours(g, jx, jy);
-    C2<T> a{0, 0};
-    if (comp == 0) {
-        a = sy3(a, c, NP, CF_DY + 1 * 3, s1, g.Mx, n, jx, jy);
-    } else {
-        a = sy3(a, c, NP, CF_DY + 2 * 3, s0, g.Mx, n, jx, jy);
-        a = sxy4(a, c, NP, CF_XY + 2 * 4, s0, g.Mx, n);
-        a = sxy4(a, c, NP, CF_XY + 3 * 4, s1, g.Mx, n);
-    }
-    return a;
-}
-
-// ---------------------------------------------------------------------------
 // cyclic line solves (I - mu T) x = d, T = DX00/DX11 along x or DY00/DY11
 // along y (line solver of BlockSparseMatrix.cpp:194-258, 385-450).
 //
-// A line of M unknowns is cut into P chunks (P a power of two).  One thread
-// owns one chunk: a Thomas elimination of the chunk interior for the
+// A block holds a set of lines for both polarisations of one (wavelength,
+// wavevector): their right-hand sides are gathered together (the two
+// polarisations share every coefficient load) and staged in LDS, then each
+// line is cut into P chunks (P a power of two) and one thread owns one chunk
+// of one polarisation: a Thomas elimination of the chunk interior for the
 // right-hand side and for the two coupling responses, then the P
 // chunk-boundary unknowns form a cyclic tridiagonal system solved by
 // parallel cyclic reduction in LDS, and the thread rebuilds its chunk.
@@ -704,25 +804,25 @@ struct Chunk {
 };
 
 // forward elimination + first backward pass (boundary values of the chunk
-// interior responses); load(i, a, b, c, d) returns row i of the interior
-template <class T, int LMAX, class F>
+// interior responses); load(i, a, b, c, d) returns row i of the interior.
+// PREFETCH: all LMAX rows are fetched first (load() must accept every
+// i < LMAX; rows past the chunk are not used) so that the loads overlap the
+// dependent elimination chain.
+template <class T, int LMAX, bool PREFETCH, class F>
 __device__ __forceinline__ void chunk_eliminate(Chunk<T, LMAX> &ck, int n, F &&load) {
     using C = C2<T>;
     C binv{0, 0}, cprev{0, 0};
-    // FP32: every row of the chunk is fetched before the dependent elimination
-    // chain starts, so its loads overlap; FP64 keeps the registers for the chain
-    constexpr bool kPrefetch = sizeof(T) == 4;
-    C pa[kPrefetch ? LMAX : 1], pb[kPrefetch ? LMAX : 1], pc[kPrefetch ? LMAX : 1], pd[kPrefetch ? LMAX : 1];
-    if (kPrefetch) {
+    constexpr int NPF = PREFETCH ? LMAX : 1;
+    C pa[NPF], pb[NPF], pc[NPF], pd[NPF];
+    if (PREFETCH) {
 #pragma unroll
-        for (int i = 0; i < LMAX; i++)
-            if (i < n) load(i, pa[i], pb[i], pc[i], pd[i]);
+        for (int i = 0; i < LMAX; i++) load(i, pa[i], pb[i], pc[i], pd[i]);
     }
 #pragma unroll
     for (int i = 0; i < LMAX; i++) {
         if (i < n) {
             C a, b, c, d;
-            if (kPrefetch) {
+            if (PREFETCH) {
                 a = pa[i];
                 b = pb[i];
                 c = pc[i];
@@ -791,71 +891,110 @@ __device__ __forceinline__ void chunk_rebuild(const Chunk<T, LMAX> &ck, int n, C
     }
 }
 
-// interface equation of chunk p: A xi_{p-1} + B xi_p + C xi_{p+1} = D, from
-// the chunk's own responses and the first interior values of chunk p+1
+// LDS of the reduced systems: first interior values of each chunk (uf, gf,
+// hf) and the interface equations A xi_{p-1} + B xi_p + C xi_{p+1} = D;
+// the solutions overwrite uf
+template <class T>
+struct RedLds {
+    static constexpr int kArrays = 7;
+    C2<T> *uf, *gf, *hf, *A, *B, *Cc, *D;
+    __device__ __forceinline__ RedLds(C2<T> *base, int n) {
+        uf = base;
+        gf = uf + n;
+        hf = gf + n;
+        A = hf + n;
+        B = A + n;
+        Cc = B + n;
+        D = Cc + n;
+    }
+};
+
+// interface equation of chunk p from its own responses, its last row
+// (ae, be, ce, de) and the first interior values of chunk p+1 at slot nx
 template <class T, int LMAX>
 __device__ __forceinline__ void interface_row(const Chunk<T, LMAX> &ck, C2<T> ae, C2<T> be, C2<T> ce, C2<T> de,
-                                              C2<T> uf_next, C2<T> gf_next, C2<T> hf_next, C2<T> &A, C2<T> &B,
-                                              C2<T> &Cv, C2<T> &D) {
+                                              const RedLds<T> &r, int nx, C2<T> &A, C2<T> &B, C2<T> &Cv,
+                                              C2<T> &D) {
     A = ae * ck.gl;
-    B = be + ae * ck.hl + ce * gf_next;
-    Cv = ce * hf_next;
-    D = de - ae * ck.ul - ce * uf_next;
+    B = be + ae * ck.hl + ce * r.gf[nx];
+    Cv = ce * r.hf[nx];
+    D = de - ae * ck.ul - ce * r.uf[nx];
 }
 
 // cyclic tridiagonal system over the P chunk boundaries of one line, held in
 // LDS at slots base + q*stride; every thread of the block must call it.
 // Parallel cyclic reduction until the couplings reach P/2, then 2x2 pairs.
 template <class T>
-__device__ __forceinline__ void reduced_pcr(C2<T> *sA, C2<T> *sB, C2<T> *sC, C2<T> *sD, C2<T> *sX, int P, int p,
-                                            int base, int stride, bool active) {
+__device__ __forceinline__ void reduced_pcr(const RedLds<T> &r, int P, int p, int base, int stride, bool active) {
     using C = C2<T>;
+    const int me = base + p * stride;
     for (int st = 1; 2 * st < P; st <<= 1) {
         C nA{0, 0}, nB{0, 0}, nC{0, 0}, nD{0, 0};
         if (active) {
-            const int me = base + p * stride;
             const int im = base + ((p - st) & (P - 1)) * stride, iq = base + ((p + st) & (P - 1)) * stride;
-            const C k1 = cdiv(sA[me], sB[im]), k2 = cdiv(sC[me], sB[iq]);
-            nA = -(k1 * sA[im]);
-            nC = -(k2 * sC[iq]);
-            nB = sB[me] - k1 * sC[im] - k2 * sA[iq];
-            nD = sD[me] - k1 * sD[im] - k2 * sD[iq];
+            const C k1 = cdiv(r.A[me], r.B[im]), k2 = cdiv(r.Cc[me], r.B[iq]);
+            nA = -(k1 * r.A[im]);
+            nC = -(k2 * r.Cc[iq]);
+            nB = r.B[me] - k1 * r.Cc[im] - k2 * r.A[iq];
+            nD = r.D[me] - k1 * r.D[im] - k2 * r.D[iq];
         }
         __syncthreads();
         if (active) {
-            const int me = base + p * stride;
-            sA[me] = nA;
-            sB[me] = nB;
-            sC[me] = nC;
-            sD[me] = nD;
+            r.A[me] = nA;
+            r.B[me] = nB;
+            r.Cc[me] = nC;
+            r.D[me] = nD;
         }
         __syncthreads();
     }
     if (active) {
-        const int me = base + p * stride;
         C xi;
         if (P == 1) {
-            xi = cdiv(sD[me], sA[me] + sB[me] + sC[me]);
+            xi = cdiv(r.D[me], r.A[me] + r.B[me] + r.Cc[me]);
         } else {
             const int q = base + ((p + P / 2) & (P - 1)) * stride;
-            const C apc = sA[me] + sC[me], aqc = sA[q] + sC[q];
-            xi = cdiv(sD[me] * sB[q] - apc * sD[q], sB[me] * sB[q] - apc * aqc);
+            const C apc = r.A[me] + r.Cc[me], aqc = r.A[q] + r.Cc[q];
+            xi = cdiv(r.D[me] * r.B[q] - apc * r.D[q], r.B[me] * r.B[q] - apc * aqc);
         }
-        sX[me] = xi;
+        r.uf[me] = xi;
     }
     __syncthreads();
 }
 
-// LDS bytes of the reduced-system arrays (first values + A, B, C, D) per line
-template <class T>
-__host__ __device__ constexpr int reduced_lds_per_chunk() {
-    return 7 * (int)sizeof(C2<T>);
+// dst = v and acc (+)= v at one or two field positions of both
+// polarisations, loads first (one round trip for both points)
+template <class T, int NEU>
+__device__ __forceinline__ void store_pair2(C2<T> *__restrict__ d0, C2<T> *__restrict__ d1,
+                                            C2<T> *__restrict__ a0, C2<T> *__restrict__ a1, unsigned qa,
+                                            unsigned qb, bool vb, const C2<T> (&va)[2], const C2<T> (&vb2)[2]) {
+    C2<T> oa0{0, 0}, oa1{0, 0}, ob0{0, 0}, ob1{0, 0};
+    if (NEU) {
+        oa0 = a0[qa];
+        oa1 = a1[qa];
+        if (vb) {
+            ob0 = a0[qb];
+            ob1 = a1[qb];
+        }
+    }
+    d0[qa] = va[0];
+    d1[qa] = va[1];
+    a0[qa] = NEU ? oa0 + va[0] : va[0];
+    a1[qa] = NEU ? oa1 + va[1] : va[1];
+    if (vb) {
+        d0[qb] = vb2[0];
+        d1[qb] = vb2[1];
+        a0[qb] = NEU ? ob0 + vb2[0] : vb2[0];
+        a1[qb] = NEU ? ob1 + vb2[1] : vb2[1];
+    }
 }
 
-// ---- x lines: R rows per block, P chunks per row; the right-hand sides are
-// staged through LDS in the chunk-interleaved order (element (p, i) of a row
-// at i*P + p) so that global loads and stores stay coalesced; the tridiagonal
-// coefficients come pre-interleaved from the coefficient kernel -------------
+// ---- x lines: R rows per block for both polarisations (2 R P threads,
+// polarisation-major).  The right-hand sides are staged through LDS in the
+// chunk-interleaved order (element (p, i) of a row at i*P + p) so that global
+// loads and stores stay coalesced; the tridiagonal rows come pre-interleaved
+// from the coefficient kernel.
+//   neumann 0: dst = M_x (I + mu A_y) src, acc = dst
+//   neumann 1: dst = M_x (mu O_x src),     acc += dst
 template <class T, int LMAX, int NEU>
 __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
                                                  C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf,
@@ -863,204 +1002,253 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
                                                  const PropParams *__restrict__ pp, LinePlan lp, int R) {
     using C = C2<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int P = lp.P, SL = LMAX * P, M = g.Mx;
-    C *sd = reinterpret_cast<C *>(smem_raw);
-    C *sUF = sd + R * SL, *sGF = sUF + R * P, *sHF = sGF + R * P;
-    C *sA = sHF + R * P, *sB = sA + R * P, *sC = sB + R * P, *sD = sC + R * P;
-    C *sX = sUF;
-    int *smap = reinterpret_cast<int *>(sD + R * P);  // element x -> i*P + p
-    const int row0 = blockIdx.x * R, comp = blockIdx.y, prop = blockIdx.z;
+    const int P = lp.P, SL = LMAX * P, M = g.Mx, RP = R * P;
+    C *sd = reinterpret_cast<C *>(smem_raw);  // [pol][row][SL]
+    const RedLds<T> red(sd + 2 * R * SL, 2 * RP);
+    int *smap = reinterpret_cast<int *>(sd + 2 * R * SL + RedLds<T>::kArrays * 2 * RP);  // x -> i*P + p
+    const int row0 = blockIdx.x * R, comp = blockIdx.y, prop0 = blockIdx.z * 2;
     if ((int)threadIdx.x < P) {
         const int pc = threadIdx.x, s0c = chunk_start(lp, pc), lc = chunk_len(lp, pc);
         for (int i = 0; i < lc; i++) smap[s0c + i] = i * P + pc;
     }
     __syncthreads();
-    const int slot = pp[prop].slot;
-    const int64_t NP = g.NP;
-    const C *cfs = cf + (int64_t)slot * CF_COUNT * NP;
-    const C *s0 = src + ((int64_t)prop * 2) * NP, *s1 = s0 + NP;
+    const int slot = pp[prop0].slot;
+    const size_t NP = (size_t)g.NP;
+    const C *cfs = cf + (size_t)slot * cf_slot_size(g.NP);
+    const PairFields<T> fin(src, prop0, NP);
+    const int nrow = min(R, g.My - row0), bs = blockDim.x, ntot = nrow * M;
+    C *sd0 = sd, *sd1 = sd + R * SL;
 
+    // right-hand sides of the block's rows, both polarisations
     {
-        const int bs = blockDim.x;
-        for (int r = 0; r < R; r++) {
-            const int y = row0 + r;
-            if (y >= g.My) break;
-            for (int x0 = threadIdx.x; x0 < M; x0 += 4 * bs) {
-                C d[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int x = x0 + u * bs;
-                    if (x < M)
-                        d[u] = NEU ? rhs_ox<T>(comp, x, y, cfs, s0, s1, g) : rhs_yplus<T>(comp, x, y, cfs, s0, s1, g);
-                }
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int x = x0 + u * bs;
-                    if (x < M) sd[r * SL + smap[x]] = d[u];
-                }
+        constexpr int KIND = NEU ? RHS_OX : RHS_YPLUS, NPT = rhs_points<T>();
+        for (int e0 = threadIdx.x; e0 < ntot; e0 += NPT * bs) {
+            const int e1 = e0 + bs;
+            const bool v1 = NPT == 2 && e1 < ntot;
+            const int r0 = e0 / M, x0 = e0 - r0 * M, r1 = v1 ? e1 / M : 0, x1 = v1 ? e1 - r1 * M : 0;
+            C da[2], db[2];
+            rhs_pair2<T, KIND>(comp, x0, row0 + r0, x1, row0 + r1, v1, cfs, fin, g, da, db);
+            const int ia = r0 * SL + smap[x0];
+            sd0[ia] = da[0];
+            sd1[ia] = da[1];
+            if (v1) {
+                const int ib = r1 * SL + smap[x1];
+                sd0[ib] = db[0];
+                sd1[ib] = db[1];
             }
         }
     }
     __syncthreads();
 
-    const int r = threadIdx.x / P, p = threadIdx.x - r * P, y = row0 + r;
-    const bool active = r < R && y < g.My;
-    const int n = chunk_len(lp, p) - 1;
-    const int64_t xrow = ((((int64_t)slot * 2 + comp) * 3) * g.My + y) * SL;
+    // one chunk of one row of one polarisation per thread
+    const int pol = threadIdx.x / RP, t = threadIdx.x - pol * RP, r = t / P, p = t - r * P, y = row0 + r;
+    const bool active = r < nrow;
+    const int n = chunk_len(lp, p) - 1, line = pol * R + r;
+    C *sdl = sd + line * SL;
+    const int64_t xrow = ((((int64_t)slot * 2 + comp) * 3) * g.My + min(y, g.My - 1)) * SL;
     const int64_t xstride = (int64_t)g.My * SL;
     const C *xa = xt + xrow, *xb = xa + xstride, *xc = xb + xstride;
+    const int me = line * P + p, nx = line * P + ((p + 1) & (P - 1));
     Chunk<T, LMAX> ck;
     if (active) {
-        chunk_eliminate(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
+        chunk_eliminate<T, LMAX, sizeof(T) == 4>(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
             const int q = i * P + p;
             a = xa[q];
             b = xb[q];
             c = xc[q];
-            d = sd[r * SL + q];
+            d = sdl[q];
         });
-        sUF[r * P + p] = ck.uf;
-        sGF[r * P + p] = ck.gf;
-        sHF[r * P + p] = ck.hf;
+        red.uf[me] = ck.uf;
+        red.gf[me] = ck.gf;
+        red.hf[me] = ck.hf;
     }
     __syncthreads();
     C A{0, 0}, B{0, 0}, Cv{0, 0}, D{0, 0};
     if (active) {
-        const int nx = r * P + ((p + 1) & (P - 1)), q = n * P + p;
-        interface_row(ck, xa[q], xb[q], xc[q], sd[r * SL + q], sUF[nx], sGF[nx], sHF[nx], A, B, Cv, D);
+        const int q = n * P + p;
+        interface_row(ck, xa[q], xb[q], xc[q], sdl[q], red, nx, A, B, Cv, D);
     }
     __syncthreads();
     if (active) {
-        sA[r * P + p] = A;
-        sB[r * P + p] = B;
-        sC[r * P + p] = Cv;
-        sD[r * P + p] = D;
+        red.A[me] = A;
+        red.B[me] = B;
+        red.Cc[me] = Cv;
+        red.D[me] = D;
     }
     __syncthreads();
-    reduced_pcr(sA, sB, sC, sD, sX, P, p, r * P, 1, active);
+    reduced_pcr(red, P, p, line * P, 1, active);
     if (active) {
-        const C xi = sX[r * P + p], xprev = sX[r * P + ((p - 1) & (P - 1))];
-        chunk_rebuild(ck, n, xprev, xi, [&](int i, C v) { sd[r * SL + i * P + p] = v; });
-        sd[r * SL + n * P + p] = xi;
+        const C xi = red.uf[me], xprev = red.uf[line * P + ((p - 1) & (P - 1))];
+        chunk_rebuild(ck, n, xprev, xi, [&](int i, C v) { sdl[i * P + p] = v; });
+        sdl[n * P + p] = xi;
     }
     __syncthreads();
 
-    C *dd = dst + ((int64_t)prop * 2 + comp) * NP, *aa = acc + ((int64_t)prop * 2 + comp) * NP;
-    for (int rr = 0; rr < R; rr++) {
-        const int yy = row0 + rr;
-        if (yy >= g.My) break;
-        for (int x = threadIdx.x; x < M; x += blockDim.x) {
-            const C v = sd[rr * SL + smap[x]];
-            const int64_t q = x + (int64_t)M * yy;
-            dd[q] = v;
-            aa[q] = NEU ? aa[q] + v : v;
+    {
+        const size_t o0 = ((size_t)prop0 * 2 + comp) * NP, o1 = o0 + 2 * NP;
+        C *d0 = dst + o0, *d1 = dst + o1, *a0 = acc + o0, *a1 = acc + o1;
+        for (int e0 = threadIdx.x; e0 < ntot; e0 += 2 * bs) {
+            const int e1 = e0 + bs;
+            const bool v1 = e1 < ntot;
+            const int r0 = e0 / M, x0 = e0 - r0 * M, r1 = v1 ? e1 / M : r0, x1 = v1 ? e1 - r1 * M : x0;
+            const int ia = r0 * SL + smap[x0], ib = r1 * SL + smap[x1];
+            const C va[2] = {sd0[ia], sd1[ia]}, vb[2] = {sd0[ib], sd1[ib]};
+            store_pair2<T, NEU>(d0, d1, a0, a1, (unsigned)(x0 + M * (row0 + r0)), (unsigned)(x1 + M * (row0 + r1)),
+                                v1, va, vb);
         }
     }
 }
 
-// ---- y lines: W adjacent columns x P chunks per block.  The right-hand
-// sides of the W x M tile are computed by all threads (consecutive threads on
-// consecutive columns) and staged in LDS in the chunk-interleaved order
-// ((i*P + p)*W + column); the chunk threads then read the tridiagonal rows
-// from global memory one W-wide row segment at a time.
-template <class T, int LMAX, int NEU, int STAGE>
+// right-hand sides of component `comp` (uniform per block) of one
+// propagation at one or (FP32) two points; every load is issued before any sum
+template <class T, int KIND, int COMP>
+__device__ __forceinline__ void rhs_single_comp(unsigned xa, unsigned ya, unsigned xb, unsigned yb, bool vb,
+                                                const C2<T> *__restrict__ cfs, const C2<T> *__restrict__ f0,
+                                                const C2<T> *__restrict__ f1, const Geo &g, C2<T> &ra, C2<T> &rb) {
+    RhsGather<T, KIND, COMP> a;
+    a.load(xa, ya, cfs, f0, f1, g, 0);
+    if constexpr (rhs_points<T>() == 2) {
+        RhsGather<T, KIND, COMP> b;
+        if (vb) b.load(xb, yb, cfs, f0, f1, g, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        ra = a.sum();
+        if (vb) rb = b.sum();
+    } else {
+        (void)xb;
+        (void)yb;
+        (void)vb;
+        (void)rb;
+        __builtin_amdgcn_sched_barrier(0);
+        ra = a.sum();
+    }
+}
+template <class T, int KIND>
+__device__ __forceinline__ void rhs_single(int comp, unsigned xa, unsigned ya, unsigned xb, unsigned yb, bool vb,
+                                           const C2<T> *__restrict__ cfs, const C2<T> *__restrict__ f0,
+                                           const C2<T> *__restrict__ f1, const Geo &g, C2<T> &ra, C2<T> &rb) {
+    if (comp == 0) rhs_single_comp<T, KIND, 0>(xa, ya, xb, yb, vb, cfs, f0, f1, g, ra, rb);
+    else rhs_single_comp<T, KIND, 1>(xa, ya, xb, yb, vb, cfs, f0, f1, g, ra, rb);
+}
+
+// ---- y lines: W adjacent columns x P chunks per block, one propagation.
+// Blocks b and b+8 (the same XCD under round-robin dispatch) hold the two
+// polarisations of the same columns, so the second finds the coefficient
+// planes in L2.  The right-hand sides of the W x M tile are computed by all
+// threads (consecutive threads on consecutive columns) and staged in LDS in
+// the chunk-interleaved order ((i*P + p)*W + column); the chunk threads read
+// the tridiagonal rows from global memory one W-wide row segment at a time,
+// and the solution goes back through LDS to row-ordered stores.
+//   neumann 0: dst = M_y (I + mu A_x) src, acc = dst
+//   neumann 1: dst = M_y (mu O_y src),     acc += dst
+template <class T, int LMAX, int NEU>
 __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
                                                  C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf, Geo g,
                                                  const PropParams *__restrict__ pp, LinePlan lp, int W) {
     using C = C2<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int P = lp.P, M = g.My;
-    C *sd = reinterpret_cast<C *>(smem_raw);  // LMAX * P * W when the right-hand sides are staged
-    C *sUF = sd + (STAGE ? LMAX * P * W : 0);
-    C *sGF = sUF + W * P, *sHF = sGF + W * P;
-    C *sA = sHF + W * P, *sB = sA + W * P, *sC = sB + W * P, *sD = sC + W * P;
-    C *sX = sUF;
-    const int x0 = blockIdx.x * W, comp = blockIdx.y, prop = blockIdx.z;
+    const int P = lp.P, M = g.My, TL = LMAX * P * W;
+    C *sd = reinterpret_cast<C *>(smem_raw);
+    const RedLds<T> red(sd + TL, W * P);
+    const int bx = blockIdx.x, lb = (bx >> 4) * 8 + (bx & 7), pol = (bx >> 3) & 1;
+    if (lb * W >= g.Mx) return;  // padding blocks of the XCD grouping
+    const int x0 = lb * W, comp = blockIdx.y, prop = blockIdx.z * 2 + pol;
     const int slot = pp[prop].slot;
-    const int64_t NP = g.NP;
-    const C *cfs = cf + (int64_t)slot * CF_COUNT * NP;
-    const C *s0 = src + ((int64_t)prop * 2) * NP, *s1 = s0 + NP;
+    const size_t NP = (size_t)g.NP;
+    const C *cfs = cf + (size_t)slot * cf_slot_size(g.NP);
+    const C *f0 = src + (size_t)prop * 2 * NP, *f1 = f0 + NP;
+    const int wv = min(W, g.Mx - x0), ntile = M * wv, bs = blockDim.x;
 
-    // right-hand sides of the tile, four per thread per round (STAGE), or
-    // evaluated by the chunk threads themselves
-    const int ntile = STAGE ? M * W : 0, bs = blockDim.x;
-    for (int e0 = threadIdx.x; e0 < ntile; e0 += 4 * bs) {
-        C d[4];
-        int slotidx[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int e = e0 + u * bs;
-            slotidx[u] = -1;
-            if (e < ntile) {
-                const int j = e / W, c = e - j * W, x = x0 + c;
-                if (x < g.Mx) {
-                    d[u] = NEU ? rhs_oy<T>(comp, x, j, cfs, s0, s1, g) : rhs_xplus<T>(comp, x, j, cfs, s0, s1, g);
-                    int p, i;
-                    chunk_of(lp, j, p, i);
-                    slotidx[u] = (i * P + p) * W + c;
-                }
+    // right-hand sides of the tile
+    {
+        constexpr int KIND = NEU ? RHS_OY : RHS_XPLUS, NPT = rhs_points<T>();
+        for (int e0 = threadIdx.x; e0 < ntile; e0 += NPT * bs) {
+            const int e1 = e0 + bs;
+            const bool v1 = NPT == 2 && e1 < ntile;
+            const int j0 = e0 / wv, c0 = e0 - j0 * wv, j1 = v1 ? e1 / wv : 0, c1 = v1 ? e1 - j1 * wv : 0;
+            C da{0, 0}, db{0, 0};
+            rhs_single<T, KIND>(comp, x0 + c0, j0, x0 + c1, j1, v1, cfs, f0, f1, g, da, db);
+            int p0, i0;
+            chunk_of(lp, j0, p0, i0);
+            sd[(i0 * P + p0) * W + c0] = da;
+            if (v1) {
+                int p1, i1;
+                chunk_of(lp, j1, p1, i1);
+                sd[(i1 * P + p1) * W + c1] = db;
             }
         }
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-            if (slotidx[u] >= 0) sd[slotidx[u]] = d[u];
     }
     __syncthreads();
 
     const int cofs = threadIdx.x % W, p = threadIdx.x / W, x = x0 + cofs;
-    const bool active = x < g.Mx && p < P;
+    const bool active = cofs < wv;
     const int blk = comp == 0 ? 0 : 3;
-    const C *ca = cfs + (int64_t)(CF_DY + blk * 3 + 0) * NP;
-    const C *cb = cfs + (int64_t)(CF_DY + blk * 3 + 1) * NP;
-    const C *cc = cfs + (int64_t)(CF_DY + blk * 3 + 2) * NP;
+    const C *cdy = cfs + (CF_DY + blk * 3) * kCfTile;
     const int s = chunk_start(lp, p), n = chunk_len(lp, p) - 1;
     auto tri = [&](int j, C &a, C &b, C &c) {
-        const int64_t q = x + (int64_t)g.Mx * j;
-        const C va = ca[q], vb = cb[q], vc = cc[q];
+        j = min(j, M - 1);  // rows past the line end are fetched but not used
+        const C *c3 = cdy + cf_point((unsigned)(x + g.Mx * j));
+        const C va = c3[0], vb = c3[kCfTile], vc = c3[2 * kCfTile];
         a = C{-va.x, -va.y};
         b = C{(T)1 - vb.x, -vb.y};
         c = C{-vc.x, -vc.y};
     };
     const int me = p * W + cofs, nx = ((p + 1) & (P - 1)) * W + cofs;
     Chunk<T, LMAX> ck;
-    auto rhs = [&](int i) -> C {
-        if (STAGE) return sd[(i * P + p) * W + cofs];
-        return NEU ? rhs_oy<T>(comp, x, s + i, cfs, s0, s1, g) : rhs_xplus<T>(comp, x, s + i, cfs, s0, s1, g);
-    };
     if (active) {
-        chunk_eliminate(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
+        chunk_eliminate<T, LMAX, sizeof(T) == 4>(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
             tri(s + i, a, b, c);
-            d = rhs(i);
+            d = sd[(i * P + p) * W + cofs];
         });
-        sUF[me] = ck.uf;
-        sGF[me] = ck.gf;
-        sHF[me] = ck.hf;
+        red.uf[me] = ck.uf;
+        red.gf[me] = ck.gf;
+        red.hf[me] = ck.hf;
     }
     __syncthreads();
     C A{0, 0}, B{0, 0}, Cv{0, 0}, D{0, 0};
     if (active) {
         C ae, be, ce;
         tri(s + n, ae, be, ce);
-        interface_row(ck, ae, be, ce, rhs(n), sUF[nx], sGF[nx], sHF[nx], A, B, Cv, D);
+        interface_row(ck, ae, be, ce, sd[(n * P + p) * W + cofs], red, nx, A, B, Cv, D);
     }
     __syncthreads();
     if (active) {
-        sA[me] = A;
-        sB[me] = B;
-        sC[me] = Cv;
-        sD[me] = D;
+        red.A[me] = A;
+        red.B[me] = B;
+        red.Cc[me] = Cv;
+        red.D[me] = D;
     }
     __syncthreads();
-    reduced_pcr(sA, sB, sC, sD, sX, P, p, cofs, W, active);
+    reduced_pcr(red, P, p, cofs, W, active);
     if (active) {
-        const C xi = sX[me], xprev = sX[((p - 1) & (P - 1)) * W + cofs];
-        C *dd = dst + ((int64_t)prop * 2 + comp) * NP, *aa = acc + ((int64_t)prop * 2 + comp) * NP;
-        auto put = [&](int j, C v) {
-            const int64_t q = x + (int64_t)g.Mx * j;
-            dd[q] = v;
-            aa[q] = NEU ? aa[q] + v : v;
-        };
-        chunk_rebuild(ck, n, xprev, xi, [&](int i, C v) { put(s + i, v); });
-        put(s + n, xi);
+        const C xi = red.uf[me], xprev = red.uf[((p - 1) & (P - 1)) * W + cofs];
+        chunk_rebuild(ck, n, xprev, xi, [&](int i, C v) { sd[(i * P + p) * W + cofs] = v; });
+        sd[(n * P + p) * W + cofs] = xi;
+    }
+    __syncthreads();
+
+    {
+        C *dd = dst + ((size_t)prop * 2 + comp) * NP, *aa = acc + ((size_t)prop * 2 + comp) * NP;
+        for (int e0 = threadIdx.x; e0 < ntile; e0 += 2 * bs) {
+            const int e1 = e0 + bs;
+            const bool v1 = e1 < ntile;
+            const int j0 = e0 / wv, c0 = e0 - j0 * wv, j1 = v1 ? e1 / wv : j0, c1 = v1 ? e1 - j1 * wv : c0;
+            int p0, i0, p1, i1;
+            chunk_of(lp, j0, p0, i0);
+            chunk_of(lp, j1, p1, i1);
+            const C va = sd[(i0 * P + p0) * W + c0], vb = sd[(i1 * P + p1) * W + c1];
+            const unsigned qa = x0 + c0 + g.Mx * j0, qb = x0 + c1 + g.Mx * j1;
+            C oa{0, 0}, ob{0, 0};
+            if (NEU) {
+                oa = aa[qa];
+                if (v1) ob = aa[qb];
+            }
+            dd[qa] = va;
+            aa[qa] = NEU ? oa + va : va;
+            if (v1) {
+                dd[qb] = vb;
+                aa[qb] = NEU ? ob + vb : vb;
+            }
+        }
     }
 }
 
@@ -1074,45 +1262,45 @@ LinePlan make_line_plan(int M, int chunks) {
     lp.base = M / P;
     lp.rem = M % P;
     int need = lp.base + (lp.rem ? 1 : 0);
-    lp.lmax = need <= 4 ? 4 : need <= 8 ? 8 : need <= 16 ? 16 : need <= 32 ? 32 : 0;
-    if (lp.lmax == 0) throw std::runtime_error("line solver: chunks longer than 32 points (too few chunks)");
+    lp.lmax = need <= 4 ? 4 : need <= 8 ? 8 : need <= 16 ? 16 : 0;
+    if (lp.lmax == 0) throw std::runtime_error("line solver: chunks longer than 16 points (too few chunks)");
     return lp;
 }
 
+// staged right-hand sides of both polarisations + their reduced systems
 template <class T>
 size_t xsolve_lds_bytes(const LinePlan &lp, int R) {
-    return (size_t)R * ((size_t)lp.lmax * lp.P * sizeof(C2<T>) + (size_t)lp.P * reduced_lds_per_chunk<T>()) +
+    return 2 * (size_t)R * ((size_t)lp.lmax * lp.P + (size_t)RedLds<T>::kArrays * lp.P) * sizeof(C2<T>) +
            (size_t)lp.M * sizeof(int);
 }
 template <class T>
-size_t ysolve_lds_bytes(const LinePlan &lp, int W, int stage) {
-    return (size_t)W * ((stage ? (size_t)lp.lmax * lp.P * sizeof(C2<T>) : 0) + (size_t)lp.P * reduced_lds_per_chunk<T>());
+size_t ysolve_lds_bytes(const LinePlan &lp, int W) {
+    return (size_t)W * ((size_t)lp.lmax * lp.P + (size_t)RedLds<T>::kArrays * lp.P) * sizeof(C2<T>);
 }
 
 template <class T, int NEU>
 static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
                             const C2<T> *cf, const C2<T> *xt, const Geo &g, const PropParams *pp, hipStream_t st) {
-    const dim3 block(lp.P * R);
+    const dim3 block(2 * lp.P * R);
     const size_t lds = xsolve_lds_bytes<T>(lp, R);
     switch (lp.lmax) {
         case 4: hipLaunchKernelGGL((k_xsolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R); break;
         case 8: hipLaunchKernelGGL((k_xsolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R); break;
         case 16: hipLaunchKernelGGL((k_xsolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R); break;
-        case 32: hipLaunchKernelGGL((k_xsolve<T, 32, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R); break;
         default: throw std::runtime_error("x line solver: unsupported chunk length");
     }
     NMK_LAUNCH_CHECK();
 }
 
-template <class T, int NEU, int STAGE>
+template <class T, int NEU>
 static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
                             const C2<T> *cf, const Geo &g, const PropParams *pp, hipStream_t st) {
     const dim3 block(lp.P * W);
-    const size_t lds = ysolve_lds_bytes<T>(lp, W, STAGE);
+    const size_t lds = ysolve_lds_bytes<T>(lp, W);
     switch (lp.lmax) {
-        case 4: hipLaunchKernelGGL((k_ysolve<T, 4, NEU, STAGE>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
-        case 8: hipLaunchKernelGGL((k_ysolve<T, 8, NEU, STAGE>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
-        case 16: hipLaunchKernelGGL((k_ysolve<T, 16, NEU, STAGE>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
+        case 4: hipLaunchKernelGGL((k_ysolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
+        case 8: hipLaunchKernelGGL((k_ysolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
+        case 16: hipLaunchKernelGGL((k_ysolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
         default: throw std::runtime_error("y line solver: unsupported chunk length");
     }
     NMK_LAUNCH_CHECK();
@@ -1121,24 +1309,22 @@ static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *s
 template <class T>
 void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
                    const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, hipStream_t st) {
-    if (lp.P * R > 512 || lp.M != g.Mx) throw std::runtime_error("x line solver: bad launch shape");
-    dim3 grid((g.My + R - 1) / R, 2, nprops);
+    if (2 * lp.P * R > 512 || lp.M != g.Mx || R < 1) throw std::runtime_error("x line solver: bad launch shape");
+    if (nprops % 2) throw std::runtime_error("x line solver: propagations come in polarisation pairs");
+    dim3 grid((unsigned)((g.My + R - 1) / R), 2, (unsigned)(nprops / 2));
     if (neumann) xsolve_dispatch<T, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, st);
     else xsolve_dispatch<T, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, st);
 }
 
 template <class T>
 void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
-                   const PropParams *pp, int nprops, const LinePlan &lp, int W, int stage, hipStream_t st) {
-    if (lp.P * W > 512 || lp.M != g.My) throw std::runtime_error("y line solver: bad launch shape");
-    dim3 grid((g.Mx + W - 1) / W, 2, nprops);
-    if (stage) {
-        if (neumann) ysolve_dispatch<T, 1, 1>(lp, W, grid, src, dst, acc, cf, g, pp, st);
-        else ysolve_dispatch<T, 0, 1>(lp, W, grid, src, dst, acc, cf, g, pp, st);
-    } else {
-        if (neumann) ysolve_dispatch<T, 1, 0>(lp, W, grid, src, dst, acc, cf, g, pp, st);
-        else ysolve_dispatch<T, 0, 0>(lp, W, grid, src, dst, acc, cf, g, pp, st);
-    }
+                   const PropParams *pp, int nprops, const LinePlan &lp, int W, hipStream_t st) {
+    if (lp.P * W > 512 || lp.M != g.My || W < 1) throw std::runtime_error("y line solver: bad launch shape");
+    if (nprops % 2) throw std::runtime_error("y line solver: propagations come in polarisation pairs");
+    const int nlb = (g.Mx + W - 1) / W;
+    dim3 grid((unsigned)((nlb + 7) / 8 * 16), 2, (unsigned)(nprops / 2));
+    if (neumann) ysolve_dispatch<T, 1>(lp, W, grid, src, dst, acc, cf, g, pp, st);
+    else ysolve_dispatch<T, 0>(lp, W, grid, src, dst, acc, cf, g, pp, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1396,9 +1582,9 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
     template void launch_xsolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const C2<T> *, const Geo &, \
                                    const PropParams *, int, const LinePlan &, int, hipStream_t);                      \
     template void launch_ysolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const Geo &,                  \
-                                   const PropParams *, int, const LinePlan &, int, int, hipStream_t);                 \
+                                   const PropParams *, int, const LinePlan &, int, hipStream_t);                      \
     template size_t xsolve_lds_bytes<T>(const LinePlan &, int);                                                       \
-    template size_t ysolve_lds_bytes<T>(const LinePlan &, int, int);                                                  \
+    template size_t ysolve_lds_bytes<T>(const LinePlan &, int);                                                       \
     template void launch_output<T>(const C2<T> *, const double *, const uint8_t *, const Geo &, const SlotParams *,  \
                                    const PropParams *, int, int, c64 *, hipStream_t);
 NMK_INST(float)

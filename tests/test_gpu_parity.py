@@ -202,9 +202,9 @@ def test_random512_reduced_depth_fp32_vs_fp64():
 
 
 @pytest.mark.parametrize("tuning", [
-    {"x_chunks": 16, "x_rows": 8, "y_chunks": 16, "y_cols": 16, "y_stage": 1},
-    {"x_chunks": 32, "x_rows": 1, "y_chunks": 32, "y_cols": 1, "y_stage": 0},
-    {"x_chunks": 64, "x_rows": 2, "y_chunks": 64, "y_cols": 4, "y_stage": 0},
+    {"x_chunks": 8, "x_rows": 4, "y_chunks": 8, "y_cols": 4},     # chunks of 8-9 points (16-point solver)
+    {"x_chunks": 32, "x_rows": 1, "y_chunks": 32, "y_cols": 1},   # chunks of 2-3 points
+    {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8},   # chunks of 4-5 points
 ])
 def test_line_solver_shapes_fp64(tuning):
     """Every autotuner candidate shape must give the oracle's answer."""
