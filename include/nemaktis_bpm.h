@@ -107,6 +107,68 @@ int nmk_bpm_eval_expression(const char *expr, double lambda, double *value, char
 
 const char *nmk_bpm_version(void);
 
+/* ---- serving loop (no counterpart in the reference, whose backend runs one
+ * blocking propagation per call: PythonBackend/bpm_backend.cpp:7-164) --------
+ *
+ * A server owns one GPU and a pre-allocated HBM arena.  Submitted micrograph
+ * jobs (the arguments of nmk_bpm_run) are validated at once, queued, and
+ * admitted in arrival order whenever an execution slot and a contiguous arena
+ * range of the job's footprint are free; a job that does not fit may be
+ * overtaken by later jobs that do, at most max_bypass times.  Admitted jobs
+ * run concurrently, each on its own stream with its state resident in its
+ * arena range.  E_field_vals is read at admission ("Beam profile": "None")
+ * and written when the job ends; it must stay valid until then. */
+typedef struct nmk_srv nmk_srv;
+
+typedef struct {
+    int32_t device;      /* HIP device, -1: current */
+    int32_t max_active;  /* jobs on the device at once (default 4) */
+    int64_t hbm_budget;  /* arena bytes; 0: 80 % of the free device memory */
+    int32_t max_bypass;  /* default 8 */
+    int32_t reserved[3];
+} nmk_srv_config;
+
+enum nmk_srv_job_state {
+    NMK_SRV_QUEUED = 0, NMK_SRV_RUNNING = 1, NMK_SRV_DONE = 2, NMK_SRV_FAILED = 3, NMK_SRV_CANCELLED = 4
+};
+
+typedef struct {
+    int32_t state;       /* nmk_srv_job_state */
+    int32_t status;      /* nmk_bpm_status of a failed job */
+    double t_submit_ms;  /* host clock, ms since the server was created */
+    double t_admit_ms;   /* -1 until admitted */
+    double t_end_ms;     /* -1 until finished */
+    double device_ms;    /* device time of the propagation */
+    double updates;      /* voxel-slice updates of the job */
+    int64_t footprint_bytes;
+    char error[256];
+} nmk_srv_job_info;
+
+typedef struct {
+    int64_t submitted, completed, failed, cancelled, running, queued;
+    int64_t arena_bytes, arena_used_bytes, arena_peak_bytes;
+    int32_t peak_active;
+    int32_t reserved;
+    double now_ms;
+} nmk_srv_counters;
+
+void nmk_srv_default_config(nmk_srv_config *cfg);
+/* NULL on failure (no device, budget above the free memory, ...) */
+nmk_srv *nmk_srv_create(const nmk_srv_config *cfg, char *err_msg, size_t err_len);
+/* job id >= 0, or -(nmk_bpm_status) when the job is refused (invalid input,
+ * or a footprint larger than the whole arena) */
+int64_t nmk_srv_submit(nmk_srv *srv, const char *json_str, const char *mask_formula, const double *lc_field_vals,
+                       int64_t n_lc_vals, const double *mask_vals, int64_t n_mask_vals, double *E_field_vals,
+                       int64_t n_E_vals, const nmk_bpm_options *opts, char *err_msg, size_t err_len);
+/* 0: finished (done, failed or cancelled), 1: timed out (timeout_ms < 0 waits
+ * without limit), -1: unknown job */
+int nmk_srv_wait(nmk_srv *srv, int64_t job, double timeout_ms, nmk_srv_job_info *info);
+/* 0: cancelled, 1: already admitted or finished, -1: unknown job */
+int nmk_srv_cancel(nmk_srv *srv, int64_t job);
+int nmk_srv_stats(nmk_srv *srv, nmk_srv_counters *stats);
+/* cancels queued jobs, waits for the running ones, frees the arena */
+void nmk_srv_destroy(nmk_srv *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,10 +40,42 @@ class Stats(ctypes.Structure):
         }
 
 
+class SrvConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("max_active", ctypes.c_int32), ("hbm_budget", ctypes.c_int64),
+                ("max_bypass", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+
+
+class SrvJobInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("t_submit_ms", ctypes.c_double),
+                ("t_admit_ms", ctypes.c_double), ("t_end_ms", ctypes.c_double), ("device_ms", ctypes.c_double),
+                ("updates", ctypes.c_double), ("footprint_bytes", ctypes.c_int64), ("error", ctypes.c_char * 256)]
+
+    STATES = ["queued", "running", "done", "failed", "cancelled"]
+
+    def as_dict(self):
+        return {"state": self.STATES[self.state], "status": self.status, "t_submit_ms": self.t_submit_ms,
+                "t_admit_ms": self.t_admit_ms, "t_end_ms": self.t_end_ms, "device_ms": self.device_ms,
+                "updates": self.updates, "footprint_bytes": self.footprint_bytes,
+                "error": self.error.decode("utf-8", "replace")}
+
+
+class SrvCounters(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("failed", ctypes.c_int64),
+                ("cancelled", ctypes.c_int64), ("running", ctypes.c_int64), ("queued", ctypes.c_int64),
+                ("arena_bytes", ctypes.c_int64), ("arena_used_bytes", ctypes.c_int64),
+                ("arena_peak_bytes", ctypes.c_int64), ("peak_active", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("now_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
 EXPORTED = [
     "nmk_bpm_run_backend_without_mask", "nmk_bpm_run_backend_with_mask", "nmk_bpm_default_options",
     "nmk_bpm_run", "nmk_bpm_create", "nmk_bpm_upload_fields", "nmk_bpm_propagate", "nmk_bpm_download",
     "nmk_bpm_destroy", "nmk_bpm_estimate_footprint", "nmk_bpm_eval_expression", "nmk_bpm_version",
+    "nmk_srv_default_config", "nmk_srv_create", "nmk_srv_submit", "nmk_srv_wait", "nmk_srv_cancel",
+    "nmk_srv_stats", "nmk_srv_destroy",
 ]
 
 
@@ -71,6 +103,17 @@ def load():
                                                ctypes.POINTER(i64), cp, sz]
     lib.nmk_bpm_eval_expression.argtypes = [cp, ctypes.c_double, c_double_p, cp, sz]
     lib.nmk_bpm_version.restype = ctypes.c_char_p
+    vp = ctypes.c_void_p
+    lib.nmk_srv_default_config.argtypes = [ctypes.POINTER(SrvConfig)]
+    lib.nmk_srv_create.argtypes = [ctypes.POINTER(SrvConfig), cp, sz]
+    lib.nmk_srv_create.restype = vp
+    lib.nmk_srv_submit.argtypes = [vp, cp, cp, c_double_p, i64, c_double_p, i64, c_double_p, i64,
+                                   ctypes.POINTER(Options), cp, sz]
+    lib.nmk_srv_submit.restype = i64
+    lib.nmk_srv_wait.argtypes = [vp, i64, ctypes.c_double, ctypes.POINTER(SrvJobInfo)]
+    lib.nmk_srv_cancel.argtypes = [vp, i64]
+    lib.nmk_srv_stats.argtypes = [vp, ctypes.POINTER(SrvCounters)]
+    lib.nmk_srv_destroy.argtypes = [vp]
     _lib = lib
     return lib
 

@@ -9,6 +9,7 @@
 
 #include "engine.h"
 #include "expr.h"
+#include "server.h"
 #include "settings.h"
 
 using namespace nmk;
@@ -201,9 +202,12 @@ int nmk_bpm_estimate_footprint(const char *json, int64_t n_lc, int has_mask, con
             p.qx.push_back(q.first);
             p.qy.push_back(q.second);
         }
-        if (has_mask) p.mask.assign(1, 1);  // marks presence; size added below
         int64_t b = estimate_footprint(p, to_options(opts));
-        if (has_mask) b += nv - 1;
+        if (has_mask) {
+            // the mask and (upper bound) the interface weights of a mask formula
+            auto al = [](int64_t v) { return (v + (int64_t)kAllocAlign - 1) / (int64_t)kAllocAlign * (int64_t)kAllocAlign; };
+            b += al(nv) + al(2 * (int64_t)p.Nx * p.Ny * std::max(0, p.Nz - 1) * 8);
+        }
         if (bytes) *bytes = b;
     });
 }
@@ -220,6 +224,92 @@ int nmk_bpm_eval_expression(const char *expr, double lambda, double *value, char
     });
 }
 
-const char *nmk_bpm_version(void) { return "nemaktis-amd bpm 0.1 (gfx950)"; }
+const char *nmk_bpm_version(void) { return "nemaktis-amd bpm 0.2 (gfx950)"; }
+
+/* ---- serving loop ---------------------------------------------------------- */
+
+void nmk_srv_default_config(nmk_srv_config *c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->device = -1;
+    c->max_active = 4;
+    c->hbm_budget = 0;
+    c->max_bypass = 8;
+}
+
+nmk_srv *nmk_srv_create(const nmk_srv_config *cfg, char *err, size_t errlen) {
+    nmk_srv *srv = nullptr;
+    guarded(err, errlen, [&] {
+        nmk_srv_config d;
+        nmk_srv_default_config(&d);
+        if (!cfg) cfg = &d;
+        ServerConfig c;
+        c.device = cfg->device;
+        c.max_active = cfg->max_active;
+        c.hbm_budget = cfg->hbm_budget;
+        c.max_bypass = cfg->max_bypass;
+        srv = reinterpret_cast<nmk_srv *>(new Server(c));
+    });
+    return srv;
+}
+
+int64_t nmk_srv_submit(nmk_srv *srv, const char *json, const char *formula, const double *lc, int64_t n_lc,
+                       const double *mask, int64_t n_mask, double *E, int64_t n_E, const nmk_bpm_options *opts,
+                       char *err, size_t errlen) {
+    int64_t id = -1;
+    const int rc = guarded(err, errlen, [&] {
+        if (!srv) throw BpmError("null server");
+        if (!json) throw BpmError("null settings string");
+        id = reinterpret_cast<Server *>(srv)->submit(json, formula ? formula : "", lc, n_lc, mask, n_mask,
+                                                      reinterpret_cast<c64 *>(E), n_E, to_options(opts));
+    });
+    return rc == NMK_BPM_OK ? id : -(int64_t)rc;
+}
+
+int nmk_srv_wait(nmk_srv *srv, int64_t id, double timeout_ms, nmk_srv_job_info *info) {
+    if (!srv) return -1;
+    JobInfo ji;
+    const int r = reinterpret_cast<Server *>(srv)->wait(id, timeout_ms, &ji);
+    if (info && r >= 0) {
+        std::memset(info, 0, sizeof(*info));
+        info->state = ji.state;
+        info->status = ji.status;
+        info->t_submit_ms = ji.t_submit_ms;
+        info->t_admit_ms = ji.t_admit_ms;
+        info->t_end_ms = ji.t_end_ms;
+        info->device_ms = ji.device_ms;
+        info->updates = ji.updates;
+        info->footprint_bytes = ji.footprint;
+        set_err(info->error, sizeof(info->error), ji.error);
+    }
+    return r;
+}
+
+int nmk_srv_cancel(nmk_srv *srv, int64_t id) {
+    if (!srv) return -1;
+    bool known = false;
+    const bool ok = reinterpret_cast<Server *>(srv)->cancel(id, &known);
+    return !known ? -1 : ok ? 0 : 1;
+}
+
+int nmk_srv_stats(nmk_srv *srv, nmk_srv_counters *out) {
+    if (!srv || !out) return -1;
+    const ServerStats s = reinterpret_cast<Server *>(srv)->stats();
+    std::memset(out, 0, sizeof(*out));
+    out->submitted = s.submitted;
+    out->completed = s.completed;
+    out->failed = s.failed;
+    out->cancelled = s.cancelled;
+    out->running = s.running;
+    out->queued = s.queued;
+    out->arena_bytes = s.arena_bytes;
+    out->arena_used_bytes = s.arena_used;
+    out->arena_peak_bytes = s.arena_peak;
+    out->peak_active = s.peak_active;
+    out->now_ms = reinterpret_cast<Server *>(srv)->now_ms();
+    return 0;
+}
+
+void nmk_srv_destroy(nmk_srv *srv) { delete reinterpret_cast<Server *>(srv); }
 
 }  // extern "C"

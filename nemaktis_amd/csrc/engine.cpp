@@ -27,20 +27,33 @@ template <class P>
 struct DevBuf {
     P *ptr = nullptr;
     size_t count = 0;
+    DeviceAllocator *source = nullptr;  // nullptr: hipMalloc
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() {
-        if (ptr) (void)hipFree(ptr);
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) {
+            if (source) source->release(ptr);
+            else (void)hipFree(ptr);
+        }
+        ptr = nullptr;
     }
     void alloc(size_t n) {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
+        release();
         count = n;
-        if (n) HIPCHK(hipMalloc((void **)&ptr, n * sizeof(P)));
+        if (!n) return;
+        if (source) ptr = static_cast<P *>(source->allocate(n * sizeof(P)));
+        else HIPCHK(hipMalloc((void **)&ptr, n * sizeof(P)));
     }
     size_t bytes() const { return count * sizeof(P); }
+    size_t footprint() const { return (bytes() + kAllocAlign - 1) / kAllocAlign * kAllocAlign; }
 };
+
+template <class... B>
+void bind_source(DeviceAllocator *a, B &...b) {
+    ((b.source = a), ...);
+}
 
 // Catmull-Rom weights of a fractional shift nu (SimpleShiftOperator.cpp:27-30)
 void cr_weights(double nu, double *w) {
@@ -98,6 +111,8 @@ public:
             own_stream_ = true;
         }
         sz_ = sizes_of(p, o);
+        bind_source(o.allocator, lc_, weights_, q_, iso_t_, iso_n_, partials_, sm_, mask_, twx_, twy_, E_, sp_, pp_, cf_,
+                    xt_, fb_[0], fb_[1], fb_[2], fb_[3]);
         g_.Nx = p.Nx;
         g_.Ny = p.Ny;
         g_.Nz = p.Nz;
@@ -182,8 +197,11 @@ public:
     const c64 *device_output() const override { return E_.ptr; }
 
     int64_t footprint() const override {
-        return (int64_t)(lc_.bytes() + mask_.bytes() + weights_.bytes() + sm_.bytes() + cf_.bytes() +
-                         4 * fb_[0].bytes() + E_.bytes() + partials_.bytes());
+        size_t b = lc_.footprint() + weights_.footprint() + q_.footprint() + iso_t_.footprint() + iso_n_.footprint() +
+                   partials_.footprint() + sm_.footprint() + mask_.footprint() + twx_.footprint() + twy_.footprint() +
+                   E_.footprint() + sp_.footprint() + pp_.footprint() + cf_.footprint() + xt_.footprint();
+        for (auto &f : fb_) b += f.footprint();
+        return (int64_t)b;
     }
 
     void upload_fields(const c64 *E_host) override {
@@ -571,16 +589,32 @@ Tuning default_tuning(int precision, int Mx, int My) {
 }
 
 int64_t estimate_footprint(const Problem &p, const Options &o) {
-    Sizes s = sizes_of(p, o);
+    // mirrors the allocations of RunnerT, buffer by buffer
+    const Sizes s = sizes_of(p, o);
     const int64_t fsz = o.precision == 32 ? 8 : 16;
-    int64_t b = p.voxels() * p.field_dim * 8;
-    if (!p.mask.empty()) b += p.voxels();
-    b += (int64_t)p.weights.size() * 8;
-    b += (int64_t)s.nslots_max * SM_COUNT * s.Nfull * 8;
-    b += 2 * (int64_t)s.nslots_max * cf_slot_size(s.NP) * fsz;                 // double-buffered coefficients
-    b += 2 * (int64_t)s.nslots_max * 6 * (p.Ny) * 2 * ((int64_t)p.Nx + 64) * fsz;  // x-line rows (upper bound)
-    b += 4 * (int64_t)s.nprops_max * 2 * s.NP * fsz;
-    b += (int64_t)s.nwl * s.nq * 4 * s.Nfull * 16;
+    const int Mx = p.Nx > 3 ? p.Nx - 1 : p.Nx, My = p.Ny > 3 ? p.Ny - 1 : p.Ny;
+    int64_t b = 0;
+    auto add = [&](int64_t bytes) { b += (bytes + (int64_t)kAllocAlign - 1) / (int64_t)kAllocAlign * (int64_t)kAllocAlign; };
+    add(p.voxels() * p.field_dim * 8);
+    add((int64_t)p.mask.size());
+    add((int64_t)p.weights.size() * 8);
+    add((int64_t)s.nq * 2 * 8);
+    add((int64_t)std::max<size_t>(1, p.iso_thick.size()) * 8);
+    add((int64_t)std::max<size_t>(1, p.iso_index.size()) * 8);
+    add((int64_t)p.Nx * 16);
+    add((int64_t)p.Ny * 16);
+    add((int64_t)s.nslots_max * (int64_t)sizeof(SlotParams));
+    add((int64_t)s.nprops_max * (int64_t)sizeof(PropParams));
+    add((int64_t)kNrefBlocks * s.nslots_max * 8);
+    add((int64_t)s.nslots_max * SM_COUNT * s.Nfull * 8);
+    add(2 * (int64_t)s.nslots_max * cf_slot_size(s.NP) * fsz);
+    if (p.Nx > 3) {
+        const Tuning t = o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, Mx, My);
+        const LinePlan lp = make_line_plan(Mx, t.x_chunks);
+        add(2 * (int64_t)s.nslots_max * 2 * 3 * My * lp.lmax * lp.P * fsz);
+    }
+    for (int i = 0; i < 4; i++) add((int64_t)s.nprops_max * 2 * s.NP * fsz);
+    add((int64_t)s.nwl * s.nq * 4 * s.Nfull * 16);
     return b;
 }
 

@@ -20,6 +20,16 @@ struct Tuning {
 };
 Tuning default_tuning(int precision, int Mx, int My);
 
+// source of a runner's device memory: hipMalloc when none is given, or a
+// caller-owned region (the serving loop hands every admitted job a slice of
+// one pre-allocated arena, so no allocation happens while other jobs run)
+class DeviceAllocator {
+public:
+    virtual ~DeviceAllocator() = default;
+    virtual void *allocate(size_t bytes) = 0;  // throws when it cannot
+    virtual void release(void *p) = 0;
+};
+
 struct Options {
     int precision = 64;  // 64: complex128 march; 32: complex64 march, FP64 coefficients
     int wl_batch = 1;    // wavelengths marched together (<= 0: all)
@@ -27,6 +37,7 @@ struct Options {
     int device = -1;     // HIP device (-1: current)
     int profile = 0;     // time every kernel family with events (no graph)
     Tuning tuning;       // x_chunks == 0: default shapes
+    DeviceAllocator *allocator = nullptr;  // nullptr: hipMalloc / hipFree
 };
 
 // per-kernel-family device time of one propagate() (profile mode)
@@ -60,7 +71,9 @@ public:
 
 std::unique_ptr<Runner> make_runner(const Problem &p, const Options &o, hipStream_t stream = nullptr);
 
-// bytes of HBM a run of this problem needs (used for serving admission)
+// upper bound of the device bytes a runner of this problem allocates (every
+// buffer rounded up to kAllocAlign); used for serving admission
+constexpr size_t kAllocAlign = 256;
 int64_t estimate_footprint(const Problem &p, const Options &o);
 
 }  // namespace nmk

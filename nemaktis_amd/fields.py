@@ -31,16 +31,24 @@ def twisted_nematic(Nx, Ny, Nz, total_twist=math.pi / 2):
 
 
 def _gaussian_filter_fft(a, sigma_vox):
-    """Periodic Gaussian smoothing of a 3-D array with per-axis sigma (voxels)."""
-    out = np.fft.fftn(a)
+    """Periodic Gaussian smoothing of a real 3-D array with per-axis sigma
+    (voxels), by a real-to-complex FFT."""
+    try:  # multithreaded transforms when scipy is present
+        import scipy.fft as sfft
+        fwd = lambda x: sfft.rfftn(x, workers=-1)  # noqa: E731
+        inv = lambda x, shape: sfft.irfftn(x, s=shape, workers=-1)  # noqa: E731
+    except ImportError:
+        fwd = np.fft.rfftn
+        inv = lambda x, shape: np.fft.irfftn(x, s=shape)  # noqa: E731
+    out = fwd(a)
     for ax, s in enumerate(sigma_vox):
         n = a.shape[ax]
-        k = np.fft.fftfreq(n) * 2 * math.pi
+        k = (np.fft.rfftfreq(n) if ax == a.ndim - 1 else np.fft.fftfreq(n)) * 2 * math.pi
         g = np.exp(-0.5 * (s * k) ** 2)
         shape = [1] * a.ndim
-        shape[ax] = n
+        shape[ax] = k.size
         out *= g.reshape(shape)
-    return np.fft.ifftn(out).real
+    return inv(out, a.shape)
 
 
 def random_smooth(Nx, Ny, Nz, spacing, seed, corr_length=1.0):
