@@ -32,22 +32,93 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+KERNELS = {"solve_x": "k_xsolve", "solve_y": "k_ysolve", "boundary": "k_boundary", "coefs": "k_coefs",
+           "material": "k_slab_material", "explicit": "k_explicit"}
+
+
 def algorithmic_bytes(fam, c, nprops, nslots, NP, Nfull, fd, nw):
     """Unique HBM bytes one launch of a kernel family must move: every array
-    element it reads or writes counted once (c = bytes per complex value)."""
+    element it reads or writes counted once (c = bytes per complex value,
+    NP march-grid points, Nfull full-grid points).
+      line solves, first launch of a half-step: per propagation the source
+      field (2 components) in, dst and acc (2 components each) out; per
+      wavelength slot 20 right-hand-side coefficient planes + 6 line-matrix
+      planes.  Neumann launches: source in, dst out, acc in and out (8
+      planes per propagation); 14 + 6 coefficient planes per slot.  The
+      family average weighs one first launch and nw Neumann launches."""
     if fam == "boundary":
         return 4 * nprops * NP * c + 6 * nslots * NP * c
     if fam == "explicit":
         return 4 * nprops * NP * c + 20 * nslots * NP * c
     if fam in ("solve_x", "solve_y"):
-        init = (6 * nprops + 6 * nslots) * NP * c
+        init = (6 * nprops + 26 * nslots) * NP * c
         neu = (8 * nprops + 20 * nslots) * NP * c
         return (init + nw * neu) / (1 + nw)
     if fam == "coefs":
-        return 15 * 8 * nslots * Nfull + 43 * nslots * NP * c
+        return 15 * 8 * nslots * Nfull + 49 * nslots * NP * c
     if fam == "material":
         return 2 * fd * 8 * Nfull + 15 * 8 * nslots * Nfull
     return None
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_random512.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain
+    copy); (None, None) when there is none."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == kernel and parts[1] == "hbm_bytes_per_launch":
+                    return float(parts[2]), os.path.relpath(TRAFFIC_FILE, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
+def probe(kind, steps):
+    """Short runs for hardware-counter passes (rocprofv3 --pmc ... -- python
+    bench.py --probe KIND): 'copy' copies 256 MiB device to device `steps`
+    times (counter calibration), 'march' runs the FP32 workload."""
+    if kind == "copy":
+        import torch
+        n = 64 << 20
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: %d copies of %d bytes" % (steps, 4 * n))
+        return
+    from nemaktis_amd import fields, autotune, bpm_backend
+    s, lc, _, _ = fields.workload("random512")
+    tuning, _ = autotune.tune(s, lc, precision=32, wl_batch=0, use_cache=True, log=log)
+    js = json.dumps(s)
+    n_E = int(np.prod(fields.output_shape(s)))
+    with bpm_backend.Context(js, lc, n_E, precision=32, wl_batch=0, use_graph=False, tuning=tuning) as ctx:
+        for _ in range(steps):
+            st = ctx.propagate()
+    log("march probe: %d propagations, last %.2f ms, tuning %s" % (steps, st["total_ms"], tuning))
+
+
+def serve_leg(n_jobs, rate_hz, seed, max_active):
+    """Seeded Poisson trace of micrograph jobs against the serving loop."""
+    from nemaktis_amd import serving
+    specs = serving.poisson_trace(n_jobs, rate_hz, seed)
+    cache = {}
+    t0 = time.perf_counter()
+    for sp in specs:
+        serving.director(sp["mesh"], sp["generator"], cache)
+    log("serving trace: %d jobs, directors generated in %.1f s" % (n_jobs, time.perf_counter() - t0))
+    with serving.Server(max_active=max_active) as srv:
+        res = serving.run_trace(srv, specs, cache, precision=32, wl_batch=0, log=log)
+    res.update({"trace": "%d jobs, Poisson arrivals %.2f/s, seed %d, meshes 128^2x40 / 256^2x80 / 512^2x100 / "
+                         "1024^2x200, 1-21 wavelengths x 1-7 wavevectors x 2 polarisations, complex64"
+                         % (n_jobs, rate_hz, seed), "max_active": max_active})
+    return res
 
 
 def rel_errors(a, b):
@@ -110,6 +181,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="built-in line-solver shapes instead of the autotuner")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop trace")
+    ap.add_argument("--serve-jobs", type=int, default=8)
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="mean job arrivals per second")
+    ap.add_argument("--serve-active", type=int, default=4, help="jobs on the device at once")
+    ap.add_argument("--probe", choices=["copy", "march"], help="short run for a hardware-counter pass")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -118,6 +194,9 @@ def main():
               % max(args.gpus, world), file=sys.stderr)
         sys.exit(2)
 
+    if args.probe:
+        probe(args.probe, args.steps)
+        return
     from nemaktis_amd import _lib, fields
     _lib.load()
     Nx = Ny = 512
@@ -139,7 +218,7 @@ def main():
     if not args.no_tune:
         from nemaktis_amd import autotune
         log("autotuning the line-solver shapes (FP32, shallow copy of the job) ...")
-        tuning, _ = autotune.tune(s, lc, precision=32, wl_batch=0, use_cache=False, log=log)
+        tuning, _ = autotune.tune(s, lc, precision=32, wl_batch=0, use_cache=True, log=log)
 
     log("FP32 batched path (graph) ...")
     wall32, dev32, st32, out32 = run_gpu_leg(js, lc, n_E, shape, 32, 0, True, args.steps, args.warmup, tuning)
@@ -164,11 +243,11 @@ def main():
     Mx, My = Nx - 1, Ny - 1
     algo = algorithmic_bytes(dom, 8, 2, 1, Mx * My, Nx * Ny, 3, 2)
     achieved = algo / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_line_solve" if dom.startswith("solve") else "k_" + dom,
-                "family": dom, "avg_launch_ms": avg_ms, "launches": fam_n[dom],
-                "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "algorithmic_bytes_per_launch": algo,
+    traffic, traffic_src = measured_traffic(KERNELS.get(dom, dom))
+    roofline = {"bound": "hbm", "kernel": KERNELS.get(dom, dom), "family": dom, "avg_launch_ms": avg_ms,
+                "launches": fam_n[dom], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": traffic_src, "algorithmic_bytes_per_launch": algo,
                 "family_share": {f: round(fam_ms[f] / max(1e-9, sum(fam_ms.values())), 4) for f in FAMILIES}}
 
     cpu = None
@@ -179,8 +258,14 @@ def main():
                "sample": "FP64 oracle, 512x512x%d slice of the same director (%d slabs, 2 polarisations), "
                          "%.1f s wall" % (args.cpu_nz, args.cpu_nz - 1, dt)}
 
+    serve = None
+    if not args.no_serve:
+        log("serving loop trace ...")
+        serve = serve_leg(args.serve_jobs, args.serve_rate, 5, args.serve_active)
+
     line = {
-        "metric": "voxel-slice updates/s at 512x512x100 (FP32 path; FP64 plain path beside it)",
+        "metric": "voxel-slice updates/s at 512x512x100 (FP32 path; FP64 plain path beside it); "
+                  "p50/p99 micrograph latency of the serving trace",
         "value": value, "unit": "voxel-slice updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "c64 (FP64 coefficient precompute)", "data": "synthetic",
@@ -190,6 +275,7 @@ def main():
         "fp64_plain": fp64,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serve,
     }
     if cpu:
         line["vs_cpu"] = value / cpu["value"]

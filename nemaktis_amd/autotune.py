@@ -18,6 +18,8 @@ import numpy as np
 from . import bpm_backend, fields
 
 _CACHE_ENV = "NEMAKTIS_AMD_TUNING_CACHE"
+# LDS of one CDNA4 compute unit; shapes the device refuses are skipped when timed
+LDS_MAX = 160 * 1024
 
 
 def _pow2s(lo, hi):
@@ -54,7 +56,7 @@ def x_candidates(Mx, precision):
         if lmax == 0 or lmax > 16 or P != chunks:
             continue
         for rows in (1, 2, 4, 8):
-            if 2 * P * rows <= 512 and _lds(Mx, chunks, rows, c, 2, True) <= 64 * 1024:
+            if 2 * P * rows <= 512 and _lds(Mx, chunks, rows, c, 2, True) <= LDS_MAX:
                 out.append((chunks, rows))
     return out
 
@@ -66,8 +68,8 @@ def y_candidates(My, precision):
         P, lmax = _plan(My, chunks)
         if lmax == 0 or lmax > 16 or P != chunks:
             continue
-        for cols in (1, 2, 4, 8, 16):
-            if P * cols <= 512 and _lds(My, chunks, cols, c, 1, False) <= 64 * 1024:
+        for cols in (1, 2, 4, 8, 16, 32):
+            if P * cols <= 512 and _lds(My, chunks, cols, c, 1, False) <= LDS_MAX:
                 out.append((chunks, cols))
     return out
 
@@ -98,12 +100,15 @@ def _save_cache(c):
 
 
 def cache_key(precision, Nx, Ny, nprops):
-    return "gfx950|pair-solver|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
+    return "gfx950|solver-v3|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
 
 
 def _time(js, lc, n_E, precision, wl_batch, tuning, reps):
-    with bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, use_graph=True,
-                             tuning=tuning) as ctx:
+    try:
+        ctx = bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, use_graph=True, tuning=tuning)
+    except bpm_backend.BpmBackendError:
+        return float("inf")  # shape beyond the device limits
+    with ctx:
         ctx.propagate()
         t = []
         for _ in range(reps):
@@ -133,7 +138,10 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
     xs, ys = x_candidates(Mx, precision), y_candidates(My, precision)
     if not xs or not ys:
         return None, []
-    best = {"x_chunks": xs[0][0], "x_rows": xs[0][1], "y_chunks": ys[0][0], "y_cols": ys[0][1]}
+    c = 8 if precision == 32 else 16
+    x0 = next((x for x in xs if _lds(Mx, x[0], x[1], c, 2, True) <= 64 * 1024), xs[0])
+    y0 = next((y for y in ys if _lds(My, y[0], y[1], c, 1, False) <= 64 * 1024), ys[0])
+    best = {"x_chunks": x0[0], "x_rows": x0[1], "y_chunks": y0[0], "y_cols": y0[1]}
     table = []
     t0 = time.time()
     best_ms = None
@@ -152,6 +160,8 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
     if log:
         log("autotune %s: %d candidates in %.1f s, best %s (%.3f ms for %d slabs)"
             % (key, len(table), time.time() - t0, best, best_ms, depth - 1))
+        for c, ms in sorted(table, key=lambda cm: cm[1])[:8]:
+            log("   %.3f ms  %s" % (ms, c))
     if use_cache:
         cache[key] = {"tuning": best, "table": [[c, m] for c, m in table]}
         _save_cache(cache)

@@ -535,11 +535,11 @@ private:
         if (g_.px) lpx_ = make_line_plan(g_.Mx, t.x_chunks);
         if (g_.py) lpy_ = make_line_plan(g_.My, t.y_chunks);
         if (g_.px) {
-            if (2 * lpx_.P * t.x_rows > 512 || xsolve_lds_bytes<T>(lpx_, t.x_rows) > 64 * 1024)
+            if (2 * lpx_.P * t.x_rows > 512 || xsolve_lds_bytes<T>(lpx_, t.x_rows) > max_block_lds())
                 throw std::runtime_error("x line solver tuning exceeds the block limits");
             xt_.alloc(2 * (size_t)sz_.nslots_max * 2 * 3 * g_.My * lpx_.lmax * lpx_.P);
         }
-        if (g_.py && (lpy_.P * t.y_cols > 512 || ysolve_lds_bytes<T>(lpy_, t.y_cols) > 64 * 1024))
+        if (g_.py && (lpy_.P * t.y_cols > 512 || ysolve_lds_bytes<T>(lpy_, t.y_cols) > max_block_lds()))
             throw std::runtime_error("y line solver tuning exceeds the block limits");
         for (auto &kv : graphs_) {
             (void)hipGraphExecDestroy(kv.second.exec);

@@ -46,6 +46,8 @@ void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, cons
 //   y lines, neumann == 0: dst = M_y (I + mu A_x) src, acc = dst
 //   y lines, neumann == 1: dst = M_y (mu O_y src),     acc += dst
 LinePlan make_line_plan(int M, int chunks);
+// LDS bytes one block may hold on the current device
+size_t max_block_lds();
 template <class T>
 size_t xsolve_lds_bytes(const LinePlan &lp, int R);
 template <class T>

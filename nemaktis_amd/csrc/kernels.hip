@@ -18,6 +18,10 @@
 
 #include "kernels.h"
 
+#include <map>
+#include <mutex>
+#include <string>
+
 namespace nmk {
 
 #define NMK_LAUNCH_CHECK()                                                                       \
@@ -1131,15 +1135,18 @@ __device__ __forceinline__ void rhs_single(int comp, unsigned xa, unsigned ya, u
 }
 
 // ---- y lines: W adjacent columns x P chunks per block, one propagation.
-// Blocks b and b+8 (the same XCD under round-robin dispatch) hold the two
-// polarisations of the same columns, so the second finds the coefficient
-// planes in L2.  The right-hand sides of the W x M tile are computed by all
+// Blocks b, b+8, ..., b+8(2G-1) (the same XCD under round-robin dispatch)
+// hold G adjacent column strips for both polarisations, so that narrow
+// strips share the 128-byte lines of the fields and coefficient planes in
+// L2, as do the two polarisations.  The right-hand sides of the W x M tile are computed by all
 // threads (consecutive threads on consecutive columns) and staged in LDS in
 // the chunk-interleaved order ((i*P + p)*W + column); the chunk threads read
 // the tridiagonal rows from global memory one W-wide row segment at a time,
 // and the solution goes back through LDS to row-ordered stores.
 //   neumann 0: dst = M_y (I + mu A_x) src, acc = dst
 //   neumann 1: dst = M_y (mu O_y src),     acc += dst
+constexpr int kYGroup = 8;  // adjacent column strips per XCD group
+
 template <class T, int LMAX, int NEU>
 __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
                                                  C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf, Geo g,
@@ -1149,7 +1156,8 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     const int P = lp.P, M = g.My, TL = LMAX * P * W;
     C *sd = reinterpret_cast<C *>(smem_raw);
     const RedLds<T> red(sd + TL, W * P);
-    const int bx = blockIdx.x, lb = (bx >> 4) * 8 + (bx & 7), pol = (bx >> 3) & 1;
+    const int bx = blockIdx.x, jg = (bx >> 3) % (2 * kYGroup), pol = jg / kYGroup;
+    const int lb = ((bx / (16 * kYGroup)) * 8 + (bx & 7)) * kYGroup + jg % kYGroup;
     if (lb * W >= g.Mx) return;  // padding blocks of the XCD grouping
     const int x0 = lb * W, comp = blockIdx.y, prop = blockIdx.z * 2 + pol;
     const int slot = pp[prop].slot;
@@ -1252,6 +1260,29 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     }
 }
 
+// dynamic LDS above the 64 KiB default needs an opt-in per kernel
+size_t max_block_lds() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+        return 64 * 1024;
+    return (size_t)v;
+}
+
+template <class K>
+static void allow_lds(K kernel, size_t lds) {
+    if (lds <= 64 * 1024) return;
+    static std::mutex mu;
+    static std::map<const void *, size_t> granted;
+    std::lock_guard<std::mutex> lk(mu);
+    const void *f = reinterpret_cast<const void *>(kernel);
+    auto it = granted.find(f);
+    if (it != granted.end() && it->second >= lds) return;
+    if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        throw std::runtime_error("line solver: the device refuses " + std::to_string(lds) + " bytes of LDS per block");
+    granted[f] = lds;
+}
+
 LinePlan make_line_plan(int M, int chunks) {
     LinePlan lp{};
     lp.M = M;
@@ -1284,9 +1315,18 @@ static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *s
     const dim3 block(2 * lp.P * R);
     const size_t lds = xsolve_lds_bytes<T>(lp, R);
     switch (lp.lmax) {
-        case 4: hipLaunchKernelGGL((k_xsolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R); break;
-        case 8: hipLaunchKernelGGL((k_xsolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R); break;
-        case 16: hipLaunchKernelGGL((k_xsolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R); break;
+        case 4:
+            allow_lds(k_xsolve<T, 4, NEU>, lds);
+            hipLaunchKernelGGL((k_xsolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R);
+            break;
+        case 8:
+            allow_lds(k_xsolve<T, 8, NEU>, lds);
+            hipLaunchKernelGGL((k_xsolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R);
+            break;
+        case 16:
+            allow_lds(k_xsolve<T, 16, NEU>, lds);
+            hipLaunchKernelGGL((k_xsolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R);
+            break;
         default: throw std::runtime_error("x line solver: unsupported chunk length");
     }
     NMK_LAUNCH_CHECK();
@@ -1298,9 +1338,18 @@ static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *s
     const dim3 block(lp.P * W);
     const size_t lds = ysolve_lds_bytes<T>(lp, W);
     switch (lp.lmax) {
-        case 4: hipLaunchKernelGGL((k_ysolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
-        case 8: hipLaunchKernelGGL((k_ysolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
-        case 16: hipLaunchKernelGGL((k_ysolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W); break;
+        case 4:
+            allow_lds(k_ysolve<T, 4, NEU>, lds);
+            hipLaunchKernelGGL((k_ysolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W);
+            break;
+        case 8:
+            allow_lds(k_ysolve<T, 8, NEU>, lds);
+            hipLaunchKernelGGL((k_ysolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W);
+            break;
+        case 16:
+            allow_lds(k_ysolve<T, 16, NEU>, lds);
+            hipLaunchKernelGGL((k_ysolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W);
+            break;
         default: throw std::runtime_error("y line solver: unsupported chunk length");
     }
     NMK_LAUNCH_CHECK();
@@ -1321,8 +1370,8 @@ void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const 
                    const PropParams *pp, int nprops, const LinePlan &lp, int W, hipStream_t st) {
     if (lp.P * W > 512 || lp.M != g.My || W < 1) throw std::runtime_error("y line solver: bad launch shape");
     if (nprops % 2) throw std::runtime_error("y line solver: propagations come in polarisation pairs");
-    const int nlb = (g.Mx + W - 1) / W;
-    dim3 grid((unsigned)((nlb + 7) / 8 * 16), 2, (unsigned)(nprops / 2));
+    const int nlb = (g.Mx + W - 1) / W, ngroups = (nlb + kYGroup - 1) / kYGroup;
+    dim3 grid((unsigned)((ngroups + 7) / 8 * 16 * kYGroup), 2, (unsigned)(nprops / 2));
     if (neumann) ysolve_dispatch<T, 1>(lp, W, grid, src, dst, acc, cf, g, pp, st);
     else ysolve_dispatch<T, 0>(lp, W, grid, src, dst, acc, cf, g, pp, st);
 }
