@@ -14,6 +14,22 @@
  * stores each sparse sub-matrix as explicit 3- or 4-point stencils, and
  * follows the reference's operation order.  Citations are relative to the
  * reference repository root.
+ *
+ * How each part is pinned (tests/test_oracle_pins.py): permittivity and
+ * sqrt(K) by closed form and S*S = K; the phase operator by a 2x2 matrix
+ * exponential; cyclic / block line solves by dense solves; the R
+ * coefficients by second-order convergence to the continuum operator; the ADI
+ * step by its exact Fourier symbol in homogeneous anisotropic media; the whole
+ * march by x,y-invariant media (products of 2x2 matrices); Fresnel by the
+ * textbook isotropic transmission; the shift by translation and linear
+ * precision; the FFT by numpy; the screen filter by the NA low-pass, a single
+ * layer at normal incidence and the angular-spectrum propagator.
+ * Parity unpinned (no reference vectors and no closed form): the bit-level
+ * composition of the R assembly in inhomogeneous media with walk-off, the
+ * reference's Eigen summation order and FFTW rounding, the mask-formula
+ * bisection weights, the outcome of the reference's racy W loop (the full
+ * sum is computed here) and the seam rule (+x/+y neighbours read the
+ * duplicate column/row), which is followed from reading the reference only.
  */
 #ifndef NEMAKTIS_BPM_ORACLE_H
 #define NEMAKTIS_BPM_ORACLE_H
