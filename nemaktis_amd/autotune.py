@@ -40,10 +40,10 @@ def _plan(M, chunks):
     return P, lmax
 
 
-def _lds(M, chunks, lines, c, pols, smap):
+def _lds(M, chunks, lines, c, pols, smap=True):
     """LDS bytes of one line-solver block: the staged right-hand sides and the
     seven reduced-system arrays of each polarisation held by the block
-    (x blocks hold both, y blocks one), and the x solver's slot map."""
+    (x blocks hold both, y blocks one), and the element-to-slot map."""
     P, lmax = _plan(M, chunks)
     return pols * lines * (lmax * P + 7 * P) * c + (M * 4 if smap else 0)
 
@@ -69,7 +69,7 @@ def y_candidates(My, precision):
         if lmax == 0 or lmax > 16 or P != chunks:
             continue
         for cols in (1, 2, 4, 8, 16, 32):
-            if P * cols <= 512 and _lds(My, chunks, cols, c, 1, False) <= LDS_MAX:
+            if P * cols <= 512 and _lds(My, chunks, cols, c, 1, True) <= LDS_MAX:
                 out.append((chunks, cols))
     return out
 
@@ -140,7 +140,7 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
         return None, []
     c = 8 if precision == 32 else 16
     x0 = next((x for x in xs if _lds(Mx, x[0], x[1], c, 2, True) <= 64 * 1024), xs[0])
-    y0 = next((y for y in ys if _lds(My, y[0], y[1], c, 1, False) <= 64 * 1024), ys[0])
+    y0 = next((y for y in ys if _lds(My, y[0], y[1], c, 1, True) <= 64 * 1024), ys[0])
     best = {"x_chunks": x0[0], "x_rows": x0[1], "y_chunks": y0[0], "y_cols": y0[1]}
     table = []
     t0 = time.time()

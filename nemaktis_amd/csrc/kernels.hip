@@ -204,10 +204,15 @@ void launch_slab_material(const double *lc, const uint8_t *mask, const double *w
 //   R  = i W + (D S^-1 + S^-1 D) / 4,    W = (v_a d_b + d_a v_b) / 2
 // for the three stencil families DX, DY, DXDY and their 2x2 blocks.
 // ---------------------------------------------------------------------------
+// Under round-robin dispatch XCD k receives blocks k, k+8, k+16, ...; this
+// remap gives them consecutive tiles (nbx = gridDim.x, a multiple of 8), so
+// the neighbouring rows a tile reads were just read into the same L2.
+__device__ __forceinline__ int xcd_tile(int bx, int nbx) { return (bx & 7) * (nbx >> 3) + (bx >> 3); }
+
 template <class T>
 __global__ __launch_bounds__(256) void k_coefs(const double *__restrict__ sm, Geo g, const SlotParams *__restrict__ sp,
                                                C2<T> *__restrict__ cf, C2<T> *__restrict__ xt, LinePlan lpx) {
-    const int64_t ip = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t ip = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (ip >= g.NP) return;
     const int slot = blockIdx.y;
     const SlotParams s = sp[slot];
@@ -356,7 +361,8 @@ __global__ __launch_bounds__(256) void k_coefs(const double *__restrict__ sm, Ge
 template <class T>
 void launch_coefs(const double *sm, const Geo &g, const SlotParams *sp, int nslots, C2<T> *cf, C2<T> *xt,
                   const LinePlan &lpx, hipStream_t st) {
-    hipLaunchKernelGGL(k_coefs<T>, dim3(blocks_for(g.NP, 256), nslots), dim3(256), 0, st, sm, g, sp, cf, xt, lpx);
+    hipLaunchKernelGGL(k_coefs<T>, dim3((blocks_for(g.NP, 256) + 7) / 8 * 8, nslots), dim3(256), 0, st, sm, g, sp, cf,
+                       xt, lpx);
     NMK_LAUNCH_CHECK();
 }
 
@@ -965,30 +971,13 @@ __device__ __forceinline__ void reduced_pcr(const RedLds<T> &r, int P, int p, in
     __syncthreads();
 }
 
-// dst = v and acc (+)= v at one or two field positions of both
-// polarisations, loads first (one round trip for both points)
-template <class T, int NEU>
-__device__ __forceinline__ void store_pair2(C2<T> *__restrict__ d0, C2<T> *__restrict__ d1,
-                                            C2<T> *__restrict__ a0, C2<T> *__restrict__ a1, unsigned qa,
-                                            unsigned qb, bool vb, const C2<T> (&va)[2], const C2<T> (&vb2)[2]) {
-    C2<T> oa0{0, 0}, oa1{0, 0}, ob0{0, 0}, ob1{0, 0};
-    if (NEU) {
-        oa0 = a0[qa];
-        oa1 = a1[qa];
-        if (vb) {
-            ob0 = a0[qb];
-            ob1 = a1[qb];
-        }
-    }
-    d0[qa] = va[0];
-    d1[qa] = va[1];
-    a0[qa] = NEU ? oa0 + va[0] : va[0];
-    a1[qa] = NEU ? oa1 + va[1] : va[1];
-    if (vb) {
-        d0[qb] = vb2[0];
-        d1[qb] = vb2[1];
-        a0[qb] = NEU ? ob0 + vb2[0] : vb2[0];
-        a1[qb] = NEU ? ob1 + vb2[1] : vb2[1];
+// (row, column) of flat element e + step of rows of length M, from those of e
+// (no integer division in the loops)
+__device__ __forceinline__ void advance_rc(int &r, int &x, int step, int M) {
+    x += step;
+    while (x >= M) {
+        x -= M;
+        r++;
     }
 }
 
@@ -1026,10 +1015,11 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     // right-hand sides of the block's rows, both polarisations
     {
         constexpr int KIND = NEU ? RHS_OX : RHS_YPLUS, NPT = rhs_points<T>();
+        int r0 = (int)threadIdx.x / M, x0 = (int)threadIdx.x - r0 * M;
         for (int e0 = threadIdx.x; e0 < ntot; e0 += NPT * bs) {
-            const int e1 = e0 + bs;
-            const bool v1 = NPT == 2 && e1 < ntot;
-            const int r0 = e0 / M, x0 = e0 - r0 * M, r1 = v1 ? e1 / M : 0, x1 = v1 ? e1 - r1 * M : 0;
+            const bool v1 = NPT == 2 && e0 + bs < ntot;
+            int r1 = r0, x1 = x0;
+            advance_rc(r1, x1, bs, M);
             C da[2], db[2];
             rhs_pair2<T, KIND>(comp, x0, row0 + r0, x1, row0 + r1, v1, cfs, fin, g, da, db);
             const int ia = r0 * SL + smap[x0];
@@ -1040,6 +1030,9 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
                 sd0[ib] = db[0];
                 sd1[ib] = db[1];
             }
+            r0 = r1;
+            x0 = x1;
+            if (NPT == 2) advance_rc(r0, x0, bs, M);
         }
     }
     __syncthreads();
@@ -1089,16 +1082,40 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     __syncthreads();
 
     {
+        // a thread stores at most LMAX/2 points (nrow*M <= R*P*LMAX, 2*R*P threads);
+        // the accumulator values of all of them are fetched in one round trip
         const size_t o0 = ((size_t)prop0 * 2 + comp) * NP, o1 = o0 + 2 * NP;
         C *d0 = dst + o0, *d1 = dst + o1, *a0 = acc + o0, *a1 = acc + o1;
-        for (int e0 = threadIdx.x; e0 < ntot; e0 += 2 * bs) {
-            const int e1 = e0 + bs;
-            const bool v1 = e1 < ntot;
-            const int r0 = e0 / M, x0 = e0 - r0 * M, r1 = v1 ? e1 / M : r0, x1 = v1 ? e1 - r1 * M : x0;
-            const int ia = r0 * SL + smap[x0], ib = r1 * SL + smap[x1];
-            const C va[2] = {sd0[ia], sd1[ia]}, vb[2] = {sd0[ib], sd1[ib]};
-            store_pair2<T, NEU>(d0, d1, a0, a1, (unsigned)(x0 + M * (row0 + r0)), (unsigned)(x1 + M * (row0 + r1)),
-                                v1, va, vb);
+        constexpr int NO = LMAX / 2;
+        C o0v[NO], o1v[NO];
+        const int rs = (int)threadIdx.x / M, xs = (int)threadIdx.x - rs * M;
+        if (NEU) {
+            int rr = rs, x = xs;
+#pragma unroll
+            for (int k = 0; k < NO; k++) {
+                const int e = threadIdx.x + k * bs;
+                if (e < ntot) {
+                    const unsigned q = x + M * (row0 + rr);
+                    o0v[k] = a0[q];
+                    o1v[k] = a1[q];
+                }
+                advance_rc(rr, x, bs, M);
+            }
+        }
+        int rr = rs, x = xs;
+#pragma unroll
+        for (int k = 0; k < NO; k++) {
+            const int e = threadIdx.x + k * bs;
+            if (e < ntot) {
+                const int ia = rr * SL + smap[x];
+                const unsigned q = x + M * (row0 + rr);
+                const C v0 = sd0[ia], v1 = sd1[ia];
+                d0[q] = v0;
+                d1[q] = v1;
+                a0[q] = NEU ? o0v[k] + v0 : v0;
+                a1[q] = NEU ? o1v[k] + v1 : v1;
+            }
+            advance_rc(rr, x, bs, M);
         }
     }
 }
@@ -1156,6 +1173,7 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     const int P = lp.P, M = g.My, TL = LMAX * P * W;
     C *sd = reinterpret_cast<C *>(smem_raw);
     const RedLds<T> red(sd + TL, W * P);
+    int *ymap = reinterpret_cast<int *>(sd + TL + RedLds<T>::kArrays * W * P);  // row j -> i*P + p
     const int bx = blockIdx.x, jg = (bx >> 3) % (2 * kYGroup), pol = jg / kYGroup;
     const int lb = ((bx / (16 * kYGroup)) * 8 + (bx & 7)) * kYGroup + jg % kYGroup;
     if (lb * W >= g.Mx) return;  // padding blocks of the XCD grouping
@@ -1164,31 +1182,33 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     const size_t NP = (size_t)g.NP;
     const C *cfs = cf + (size_t)slot * cf_slot_size(g.NP);
     const C *f0 = src + (size_t)prop * 2 * NP, *f1 = f0 + NP;
-    const int wv = min(W, g.Mx - x0), ntile = M * wv, bs = blockDim.x;
+    const int wv = min(W, g.Mx - x0);
+    if ((int)threadIdx.x < P) {
+        const int pc = threadIdx.x, s0c = chunk_start(lp, pc), lc = chunk_len(lp, pc);
+        for (int i = 0; i < lc; i++) ymap[s0c + i] = i * P + pc;
+    }
+    __syncthreads();
+    // W is a power of two and the block has P*W threads: a thread keeps its
+    // column and steps P rows at a time
+    const int cc = threadIdx.x & (W - 1), j0 = (int)threadIdx.x >> (__ffs(W) - 1);
+    const bool col_ok = cc < wv;
 
     // right-hand sides of the tile
-    {
+    if (col_ok) {
         constexpr int KIND = NEU ? RHS_OY : RHS_XPLUS, NPT = rhs_points<T>();
-        for (int e0 = threadIdx.x; e0 < ntile; e0 += NPT * bs) {
-            const int e1 = e0 + bs;
-            const bool v1 = NPT == 2 && e1 < ntile;
-            const int j0 = e0 / wv, c0 = e0 - j0 * wv, j1 = v1 ? e1 / wv : 0, c1 = v1 ? e1 - j1 * wv : 0;
+        for (int j = j0; j < M; j += NPT * P) {
+            const int j1 = j + P;
+            const bool v1 = NPT == 2 && j1 < M;
             C da{0, 0}, db{0, 0};
-            rhs_single<T, KIND>(comp, x0 + c0, j0, x0 + c1, j1, v1, cfs, f0, f1, g, da, db);
-            int p0, i0;
-            chunk_of(lp, j0, p0, i0);
-            sd[(i0 * P + p0) * W + c0] = da;
-            if (v1) {
-                int p1, i1;
-                chunk_of(lp, j1, p1, i1);
-                sd[(i1 * P + p1) * W + c1] = db;
-            }
+            rhs_single<T, KIND>(comp, x0 + cc, j, x0 + cc, j1, v1, cfs, f0, f1, g, da, db);
+            sd[ymap[j] * W + cc] = da;
+            if (v1) sd[ymap[j1] * W + cc] = db;
         }
     }
     __syncthreads();
 
-    const int cofs = threadIdx.x % W, p = threadIdx.x / W, x = x0 + cofs;
-    const bool active = cofs < wv;
+    const int cofs = cc, p = j0, x = x0 + cofs;
+    const bool active = col_ok;
     const int blk = comp == 0 ? 0 : 3;
     const C *cdy = cfs + (CF_DY + blk * 3) * kCfTile;
     const int s = chunk_start(lp, p), n = chunk_len(lp, p) - 1;
@@ -1235,26 +1255,27 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     __syncthreads();
 
     {
+        // a thread stores at most LMAX points of its column (M <= P*LMAX);
+        // the accumulator values of all of them are fetched in one round trip
         C *dd = dst + ((size_t)prop * 2 + comp) * NP, *aa = acc + ((size_t)prop * 2 + comp) * NP;
-        for (int e0 = threadIdx.x; e0 < ntile; e0 += 2 * bs) {
-            const int e1 = e0 + bs;
-            const bool v1 = e1 < ntile;
-            const int j0 = e0 / wv, c0 = e0 - j0 * wv, j1 = v1 ? e1 / wv : j0, c1 = v1 ? e1 - j1 * wv : c0;
-            int p0, i0, p1, i1;
-            chunk_of(lp, j0, p0, i0);
-            chunk_of(lp, j1, p1, i1);
-            const C va = sd[(i0 * P + p0) * W + c0], vb = sd[(i1 * P + p1) * W + c1];
-            const unsigned qa = x0 + c0 + g.Mx * j0, qb = x0 + c1 + g.Mx * j1;
-            C oa{0, 0}, ob{0, 0};
-            if (NEU) {
-                oa = aa[qa];
-                if (v1) ob = aa[qb];
+        C ov[LMAX];
+        if (NEU && col_ok) {
+#pragma unroll
+            for (int k = 0; k < LMAX; k++) {
+                const int j = j0 + k * P;
+                if (j < M) ov[k] = aa[x0 + cc + g.Mx * j];
             }
-            dd[qa] = va;
-            aa[qa] = NEU ? oa + va : va;
-            if (v1) {
-                dd[qb] = vb;
-                aa[qb] = NEU ? ob + vb : vb;
+        }
+        if (col_ok) {
+#pragma unroll
+            for (int k = 0; k < LMAX; k++) {
+                const int j = j0 + k * P;
+                if (j < M) {
+                    const C v = sd[ymap[j] * W + cc];
+                    const unsigned q = x0 + cc + g.Mx * j;
+                    dd[q] = v;
+                    aa[q] = NEU ? ov[k] + v : v;
+                }
             }
         }
     }
@@ -1306,7 +1327,8 @@ size_t xsolve_lds_bytes(const LinePlan &lp, int R) {
 }
 template <class T>
 size_t ysolve_lds_bytes(const LinePlan &lp, int W) {
-    return (size_t)W * ((size_t)lp.lmax * lp.P + (size_t)RedLds<T>::kArrays * lp.P) * sizeof(C2<T>);
+    return (size_t)W * ((size_t)lp.lmax * lp.P + (size_t)RedLds<T>::kArrays * lp.P) * sizeof(C2<T>) +
+           (size_t)lp.M * sizeof(int);
 }
 
 template <class T, int NEU>
@@ -1358,7 +1380,8 @@ static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *s
 template <class T>
 void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
                    const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, hipStream_t st) {
-    if (2 * lp.P * R > 512 || lp.M != g.Mx || R < 1) throw std::runtime_error("x line solver: bad launch shape");
+    if (2 * lp.P * R > 512 || lp.M != g.Mx || R < 1 || lp.P * lp.lmax < lp.M)
+        throw std::runtime_error("x line solver: bad launch shape");
     if (nprops % 2) throw std::runtime_error("x line solver: propagations come in polarisation pairs");
     dim3 grid((unsigned)((g.My + R - 1) / R), 2, (unsigned)(nprops / 2));
     if (neumann) xsolve_dispatch<T, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, st);
@@ -1368,7 +1391,8 @@ void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const 
 template <class T>
 void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
                    const PropParams *pp, int nprops, const LinePlan &lp, int W, hipStream_t st) {
-    if (lp.P * W > 512 || lp.M != g.My || W < 1) throw std::runtime_error("y line solver: bad launch shape");
+    if (lp.P * W > 512 || lp.M != g.My || W < 1 || (W & (W - 1)) || lp.P * lp.lmax < lp.M)
+        throw std::runtime_error("y line solver: bad launch shape");
     if (nprops % 2) throw std::runtime_error("y line solver: propagations come in polarisation pairs");
     const int nlb = (g.Mx + W - 1) / W, ngroups = (nlb + kYGroup - 1) / kYGroup;
     dim3 grid((unsigned)((ngroups + 7) / 8 * 16 * kYGroup), 2, (unsigned)(nprops / 2));
