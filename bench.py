@@ -229,11 +229,16 @@ def main():
     if not args.no_fp64:
         log("FP64 plain path ...")
         k64 = max(1, args.steps // 2)
-        wall64, dev64, st64, out64 = run_gpu_leg(js, lc, n_E, shape, 64, 1, False, k64, 1)
+        tuning64 = None
+        if not args.no_tune:
+            from nemaktis_amd import autotune
+            tuning64, _ = autotune.tune(s, lc, precision=64, wl_batch=1, use_cache=True, log=log)
+        wall64, dev64, st64, out64 = run_gpu_leg(js, lc, n_E, shape, 64, 1, False, k64, 1, tuning64)
         errs = rel_errors(out32, out64)
         fp64 = {"value": updates * k64 / wall64, "ms_per_step": 1e3 * wall64 / k64,
                 "fp32_speedup": (updates * args.steps / wall32) / (updates * k64 / wall64),
-                "fp32_rel_l2_max": float(errs.max()), "fp32_rel_l2_median": float(np.median(errs))}
+                "fp32_rel_l2_max": float(errs.max()), "fp32_rel_l2_median": float(np.median(errs)),
+                "line_solver_shapes": tuning64}
 
     log("profiled pass (kernel-family event timing) ...")
     fam_ms, fam_n = run_profile_leg(js, lc, n_E, 32, 0, max(1, min(args.steps, 3)), tuning)

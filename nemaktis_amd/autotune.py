@@ -100,7 +100,7 @@ def _save_cache(c):
 
 
 def cache_key(precision, Nx, Ny, nprops):
-    return "gfx950|solver-v3|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
+    return "gfx950|solver-v4|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
 
 
 def _time(js, lc, n_E, precision, wl_batch, tuning, reps):
@@ -153,6 +153,12 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
             best, best_ms = cand, ms
     for (yc, yw) in ys:
         cand = dict(best, y_chunks=yc, y_cols=yw)
+        ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+        table.append((cand, ms))
+        if ms < best_ms:
+            best, best_ms = cand, ms
+    for order in range(4):  # which items share an XCD's L2 (components or neighbours)
+        cand = dict(best, order=order)
         ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
         table.append((cand, ms))
         if ms < best_ms:

@@ -479,18 +479,21 @@ private:
         if (nsl > 1) ready[1] = coef(1);
 
         C2<T> *cur = fb_[1].ptr, *other = fb_[0].ptr, *A = fb_[2].ptr, *B = fb_[3].ptr;
+        // XCD work order: a lone polarisation pair shares L2 best across the two
+        // components, larger batches across neighbouring rows/strips (measured)
+        const int ord = tune_.order >= 0 ? tune_.order : (nprops <= 2 ? 3 : 0);
         HIPCHK(hipStreamWaitEvent(st_, ready[0], 0));
         timed(KF_BOUNDARY, [&] { launch_boundary<T>(other, cur, nullptr, CF(0), g_, pp_.ptr, nprops, 1, st_); });
         for (int k = 0; k < nsl; k++) {
             // x half-steps: other = (I - mu A_x)^-1 (I + mu A_y) cur by its Neumann series
             if (g_.px) {
                 timed(KF_SOLVE_X, [&] {
-                    launch_xsolve<T>(0, cur, A, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, st_);
+                    launch_xsolve<T>(0, cur, A, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, st_);
                 });
                 C2<T> *a = A, *b = B;
                 for (int it = 0; it < nw; it++) {
                     timed(KF_SOLVE_X, [&] {
-                        launch_xsolve<T>(1, a, b, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, st_);
+                        launch_xsolve<T>(1, a, b, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, st_);
                     });
                     std::swap(a, b);
                 }
@@ -500,12 +503,12 @@ private:
             // y half-steps: cur = (I - mu A_y)^-1 (I + mu A_x) other by its Neumann series
             if (g_.py) {
                 timed(KF_SOLVE_Y, [&] {
-                    launch_ysolve<T>(0, other, A, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, st_);
+                    launch_ysolve<T>(0, other, A, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, st_);
                 });
                 C2<T> *a = A, *b = B;
                 for (int it = 0; it < nw; it++) {
                     timed(KF_SOLVE_Y, [&] {
-                        launch_ysolve<T>(1, a, b, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, st_);
+                        launch_ysolve<T>(1, a, b, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, st_);
                     });
                     std::swap(a, b);
                 }
@@ -567,7 +570,7 @@ private:
 
 Tuning default_tuning(int precision, int Mx, int My) {
     Tuning t;
-    const size_t budget = 64 * 1024;
+    const size_t budget = max_block_lds();
     auto xlds = [&](int chunks, int rows) {
         const LinePlan lp = make_line_plan(Mx, chunks);
         return precision == 32 ? xsolve_lds_bytes<float>(lp, rows) : xsolve_lds_bytes<double>(lp, rows);
@@ -578,7 +581,7 @@ Tuning default_tuning(int precision, int Mx, int My) {
     };
     // chunks of at most 8 points (16 for lines longer than 1024)
     t.x_chunks = std::min(128, std::max(16, pow2_at_least((Mx + 7) / 8)));
-    t.x_rows = 4;
+    t.x_rows = 2;
     if (Mx > 3)
         while (t.x_rows > 1 && (2 * t.x_chunks * t.x_rows > 512 || xlds(t.x_chunks, t.x_rows) > budget)) t.x_rows /= 2;
     t.y_chunks = std::min(128, std::max(16, pow2_at_least((My + 7) / 8)));

@@ -722,42 +722,40 @@ struct PairFields {
     }
 };
 
-// points per thread per round of the right-hand-side gathers: two in FP32,
-// one in FP64 (register budget)
-template <class T>
+// points per thread per gather round: four in FP32 (two beside the registers
+// of 16-point chunks), one in FP64; stencils wider than 8 taps gather half of
+// them at a time
+template <class T, int LMAX>
 constexpr int rhs_points() {
-    return sizeof(T) == 4 ? 2 : 1;
+    return sizeof(T) == 4 ? (LMAX <= 8 ? 4 : 2) : 1;
 }
 
-// right-hand sides of component `comp` (uniform per block), both
-// polarisations, at one point or (FP32) two; every load is issued before any sum
-template <class T, int KIND, int COMP>
-__device__ __forceinline__ void rhs_pair_comp(unsigned xa, unsigned ya, unsigned xb, unsigned yb, bool vb,
-                                              const C2<T> *__restrict__ cfs, const PairFields<T> &f, const Geo &g,
-                                              C2<T> (&ra)[2], C2<T> (&rb)[2]) {
-    RhsGather2<T, KIND, COMP> a;
-    a.load(xa, ya, cfs, f.a0, f.a1, f.b0, f.b1, g);
-    if constexpr (rhs_points<T>() == 2) {
-        RhsGather2<T, KIND, COMP> b;
-        if (vb) b.load(xb, yb, cfs, f.a0, f.a1, f.b0, f.b1, g);
+// right-hand sides of component `comp` (uniform per block) for both
+// polarisations at NPR points (v[k] false: skipped); every load of a group of
+// points is issued before any sum
+template <class T, int KIND, int COMP, int NPR>
+__device__ __forceinline__ void rhs_pair_comp(const unsigned (&xs)[NPR], const unsigned (&ys)[NPR],
+                                              const bool (&v)[NPR], const C2<T> *__restrict__ cfs,
+                                              const PairFields<T> &f, const Geo &g, C2<T> (&r)[NPR][2]) {
+    constexpr int G = RhsGather2<T, KIND, COMP>::N > 8 && NPR > 1 ? NPR / 2 : NPR;
+#pragma unroll
+    for (int k0 = 0; k0 < NPR; k0 += G) {
+        RhsGather2<T, KIND, COMP> gs[G];
+#pragma unroll
+        for (int k = 0; k < G; k++)
+            if (v[k0 + k]) gs[k].load(xs[k0 + k], ys[k0 + k], cfs, f.a0, f.a1, f.b0, f.b1, g);
         __builtin_amdgcn_sched_barrier(0);
-        a.sum(ra[0], ra[1]);
-        if (vb) b.sum(rb[0], rb[1]);
-    } else {
-        (void)xb;
-        (void)yb;
-        (void)vb;
-        (void)rb;
-        __builtin_amdgcn_sched_barrier(0);
-        a.sum(ra[0], ra[1]);
+#pragma unroll
+        for (int k = 0; k < G; k++)
+            if (v[k0 + k]) gs[k].sum(r[k0 + k][0], r[k0 + k][1]);
     }
 }
-template <class T, int KIND>
-__device__ __forceinline__ void rhs_pair2(int comp, unsigned xa, unsigned ya, unsigned xb, unsigned yb, bool vb,
-                                          const C2<T> *__restrict__ cfs, const PairFields<T> &f, const Geo &g,
-                                          C2<T> (&ra)[2], C2<T> (&rb)[2]) {
-    if (comp == 0) rhs_pair_comp<T, KIND, 0>(xa, ya, xb, yb, vb, cfs, f, g, ra, rb);
-    else rhs_pair_comp<T, KIND, 1>(xa, ya, xb, yb, vb, cfs, f, g, ra, rb);
+template <class T, int KIND, int NPR>
+__device__ __forceinline__ void rhs_pair_n(int comp, const unsigned (&xs)[NPR], const unsigned (&ys)[NPR],
+                                           const bool (&v)[NPR], const C2<T> *__restrict__ cfs,
+                                           const PairFields<T> &f, const Geo &g, C2<T> (&r)[NPR][2]) {
+    if (comp == 0) rhs_pair_comp<T, KIND, 0, NPR>(xs, ys, v, cfs, f, g, r);
+    else rhs_pair_comp<T, KIND, 1, NPR>(xs, ys, v, cfs, f, g, r);
 }
 
 // explicit half-steps, used along an axis too short for line solves
@@ -992,14 +990,30 @@ template <class T, int LMAX, int NEU>
 __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
                                                  C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf,
                                                  const C2<T> *__restrict__ xt, Geo g,
-                                                 const PropParams *__restrict__ pp, LinePlan lp, int R) {
+                                                 const PropParams *__restrict__ pp, LinePlan lp, int R,
+                                                 int nitems_pairs, int order) {
     using C = C2<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int P = lp.P, SL = LMAX * P, M = g.Mx, RP = R * P;
     C *sd = reinterpret_cast<C *>(smem_raw);  // [pol][row][SL]
     const RedLds<T> red(sd + 2 * R * SL, 2 * RP);
     int *smap = reinterpret_cast<int *>(sd + 2 * R * SL + RedLds<T>::kArrays * 2 * RP);  // x -> i*P + p
-    const int row0 = blockIdx.x * R, comp = blockIdx.y, prop0 = blockIdx.z * 2;
+    // work items (component and row block, in the order `order & 1` selects,
+    // then polarisation pair); each XCD takes a contiguous range of them, so
+    // both components of a row block or the neighbouring row blocks meet in L2
+    const int nrb = (g.My + R - 1) / R;
+    const int item = xcd_tile(blockIdx.x, gridDim.x);
+    if (item >= nrb * 2 * nitems_pairs) return;
+    int comp, row0, prop0;
+    if (order & 1) {  // component fastest: both components of a row block together
+        comp = item & 1;
+        row0 = ((item >> 1) % nrb) * R;
+        prop0 = ((item >> 1) / nrb) * 2;
+    } else {  // row block fastest: neighbouring row blocks together
+        row0 = (item % nrb) * R;
+        comp = (item / nrb) & 1;
+        prop0 = (item / (2 * nrb)) * 2;
+    }
     if ((int)threadIdx.x < P) {
         const int pc = threadIdx.x, s0c = chunk_start(lp, pc), lc = chunk_len(lp, pc);
         for (int i = 0; i < lc; i++) smap[s0c + i] = i * P + pc;
@@ -1014,25 +1028,28 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
 
     // right-hand sides of the block's rows, both polarisations
     {
-        constexpr int KIND = NEU ? RHS_OX : RHS_YPLUS, NPT = rhs_points<T>();
+        constexpr int KIND = NEU ? RHS_OX : RHS_YPLUS, NPR = rhs_points<T, LMAX>();
         int r0 = (int)threadIdx.x / M, x0 = (int)threadIdx.x - r0 * M;
-        for (int e0 = threadIdx.x; e0 < ntot; e0 += NPT * bs) {
-            const bool v1 = NPT == 2 && e0 + bs < ntot;
-            int r1 = r0, x1 = x0;
-            advance_rc(r1, x1, bs, M);
-            C da[2], db[2];
-            rhs_pair2<T, KIND>(comp, x0, row0 + r0, x1, row0 + r1, v1, cfs, fin, g, da, db);
-            const int ia = r0 * SL + smap[x0];
-            sd0[ia] = da[0];
-            sd1[ia] = da[1];
-            if (v1) {
-                const int ib = r1 * SL + smap[x1];
-                sd0[ib] = db[0];
-                sd1[ib] = db[1];
+        for (int e0 = threadIdx.x; e0 < ntot; e0 += NPR * bs) {
+            unsigned xs[NPR], ys[NPR];
+            bool v[NPR];
+            int slot_of[NPR];
+#pragma unroll
+            for (int k = 0; k < NPR; k++) {
+                v[k] = e0 + k * bs < ntot;
+                xs[k] = x0;
+                ys[k] = row0 + r0;
+                slot_of[k] = v[k] ? r0 * SL + smap[x0] : 0;
+                advance_rc(r0, x0, bs, M);
             }
-            r0 = r1;
-            x0 = x1;
-            if (NPT == 2) advance_rc(r0, x0, bs, M);
+            C d[NPR][2];
+            rhs_pair_n<T, KIND, NPR>(comp, xs, ys, v, cfs, fin, g, d);
+#pragma unroll
+            for (int k = 0; k < NPR; k++)
+                if (v[k]) {
+                    sd0[slot_of[k]] = d[k][0];
+                    sd1[slot_of[k]] = d[k][1];
+                }
         }
     }
     __syncthreads();
@@ -1121,63 +1138,64 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
 }
 
 // right-hand sides of component `comp` (uniform per block) of one
-// propagation at one or (FP32) two points; every load is issued before any sum
-template <class T, int KIND, int COMP>
-__device__ __forceinline__ void rhs_single_comp(unsigned xa, unsigned ya, unsigned xb, unsigned yb, bool vb,
-                                                const C2<T> *__restrict__ cfs, const C2<T> *__restrict__ f0,
-                                                const C2<T> *__restrict__ f1, const Geo &g, C2<T> &ra, C2<T> &rb) {
-    RhsGather<T, KIND, COMP> a;
-    a.load(xa, ya, cfs, f0, f1, g, 0);
-    if constexpr (rhs_points<T>() == 2) {
-        RhsGather<T, KIND, COMP> b;
-        if (vb) b.load(xb, yb, cfs, f0, f1, g, 0);
+// propagation at NPT points of one column x, rows j0, j0 + step, ... (rows
+// >= M skipped); every load of every point is issued before any sum
+template <class T, int KIND, int COMP, int NPT>
+__device__ __forceinline__ void rhs_column_comp(unsigned x, int j0, int step, int M, const C2<T> *__restrict__ cfs,
+                                                const C2<T> *__restrict__ f0, const C2<T> *__restrict__ f1,
+                                                const Geo &g, C2<T> (&r)[NPT]) {
+    // wide stencils (more than 8 taps) gather half of the points at a time
+    constexpr int G = RhsGather<T, KIND, COMP>::N > 8 && NPT > 1 ? NPT / 2 : NPT;
+#pragma unroll
+    for (int k0 = 0; k0 < NPT; k0 += G) {
+        RhsGather<T, KIND, COMP> gs[G];
+#pragma unroll
+        for (int k = 0; k < G; k++)
+            if (j0 + (k0 + k) * step < M) gs[k].load(x, j0 + (k0 + k) * step, cfs, f0, f1, g, 0);
         __builtin_amdgcn_sched_barrier(0);
-        ra = a.sum();
-        if (vb) rb = b.sum();
-    } else {
-        (void)xb;
-        (void)yb;
-        (void)vb;
-        (void)rb;
-        __builtin_amdgcn_sched_barrier(0);
-        ra = a.sum();
+#pragma unroll
+        for (int k = 0; k < G; k++)
+            if (j0 + (k0 + k) * step < M) r[k0 + k] = gs[k].sum();
     }
 }
-template <class T, int KIND>
-__device__ __forceinline__ void rhs_single(int comp, unsigned xa, unsigned ya, unsigned xb, unsigned yb, bool vb,
+template <class T, int KIND, int NPT>
+__device__ __forceinline__ void rhs_column(int comp, unsigned x, int j0, int step, int M,
                                            const C2<T> *__restrict__ cfs, const C2<T> *__restrict__ f0,
-                                           const C2<T> *__restrict__ f1, const Geo &g, C2<T> &ra, C2<T> &rb) {
-    if (comp == 0) rhs_single_comp<T, KIND, 0>(xa, ya, xb, yb, vb, cfs, f0, f1, g, ra, rb);
-    else rhs_single_comp<T, KIND, 1>(xa, ya, xb, yb, vb, cfs, f0, f1, g, ra, rb);
+                                           const C2<T> *__restrict__ f1, const Geo &g, C2<T> (&r)[NPT]) {
+    if (comp == 0) rhs_column_comp<T, KIND, 0, NPT>(x, j0, step, M, cfs, f0, f1, g, r);
+    else rhs_column_comp<T, KIND, 1, NPT>(x, j0, step, M, cfs, f0, f1, g, r);
 }
-
-// ---- y lines: W adjacent columns x P chunks per block, one propagation.
-// Blocks b, b+8, ..., b+8(2G-1) (the same XCD under round-robin dispatch)
-// hold G adjacent column strips for both polarisations, so that narrow
-// strips share the 128-byte lines of the fields and coefficient planes in
-// L2, as do the two polarisations.  The right-hand sides of the W x M tile are computed by all
-// threads (consecutive threads on consecutive columns) and staged in LDS in
-// the chunk-interleaved order ((i*P + p)*W + column); the chunk threads read
-// the tridiagonal rows from global memory one W-wide row segment at a time,
-// and the solution goes back through LDS to row-ordered stores.
-//   neumann 0: dst = M_y (I + mu A_x) src, acc = dst
-//   neumann 1: dst = M_y (mu O_y src),     acc += dst
-constexpr int kYGroup = 8;  // adjacent column strips per XCD group
 
 template <class T, int LMAX, int NEU>
 __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
                                                  C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf, Geo g,
-                                                 const PropParams *__restrict__ pp, LinePlan lp, int W) {
+                                                 const PropParams *__restrict__ pp, LinePlan lp, int W,
+                                                 int npairs, int order) {
     using C = C2<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int P = lp.P, M = g.My, TL = LMAX * P * W;
     C *sd = reinterpret_cast<C *>(smem_raw);
     const RedLds<T> red(sd + TL, W * P);
     int *ymap = reinterpret_cast<int *>(sd + TL + RedLds<T>::kArrays * W * P);  // row j -> i*P + p
-    const int bx = blockIdx.x, jg = (bx >> 3) % (2 * kYGroup), pol = jg / kYGroup;
-    const int lb = ((bx / (16 * kYGroup)) * 8 + (bx & 7)) * kYGroup + jg % kYGroup;
-    if (lb * W >= g.Mx) return;  // padding blocks of the XCD grouping
-    const int x0 = lb * W, comp = blockIdx.y, prop = blockIdx.z * 2 + pol;
+    // work items (polarisation fastest, then component and strip in the order
+    // `order & 2` selects, then pair); each XCD takes a contiguous range, so
+    // both polarisations of a strip and its components or neighbouring strips
+    // share the 128-byte lines of fields and coefficients in L2
+    const int nstrip = (g.Mx + W - 1) / W;
+    const int item = xcd_tile(blockIdx.x, gridDim.x);
+    if (item >= nstrip * 4 * npairs) return;
+    const int pol = item & 1;
+    int comp, lb, pair;
+    if (order & 2) {
+        comp = (item >> 1) & 1;
+        lb = (item >> 2) % nstrip;
+        pair = (item >> 2) / nstrip;
+    } else {
+        lb = (item >> 1) % nstrip;
+        comp = ((item >> 1) / nstrip) & 1;
+        pair = (item >> 1) / (2 * nstrip);
+    }
+    const int x0 = lb * W, prop = pair * 2 + pol;
     const int slot = pp[prop].slot;
     const size_t NP = (size_t)g.NP;
     const C *cfs = cf + (size_t)slot * cf_slot_size(g.NP);
@@ -1195,14 +1213,13 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
 
     // right-hand sides of the tile
     if (col_ok) {
-        constexpr int KIND = NEU ? RHS_OY : RHS_XPLUS, NPT = rhs_points<T>();
+        constexpr int KIND = NEU ? RHS_OY : RHS_XPLUS, NPT = rhs_points<T, LMAX>();
         for (int j = j0; j < M; j += NPT * P) {
-            const int j1 = j + P;
-            const bool v1 = NPT == 2 && j1 < M;
-            C da{0, 0}, db{0, 0};
-            rhs_single<T, KIND>(comp, x0 + cc, j, x0 + cc, j1, v1, cfs, f0, f1, g, da, db);
-            sd[ymap[j] * W + cc] = da;
-            if (v1) sd[ymap[j1] * W + cc] = db;
+            C d[NPT];
+            rhs_column<T, KIND, NPT>(comp, x0 + cc, j, P, M, cfs, f0, f1, g, d);
+#pragma unroll
+            for (int k = 0; k < NPT; k++)
+                if (j + k * P < M) sd[ymap[j + k * P] * W + cc] = d[k];
         }
     }
     __syncthreads();
@@ -1333,21 +1350,22 @@ size_t ysolve_lds_bytes(const LinePlan &lp, int W) {
 
 template <class T, int NEU>
 static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
-                            const C2<T> *cf, const C2<T> *xt, const Geo &g, const PropParams *pp, hipStream_t st) {
+                            const C2<T> *cf, const C2<T> *xt, const Geo &g, const PropParams *pp, int npairs,
+                            int order, hipStream_t st) {
     const dim3 block(2 * lp.P * R);
     const size_t lds = xsolve_lds_bytes<T>(lp, R);
     switch (lp.lmax) {
         case 4:
             allow_lds(k_xsolve<T, 4, NEU>, lds);
-            hipLaunchKernelGGL((k_xsolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R);
+            hipLaunchKernelGGL((k_xsolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order);
             break;
         case 8:
             allow_lds(k_xsolve<T, 8, NEU>, lds);
-            hipLaunchKernelGGL((k_xsolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R);
+            hipLaunchKernelGGL((k_xsolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order);
             break;
         case 16:
             allow_lds(k_xsolve<T, 16, NEU>, lds);
-            hipLaunchKernelGGL((k_xsolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R);
+            hipLaunchKernelGGL((k_xsolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order);
             break;
         default: throw std::runtime_error("x line solver: unsupported chunk length");
     }
@@ -1356,21 +1374,22 @@ static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *s
 
 template <class T, int NEU>
 static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
-                            const C2<T> *cf, const Geo &g, const PropParams *pp, hipStream_t st) {
+                            const C2<T> *cf, const Geo &g, const PropParams *pp, int npairs, int order,
+                            hipStream_t st) {
     const dim3 block(lp.P * W);
     const size_t lds = ysolve_lds_bytes<T>(lp, W);
     switch (lp.lmax) {
         case 4:
             allow_lds(k_ysolve<T, 4, NEU>, lds);
-            hipLaunchKernelGGL((k_ysolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W);
+            hipLaunchKernelGGL((k_ysolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order);
             break;
         case 8:
             allow_lds(k_ysolve<T, 8, NEU>, lds);
-            hipLaunchKernelGGL((k_ysolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W);
+            hipLaunchKernelGGL((k_ysolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order);
             break;
         case 16:
             allow_lds(k_ysolve<T, 16, NEU>, lds);
-            hipLaunchKernelGGL((k_ysolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W);
+            hipLaunchKernelGGL((k_ysolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order);
             break;
         default: throw std::runtime_error("y line solver: unsupported chunk length");
     }
@@ -1379,25 +1398,29 @@ static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *s
 
 template <class T>
 void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
-                   const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, hipStream_t st) {
+                   const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, int order,
+                   hipStream_t st) {
     if (2 * lp.P * R > 512 || lp.M != g.Mx || R < 1 || lp.P * lp.lmax < lp.M)
         throw std::runtime_error("x line solver: bad launch shape");
     if (nprops % 2) throw std::runtime_error("x line solver: propagations come in polarisation pairs");
-    dim3 grid((unsigned)((g.My + R - 1) / R), 2, (unsigned)(nprops / 2));
-    if (neumann) xsolve_dispatch<T, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, st);
-    else xsolve_dispatch<T, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, st);
+    const int npairs = nprops / 2;
+    const int64_t items = (int64_t)((g.My + R - 1) / R) * 2 * npairs;
+    dim3 grid((unsigned)((items + 7) / 8 * 8));
+    if (neumann) xsolve_dispatch<T, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, st);
+    else xsolve_dispatch<T, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, st);
 }
 
 template <class T>
 void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
-                   const PropParams *pp, int nprops, const LinePlan &lp, int W, hipStream_t st) {
+                   const PropParams *pp, int nprops, const LinePlan &lp, int W, int order, hipStream_t st) {
     if (lp.P * W > 512 || lp.M != g.My || W < 1 || (W & (W - 1)) || lp.P * lp.lmax < lp.M)
         throw std::runtime_error("y line solver: bad launch shape");
     if (nprops % 2) throw std::runtime_error("y line solver: propagations come in polarisation pairs");
-    const int nlb = (g.Mx + W - 1) / W, ngroups = (nlb + kYGroup - 1) / kYGroup;
-    dim3 grid((unsigned)((ngroups + 7) / 8 * 16 * kYGroup), 2, (unsigned)(nprops / 2));
-    if (neumann) ysolve_dispatch<T, 1>(lp, W, grid, src, dst, acc, cf, g, pp, st);
-    else ysolve_dispatch<T, 0>(lp, W, grid, src, dst, acc, cf, g, pp, st);
+    const int npairs = nprops / 2;
+    const int64_t items = (int64_t)((g.Mx + W - 1) / W) * 4 * npairs;
+    dim3 grid((unsigned)((items + 7) / 8 * 8));
+    if (neumann) ysolve_dispatch<T, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+    else ysolve_dispatch<T, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1653,9 +1676,9 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
     template void launch_explicit<T>(int, const C2<T> *, C2<T> *, const C2<T> *, const Geo &, const PropParams *,    \
                                      int, hipStream_t);                                                               \
     template void launch_xsolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const C2<T> *, const Geo &, \
-                                   const PropParams *, int, const LinePlan &, int, hipStream_t);                      \
+                                   const PropParams *, int, const LinePlan &, int, int, hipStream_t);                 \
     template void launch_ysolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const Geo &,                  \
-                                   const PropParams *, int, const LinePlan &, int, hipStream_t);                      \
+                                   const PropParams *, int, const LinePlan &, int, int, hipStream_t);                 \
     template size_t xsolve_lds_bytes<T>(const LinePlan &, int);                                                       \
     template size_t ysolve_lds_bytes<T>(const LinePlan &, int);                                                       \
     template void launch_output<T>(const C2<T> *, const double *, const uint8_t *, const Geo &, const SlotParams *,  \

@@ -212,3 +212,14 @@ def test_line_solver_shapes_fp64(tuning):
     want = oracle.propagate(s, lc)
     err = _rel(_gpu(s, lc, tuning=tuning), want)
     assert err < TOL64, err
+
+
+def test_xcd_work_orders_are_bitwise_equal():
+    """The four XCD work orders only move blocks between XCDs: identical output."""
+    s, lc = _random_case((70, 66, 5), seed=19)
+    base = {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8}
+    outs = [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o)) for o in range(4)]
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    want = oracle.propagate(s, lc)
+    assert _rel(outs[0], want) < TOL32
