@@ -62,7 +62,8 @@ typedef struct {
     int32_t y_chunks;    /*   y lines: chunks per column, columns per block      */
     int32_t y_cols;
     int32_t item_order;  /*   0: by batch size; 1 + bits: bit 0 x components, bit 1
-                              y components adjacent in the XCD work ranges      */
+                              y components, bit 2 polarisation pairs adjacent
+                              in the XCD work ranges                            */
     int32_t reserved[2];
 } nmk_bpm_options;
 

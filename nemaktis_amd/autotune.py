@@ -157,7 +157,7 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
         table.append((cand, ms))
         if ms < best_ms:
             best, best_ms = cand, ms
-    for order in range(4):  # which items share an XCD's L2 (components or neighbours)
+    for order in range(8):  # which items share an XCD's L2 (components, neighbours, wavevectors)
         cand = dict(best, order=order)
         ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
         table.append((cand, ms))

@@ -36,7 +36,7 @@ Options to_options(const nmk_bpm_options *o) {
             r.tuning.x_rows = o->x_rows;
             r.tuning.y_chunks = o->y_chunks;
             r.tuning.y_cols = o->y_cols;
-            if (o->item_order > 0) r.tuning.order = (o->item_order - 1) & 3;
+            if (o->item_order > 0) r.tuning.order = (o->item_order - 1) & 7;
         }
     }
     return r;

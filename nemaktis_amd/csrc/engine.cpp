@@ -481,7 +481,7 @@ private:
         C2<T> *cur = fb_[1].ptr, *other = fb_[0].ptr, *A = fb_[2].ptr, *B = fb_[3].ptr;
         // XCD work order: a lone polarisation pair shares L2 best across the two
         // components, larger batches across neighbouring rows/strips (measured)
-        const int ord = tune_.order >= 0 ? tune_.order : (nprops <= 2 ? 3 : 0);
+        const int ord = tune_.order >= 0 ? tune_.order : (nprops <= 2 ? 3 : 4);
         HIPCHK(hipStreamWaitEvent(st_, ready[0], 0));
         timed(KF_BOUNDARY, [&] { launch_boundary<T>(other, cur, nullptr, CF(0), g_, pp_.ptr, nprops, 1, st_); });
         for (int k = 0; k < nsl; k++) {

@@ -40,7 +40,7 @@ void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, cons
                      int nprops, hipStream_t st);
 // order: bit 0 puts the two components of an x row block next to each other
 // in the XCD work ranges (else neighbouring row blocks), bit 1 the same for
-// y strips.
+// y strips, bit 2 puts the polarisation pairs innermost.
 // cyclic block-tridiagonal line solves of the implicit half-steps, one
 // thread per chunk carrying both polarisations (nprops must be even, the
 // propagations ordered pol + 2 * (wavevector + n_wavevectors * slot)):

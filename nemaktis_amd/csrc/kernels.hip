@@ -998,22 +998,29 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     C *sd = reinterpret_cast<C *>(smem_raw);  // [pol][row][SL]
     const RedLds<T> red(sd + 2 * R * SL, 2 * RP);
     int *smap = reinterpret_cast<int *>(sd + 2 * R * SL + RedLds<T>::kArrays * 2 * RP);  // x -> i*P + p
-    // work items (component and row block, in the order `order & 1` selects,
-    // then polarisation pair); each XCD takes a contiguous range of them, so
-    // both components of a row block or the neighbouring row blocks meet in L2
+    // work items (component and row block in the order `order & 1` selects,
+    // polarisation pair outermost or, with `order & 4`, innermost); each XCD
+    // takes a contiguous range of them, so both components of a row block, the
+    // neighbouring row blocks or the wavevectors of one wavelength meet in L2
     const int nrb = (g.My + R - 1) / R;
     const int item = xcd_tile(blockIdx.x, gridDim.x);
     if (item >= nrb * 2 * nitems_pairs) return;
-    int comp, row0, prop0;
-    if (order & 1) {  // component fastest: both components of a row block together
-        comp = item & 1;
-        row0 = ((item >> 1) % nrb) * R;
-        prop0 = ((item >> 1) / nrb) * 2;
-    } else {  // row block fastest: neighbouring row blocks together
-        row0 = (item % nrb) * R;
-        comp = (item / nrb) & 1;
-        prop0 = (item / (2 * nrb)) * 2;
+    int comp, row0, pair, rest;
+    if (order & 4) {  // pair fastest: the wavevectors of one wavelength share the coefficients
+        pair = item % nitems_pairs;
+        rest = item / nitems_pairs;
+    } else {
+        pair = item / (2 * nrb);
+        rest = item % (2 * nrb);
     }
+    if (order & 1) {  // component before row block: both components of a row block together
+        comp = rest & 1;
+        row0 = (rest >> 1) * R;
+    } else {  // neighbouring row blocks together
+        row0 = (rest % nrb) * R;
+        comp = rest / nrb;
+    }
+    const int prop0 = pair * 2;
     if ((int)threadIdx.x < P) {
         const int pc = threadIdx.x, s0c = chunk_start(lp, pc), lc = chunk_len(lp, pc);
         for (int i = 0; i < lc; i++) smap[s0c + i] = i * P + pc;
@@ -1178,22 +1185,28 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     const RedLds<T> red(sd + TL, W * P);
     int *ymap = reinterpret_cast<int *>(sd + TL + RedLds<T>::kArrays * W * P);  // row j -> i*P + p
     // work items (polarisation fastest, then component and strip in the order
-    // `order & 2` selects, then pair); each XCD takes a contiguous range, so
-    // both polarisations of a strip and its components or neighbouring strips
-    // share the 128-byte lines of fields and coefficients in L2
+    // `order & 2` selects, pair outermost or, with `order & 4`, right after the
+    // polarisation); each XCD takes a contiguous range, so both polarisations
+    // of a strip and its components, neighbouring strips or the wavevectors of
+    // one wavelength share the 128-byte lines of fields and coefficients in L2
     const int nstrip = (g.Mx + W - 1) / W;
     const int item = xcd_tile(blockIdx.x, gridDim.x);
     if (item >= nstrip * 4 * npairs) return;
     const int pol = item & 1;
-    int comp, lb, pair;
-    if (order & 2) {
-        comp = (item >> 1) & 1;
-        lb = (item >> 2) % nstrip;
-        pair = (item >> 2) / nstrip;
+    int comp, lb, pair, rest;
+    if (order & 4) {
+        pair = (item >> 1) % npairs;
+        rest = (item >> 1) / npairs;
     } else {
-        lb = (item >> 1) % nstrip;
-        comp = ((item >> 1) / nstrip) & 1;
         pair = (item >> 1) / (2 * nstrip);
+        rest = (item >> 1) % (2 * nstrip);
+    }
+    if (order & 2) {
+        comp = rest & 1;
+        lb = rest >> 1;
+    } else {
+        lb = rest % nstrip;
+        comp = rest / nstrip;
     }
     const int x0 = lb * W, prop = pair * 2 + pol;
     const int slot = pp[prop].slot;

@@ -215,10 +215,11 @@ def test_line_solver_shapes_fp64(tuning):
 
 
 def test_xcd_work_orders_are_bitwise_equal():
-    """The four XCD work orders only move blocks between XCDs: identical output."""
+    """The XCD work orders only move blocks between XCDs: identical output
+    (two wavelengths x two wavevectors, so that every index varies)."""
     s, lc = _random_case((70, 66, 5), seed=19)
     base = {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8}
-    outs = [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o)) for o in range(4)]
+    outs = [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o)) for o in range(8)]
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
     want = oracle.propagate(s, lc)
