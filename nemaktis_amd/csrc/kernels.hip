@@ -899,6 +899,36 @@ __device__ __forceinline__ void chunk_rebuild(const Chunk<T, LMAX> &ck, int n, C
     }
 }
 
+template <class T>
+__device__ __forceinline__ C2<T> shfl_c(C2<T> v, int lane) {
+    return C2<T>{__shfl(v.x, lane), __shfl(v.y, lane)};
+}
+
+// The cyclic system of one line held by P <= 64 consecutive lanes of one wave
+// (lane lb + p holds interface row p): parallel cyclic reduction through lane
+// permutes, without LDS or barriers; returns xi_p.  Every lane of the wave
+// calls it (lanes of inactive lines compute unused values).
+template <class T>
+__device__ __forceinline__ C2<T> wave_pcr(C2<T> A, C2<T> B, C2<T> Cc, C2<T> D, int P, int p, int lb) {
+    using C = C2<T>;
+    for (int st = 1; 2 * st < P; st <<= 1) {
+        const int lm = lb + ((p - st) & (P - 1)), lq = lb + ((p + st) & (P - 1));
+        const C Am = shfl_c(A, lm), Bm = shfl_c(B, lm), Cm = shfl_c(Cc, lm), Dm = shfl_c(D, lm);
+        const C Aq = shfl_c(A, lq), Bq = shfl_c(B, lq), Cq = shfl_c(Cc, lq), Dq = shfl_c(D, lq);
+        const C k1 = cdiv(A, Bm), k2 = cdiv(Cc, Bq);
+        const C nA = -(k1 * Am), nC = -(k2 * Cq);
+        B = B - k1 * Cm - k2 * Aq;
+        D = D - k1 * Dm - k2 * Dq;
+        A = nA;
+        Cc = nC;
+    }
+    if (P == 1) return cdiv(D, A + B + Cc);
+    const int lq = lb + ((p + P / 2) & (P - 1));
+    const C Aq = shfl_c(A, lq), Bq = shfl_c(B, lq), Cq = shfl_c(Cc, lq), Dq = shfl_c(D, lq);
+    const C apc = A + Cc, aqc = Aq + Cq;
+    return cdiv(D * Bq - apc * Dq, B * Bq - apc * aqc);
+}
+
 // LDS of the reduced systems: first interior values of each chunk (uf, gf,
 // hf) and the interface equations A xi_{p-1} + B xi_p + C xi_{p+1} = D;
 // the solutions overwrite uf
@@ -1071,6 +1101,7 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     const C *xa = xt + xrow, *xb = xa + xstride, *xc = xb + xstride;
     const int me = line * P + p, nx = line * P + ((p + 1) & (P - 1));
     Chunk<T, LMAX> ck;
+    ck.uf = ck.gf = ck.hf = ck.ul = ck.gl = ck.hl = C{0, 0};
     if (active) {
         chunk_eliminate<T, LMAX, sizeof(T) == 4>(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
             const int q = i * P + p;
@@ -1079,27 +1110,48 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
             c = xc[q];
             d = sdl[q];
         });
-        red.uf[me] = ck.uf;
-        red.gf[me] = ck.gf;
-        red.hf[me] = ck.hf;
     }
-    __syncthreads();
-    C A{0, 0}, B{0, 0}, Cv{0, 0}, D{0, 0};
-    if (active) {
-        const int q = n * P + p;
-        interface_row(ck, xa[q], xb[q], xc[q], sdl[q], red, nx, A, B, Cv, D);
+    C xi, xprev;
+    if (P <= 64) {
+        // the P chunks of a line are consecutive lanes of one wave
+        const int lb = ((int)threadIdx.x & 63) - p, ln = lb + ((p + 1) & (P - 1));
+        const C ufn = shfl_c(ck.uf, ln), gfn = shfl_c(ck.gf, ln), hfn = shfl_c(ck.hf, ln);
+        C A{0, 0}, B{0, 0}, Cv{0, 0}, D{0, 0};
+        if (active) {
+            const int q = n * P + p;
+            const C ae = xa[q], be = xb[q], ce = xc[q];
+            A = ae * ck.gl;
+            B = be + ae * ck.hl + ce * gfn;
+            Cv = ce * hfn;
+            D = sdl[q] - ae * ck.ul - ce * ufn;
+        }
+        xi = wave_pcr(A, B, Cv, D, P, p, lb);
+        xprev = shfl_c(xi, lb + ((p - 1) & (P - 1)));
+    } else {
+        if (active) {
+            red.uf[me] = ck.uf;
+            red.gf[me] = ck.gf;
+            red.hf[me] = ck.hf;
+        }
+        __syncthreads();
+        C A{0, 0}, B{0, 0}, Cv{0, 0}, D{0, 0};
+        if (active) {
+            const int q = n * P + p;
+            interface_row(ck, xa[q], xb[q], xc[q], sdl[q], red, nx, A, B, Cv, D);
+        }
+        __syncthreads();
+        if (active) {
+            red.A[me] = A;
+            red.B[me] = B;
+            red.Cc[me] = Cv;
+            red.D[me] = D;
+        }
+        __syncthreads();
+        reduced_pcr(red, P, p, line * P, 1, active);
+        xi = red.uf[me];
+        xprev = red.uf[line * P + ((p - 1) & (P - 1))];
     }
-    __syncthreads();
     if (active) {
-        red.A[me] = A;
-        red.B[me] = B;
-        red.Cc[me] = Cv;
-        red.D[me] = D;
-    }
-    __syncthreads();
-    reduced_pcr(red, P, p, line * P, 1, active);
-    if (active) {
-        const C xi = red.uf[me], xprev = red.uf[line * P + ((p - 1) & (P - 1))];
         chunk_rebuild(ck, n, xprev, xi, [&](int i, C v) { sdl[i * P + p] = v; });
         sdl[n * P + p] = xi;
     }
