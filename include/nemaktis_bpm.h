@@ -64,7 +64,8 @@ typedef struct {
     int32_t item_order;  /*   0: by batch size; 1 + bits: bit 0 x components, bit 1
                               y components, bit 2 polarisation pairs adjacent
                               in the XCD work ranges                            */
-    int32_t reserved[2];
+    int32_t lean;        /*   1: register-lean solver variants (large batches)   */
+    int32_t reserved;
 } nmk_bpm_options;
 
 typedef struct {

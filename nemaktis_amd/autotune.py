@@ -100,7 +100,7 @@ def _save_cache(c):
 
 
 def cache_key(precision, Nx, Ny, nprops):
-    return "gfx950|solver-v4|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
+    return "gfx950|solver-v5|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
 
 
 def _time(js, lc, n_E, precision, wl_batch, tuning, reps):
@@ -153,6 +153,16 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
             best, best_ms = cand, ms
     for (yc, yw) in ys:
         cand = dict(best, y_chunks=yc, y_cols=yw)
+        ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+        table.append((cand, ms))
+        if ms < best_ms:
+            best, best_ms = cand, ms
+    # register-lean variants (more waves per SIMD) with the shapes found so far
+    # and with shorter chunks
+    for (xc, yc) in [(best["x_chunks"], best["y_chunks"]), (2 * best["x_chunks"], 2 * best["y_chunks"])]:
+        cand = dict(best, x_chunks=xc, y_chunks=yc, lean=1)
+        if (xc, cand["x_rows"]) not in xs or (yc, cand["y_cols"]) not in ys:
+            continue
         ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
         table.append((cand, ms))
         if ms < best_ms:

@@ -488,12 +488,12 @@ private:
             // x half-steps: other = (I - mu A_x)^-1 (I + mu A_y) cur by its Neumann series
             if (g_.px) {
                 timed(KF_SOLVE_X, [&] {
-                    launch_xsolve<T>(0, cur, A, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, st_);
+                    launch_xsolve<T>(0, cur, A, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, tune_.lean, st_);
                 });
                 C2<T> *a = A, *b = B;
                 for (int it = 0; it < nw; it++) {
                     timed(KF_SOLVE_X, [&] {
-                        launch_xsolve<T>(1, a, b, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, st_);
+                        launch_xsolve<T>(1, a, b, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, tune_.lean, st_);
                     });
                     std::swap(a, b);
                 }
@@ -503,12 +503,12 @@ private:
             // y half-steps: cur = (I - mu A_y)^-1 (I + mu A_x) other by its Neumann series
             if (g_.py) {
                 timed(KF_SOLVE_Y, [&] {
-                    launch_ysolve<T>(0, other, A, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, st_);
+                    launch_ysolve<T>(0, other, A, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, tune_.lean, st_);
                 });
                 C2<T> *a = A, *b = B;
                 for (int it = 0; it < nw; it++) {
                     timed(KF_SOLVE_Y, [&] {
-                        launch_ysolve<T>(1, a, b, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, st_);
+                        launch_ysolve<T>(1, a, b, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, tune_.lean, st_);
                     });
                     std::swap(a, b);
                 }

@@ -18,6 +18,7 @@ struct Tuning {
     int x_chunks = 0, x_rows = 0;  // x lines: chunks per row, rows per block
     int y_chunks = 0, y_cols = 0;  // y lines: chunks per column, columns per block
     int order = -1;                // XCD work order of the solvers (bits, see launch_xsolve); -1: by batch size
+    int lean = 0;                  // 1: register-lean solver variants (see launch_xsolve)
 };
 Tuning default_tuning(int precision, int Mx, int My);
 

@@ -41,6 +41,8 @@ void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, cons
 // order: bit 0 puts the two components of an x row block next to each other
 // in the XCD work ranges (else neighbouring row blocks), bit 1 the same for
 // y strips, bit 2 puts the polarisation pairs innermost.
+// lean: one gather point per round and no row prefetch (fewer registers,
+// more waves per SIMD for large batches).
 // cyclic block-tridiagonal line solves of the implicit half-steps, one
 // thread per chunk carrying both polarisations (nprops must be even, the
 // propagations ordered pol + 2 * (wavevector + n_wavevectors * slot)):
@@ -58,10 +60,11 @@ size_t ysolve_lds_bytes(const LinePlan &lp, int W);
 template <class T>
 void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
                    const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, int order,
-                   hipStream_t st);
+                   int lean, hipStream_t st);
 template <class T>
 void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
-                   const PropParams *pp, int nprops, const LinePlan &lp, int W, int order, hipStream_t st);
+                   const PropParams *pp, int nprops, const LinePlan &lp, int W, int order, int lean,
+                   hipStream_t st);
 // interior march state -> full-grid complex128 fields with the global phase
 // and the output Fresnel interface
 template <class T>

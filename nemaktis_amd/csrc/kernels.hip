@@ -1016,7 +1016,7 @@ __device__ __forceinline__ void advance_rc(int &r, int &x, int step, int M) {
 // from the coefficient kernel.
 //   neumann 0: dst = M_x (I + mu A_y) src, acc = dst
 //   neumann 1: dst = M_x (mu O_x src),     acc += dst
-template <class T, int LMAX, int NEU>
+template <class T, int LMAX, int NEU, int LEAN>
 __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
                                                  C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf,
                                                  const C2<T> *__restrict__ xt, Geo g,
@@ -1065,7 +1065,7 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
 
     // right-hand sides of the block's rows, both polarisations
     {
-        constexpr int KIND = NEU ? RHS_OX : RHS_YPLUS, NPR = rhs_points<T, LMAX>();
+        constexpr int KIND = NEU ? RHS_OX : RHS_YPLUS, NPR = LEAN ? 1 : rhs_points<T, LMAX>();
         int r0 = (int)threadIdx.x / M, x0 = (int)threadIdx.x - r0 * M;
         for (int e0 = threadIdx.x; e0 < ntot; e0 += NPR * bs) {
             unsigned xs[NPR], ys[NPR];
@@ -1103,7 +1103,7 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     Chunk<T, LMAX> ck;
     ck.uf = ck.gf = ck.hf = ck.ul = ck.gl = ck.hl = C{0, 0};
     if (active) {
-        chunk_eliminate<T, LMAX, sizeof(T) == 4>(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
+        chunk_eliminate<T, LMAX, sizeof(T) == 4 && !LEAN>(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
             const int q = i * P + p;
             a = xa[q];
             b = xb[q];
@@ -1225,7 +1225,7 @@ __device__ __forceinline__ void rhs_column(int comp, unsigned x, int j0, int ste
     else rhs_column_comp<T, KIND, 1, NPT>(x, j0, step, M, cfs, f0, f1, g, r);
 }
 
-template <class T, int LMAX, int NEU>
+template <class T, int LMAX, int NEU, int LEAN>
 __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
                                                  C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf, Geo g,
                                                  const PropParams *__restrict__ pp, LinePlan lp, int W,
@@ -1278,7 +1278,7 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
 
     // right-hand sides of the tile
     if (col_ok) {
-        constexpr int KIND = NEU ? RHS_OY : RHS_XPLUS, NPT = rhs_points<T, LMAX>();
+        constexpr int KIND = NEU ? RHS_OY : RHS_XPLUS, NPT = LEAN ? 1 : rhs_points<T, LMAX>();
         for (int j = j0; j < M; j += NPT * P) {
             C d[NPT];
             rhs_column<T, KIND, NPT>(comp, x0 + cc, j, P, M, cfs, f0, f1, g, d);
@@ -1305,7 +1305,7 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     const int me = p * W + cofs, nx = ((p + 1) & (P - 1)) * W + cofs;
     Chunk<T, LMAX> ck;
     if (active) {
-        chunk_eliminate<T, LMAX, sizeof(T) == 4>(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
+        chunk_eliminate<T, LMAX, sizeof(T) == 4 && !LEAN>(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
             tri(s + i, a, b, c);
             d = sd[(i * P + p) * W + cofs];
         });
@@ -1413,7 +1413,7 @@ size_t ysolve_lds_bytes(const LinePlan &lp, int W) {
            (size_t)lp.M * sizeof(int);
 }
 
-template <class T, int NEU>
+template <class T, int NEU, int LEAN>
 static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
                             const C2<T> *cf, const C2<T> *xt, const Geo &g, const PropParams *pp, int npairs,
                             int order, hipStream_t st) {
@@ -1421,23 +1421,23 @@ static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *s
     const size_t lds = xsolve_lds_bytes<T>(lp, R);
     switch (lp.lmax) {
         case 4:
-            allow_lds(k_xsolve<T, 4, NEU>, lds);
-            hipLaunchKernelGGL((k_xsolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order);
+            allow_lds(k_xsolve<T, 4, NEU, LEAN>, lds);
+            hipLaunchKernelGGL((k_xsolve<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order);
             break;
         case 8:
-            allow_lds(k_xsolve<T, 8, NEU>, lds);
-            hipLaunchKernelGGL((k_xsolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order);
+            allow_lds(k_xsolve<T, 8, NEU, LEAN>, lds);
+            hipLaunchKernelGGL((k_xsolve<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order);
             break;
         case 16:
-            allow_lds(k_xsolve<T, 16, NEU>, lds);
-            hipLaunchKernelGGL((k_xsolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order);
+            allow_lds(k_xsolve<T, 16, NEU, LEAN>, lds);
+            hipLaunchKernelGGL((k_xsolve<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order);
             break;
         default: throw std::runtime_error("x line solver: unsupported chunk length");
     }
     NMK_LAUNCH_CHECK();
 }
 
-template <class T, int NEU>
+template <class T, int NEU, int LEAN>
 static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
                             const C2<T> *cf, const Geo &g, const PropParams *pp, int npairs, int order,
                             hipStream_t st) {
@@ -1445,16 +1445,16 @@ static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *s
     const size_t lds = ysolve_lds_bytes<T>(lp, W);
     switch (lp.lmax) {
         case 4:
-            allow_lds(k_ysolve<T, 4, NEU>, lds);
-            hipLaunchKernelGGL((k_ysolve<T, 4, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order);
+            allow_lds(k_ysolve<T, 4, NEU, LEAN>, lds);
+            hipLaunchKernelGGL((k_ysolve<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order);
             break;
         case 8:
-            allow_lds(k_ysolve<T, 8, NEU>, lds);
-            hipLaunchKernelGGL((k_ysolve<T, 8, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order);
+            allow_lds(k_ysolve<T, 8, NEU, LEAN>, lds);
+            hipLaunchKernelGGL((k_ysolve<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order);
             break;
         case 16:
-            allow_lds(k_ysolve<T, 16, NEU>, lds);
-            hipLaunchKernelGGL((k_ysolve<T, 16, NEU>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order);
+            allow_lds(k_ysolve<T, 16, NEU, LEAN>, lds);
+            hipLaunchKernelGGL((k_ysolve<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order);
             break;
         default: throw std::runtime_error("y line solver: unsupported chunk length");
     }
@@ -1464,28 +1464,39 @@ static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *s
 template <class T>
 void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
                    const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, int order,
-                   hipStream_t st) {
+                   int lean, hipStream_t st) {
     if (2 * lp.P * R > 512 || lp.M != g.Mx || R < 1 || lp.P * lp.lmax < lp.M)
         throw std::runtime_error("x line solver: bad launch shape");
     if (nprops % 2) throw std::runtime_error("x line solver: propagations come in polarisation pairs");
     const int npairs = nprops / 2;
     const int64_t items = (int64_t)((g.My + R - 1) / R) * 2 * npairs;
     dim3 grid((unsigned)((items + 7) / 8 * 8));
-    if (neumann) xsolve_dispatch<T, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, st);
-    else xsolve_dispatch<T, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, st);
+    if (lean) {
+        if (neumann) xsolve_dispatch<T, 1, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, st);
+        else xsolve_dispatch<T, 0, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, st);
+    } else {
+        if (neumann) xsolve_dispatch<T, 1, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, st);
+        else xsolve_dispatch<T, 0, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, st);
+    }
 }
 
 template <class T>
 void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
-                   const PropParams *pp, int nprops, const LinePlan &lp, int W, int order, hipStream_t st) {
+                   const PropParams *pp, int nprops, const LinePlan &lp, int W, int order, int lean,
+                   hipStream_t st) {
     if (lp.P * W > 512 || lp.M != g.My || W < 1 || (W & (W - 1)) || lp.P * lp.lmax < lp.M)
         throw std::runtime_error("y line solver: bad launch shape");
     if (nprops % 2) throw std::runtime_error("y line solver: propagations come in polarisation pairs");
     const int npairs = nprops / 2;
     const int64_t items = (int64_t)((g.Mx + W - 1) / W) * 4 * npairs;
     dim3 grid((unsigned)((items + 7) / 8 * 8));
-    if (neumann) ysolve_dispatch<T, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
-    else ysolve_dispatch<T, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+    if (lean) {
+        if (neumann) ysolve_dispatch<T, 1, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+        else ysolve_dispatch<T, 0, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+    } else {
+        if (neumann) ysolve_dispatch<T, 1, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+        else ysolve_dispatch<T, 0, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1741,9 +1752,9 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
     template void launch_explicit<T>(int, const C2<T> *, C2<T> *, const C2<T> *, const Geo &, const PropParams *,    \
                                      int, hipStream_t);                                                               \
     template void launch_xsolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const C2<T> *, const Geo &, \
-                                   const PropParams *, int, const LinePlan &, int, int, hipStream_t);                 \
+                                   const PropParams *, int, const LinePlan &, int, int, int, hipStream_t);            \
     template void launch_ysolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const Geo &,                  \
-                                   const PropParams *, int, const LinePlan &, int, int, hipStream_t);                 \
+                                   const PropParams *, int, const LinePlan &, int, int, int, hipStream_t);            \
     template size_t xsolve_lds_bytes<T>(const LinePlan &, int);                                                       \
     template size_t ysolve_lds_bytes<T>(const LinePlan &, int);                                                       \
     template void launch_output<T>(const C2<T> *, const double *, const uint8_t *, const Geo &, const SlotParams *,  \
