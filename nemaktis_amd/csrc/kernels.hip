@@ -439,40 +439,57 @@ __device__ __forceinline__ int wrapm(int i, int M) {
 // the first and last boundaries carry a single shift.  One block computes a
 // TW x TH output tile from an LDS copy of its (TW+6) x (TH+6) input window.
 // ---------------------------------------------------------------------------
-constexpr int kBTW = 64, kBTH = 4;
+constexpr int kBTW = 64, kBTH = 8;
 
+// One block: a TW x TH output tile of both polarisations of one (wavelength,
+// wavevector) pair (they share the shift taps and the phase operators).  The
+// (TW+6) x (TH+6) input window is loaded once with the previous phase operator
+// applied, the x pass is done in place in LDS, then the y pass and the next
+// phase operator; wrapped window indices are computed once per block.
 template <class T>
 __global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
                                                   const C2<T> *__restrict__ cf_prev, const C2<T> *__restrict__ cf_next,
                                                   Geo g, const PropParams *__restrict__ pp, int nshift) {
     using C = C2<T>;
-    __shared__ C tile[2][kBTH + 6][kBTW + 7];
-    __shared__ C tmp[2][kBTH + 6][kBTW + 1];
-    const int prop = blockIdx.z;
-    const PropParams p = pp[prop];
-    const int64_t NP = g.NP;
+    constexpr int WW = kBTW + 6, WH = kBTH + 6;
+    __shared__ C tile[4][WH][kBTW + 7];
+    __shared__ int colw[WW], roww[WH];
+    const int prop0 = blockIdx.z * 2;
+    const PropParams p = pp[prop0];
+    const size_t NP = (size_t)g.NP;
     const int x0 = blockIdx.x * kBTW, y0 = blockIdx.y * kBTH;
     const double *wxs = nshift == 2 ? p.w2x : p.w1x, *wys = nshift == 2 ? p.w2y : p.w1y;
     const int ox = g.px ? (nshift == 2 ? p.o2x : p.o1x) : 0;
     const int oy = g.py ? (nshift == 2 ? p.o2y : p.o1y) : 0;
-    const C *f0 = in + ((int64_t)prop * 2) * NP, *f1 = f0 + NP;
-    const C *cp = cf_prev ? cf_prev + (size_t)p.slot * cf_slot_size(NP) + CF_P * kCfTile : nullptr;
-    auto wrapx = [&](int i) { return g.px ? wrapm(i, g.Mx) : min(max(i, 0), g.Mx - 1); };
-    auto wrapy = [&](int i) { return g.py ? wrapm(i, g.My) : min(max(i, 0), g.My - 1); };
+    if ((int)threadIdx.x < WW) {
+        const int i = x0 + ox + (int)threadIdx.x;
+        colw[threadIdx.x] = g.px ? wrapm(i, g.Mx) : min(max(i, 0), g.Mx - 1);
+    } else if ((int)threadIdx.x < WW + WH) {
+        const int i = y0 + oy + (int)threadIdx.x - WW;
+        roww[threadIdx.x - WW] = g.py ? wrapm(i, g.My) : min(max(i, 0), g.My - 1);
+    }
+    __syncthreads();
+    const C *f = in + (size_t)prop0 * 2 * NP;  // planes: pol 0 (x, y), pol 1 (x, y)
+    const C *cp = cf_prev ? cf_prev + (size_t)p.slot * cf_slot_size(g.NP) + CF_P * kCfTile : nullptr;
     // input window with the previous slab's phase operator applied
-    for (int e = threadIdx.x; e < (kBTH + 6) * (kBTW + 6); e += blockDim.x) {
-        const int r = e / (kBTW + 6), c = e - r * (kBTW + 6);
-        const int64_t q = wrapx(x0 + ox + c) + (int64_t)g.Mx * wrapy(y0 + oy + r);
-        C a = f0[q], b = f1[q];
+    for (int e = threadIdx.x; e < WH * WW; e += blockDim.x) {
+        const int r = e / WW, c = e - r * WW;
+        const unsigned q = colw[c] + (unsigned)g.Mx * roww[r];
+        C a0 = f[q], b0 = f[NP + q], a1 = f[2 * NP + q], b1 = f[3 * NP + q];
         if (cp) {
-            const C *c3 = cp + cf_point((unsigned)q);
+            const C *c3 = cp + cf_point(q);
             const C pxx = c3[0], pyy = c3[kCfTile], pxy = c3[2 * kCfTile];
-            const C na = pxx * a + pxy * b, nb = pxy * a + pyy * b;
-            a = na;
-            b = nb;
+            const C na0 = pxx * a0 + pxy * b0, nb0 = pxy * a0 + pyy * b0;
+            const C na1 = pxx * a1 + pxy * b1, nb1 = pxy * a1 + pyy * b1;
+            a0 = na0;
+            b0 = nb0;
+            a1 = na1;
+            b1 = nb1;
         }
-        tile[0][r][c] = a;
-        tile[1][r][c] = b;
+        tile[0][r][c] = a0;
+        tile[1][r][c] = b0;
+        tile[2][r][c] = a1;
+        tile[3][r][c] = b1;
     }
     __syncthreads();
     T wx[7], wy[7];
@@ -481,49 +498,73 @@ __global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, 
         wx[t] = g.px ? (T)wxs[t] : (T)(t == 0 ? 1 : 0);
         wy[t] = g.py ? (T)wys[t] : (T)(t == 0 ? 1 : 0);
     }
-    // x pass
-    for (int e = threadIdx.x; e < (kBTH + 6) * kBTW; e += blockDim.x) {
-        const int r = e / kBTW, c = e - r * kBTW;
+    // x pass in place: every thread computes its elements into registers,
+    // then all of them overwrite the window after a barrier
+    constexpr int NXE = (WH * kBTW + 255) / 256;
+    C xr[NXE][4];
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
-            C a = wx[0] * tile[k][r][c];
+    for (int k = 0; k < NXE; k++) {
+        const int e = threadIdx.x + k * 256;
+        if (e < WH * kBTW) {
+            const int r = e / kBTW, c = e - r * kBTW;
 #pragma unroll
-            for (int t = 1; t < 7; t++) a = a + wx[t] * tile[k][r][c + t];
-            tmp[k][r][c] = a;
+            for (int pl = 0; pl < 4; pl++) {
+                C a = wx[0] * tile[pl][r][c];
+#pragma unroll
+                for (int t = 1; t < 7; t++) a = a + wx[t] * tile[pl][r][c + t];
+                xr[k][pl] = a;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NXE; k++) {
+        const int e = threadIdx.x + k * 256;
+        if (e < WH * kBTW) {
+            const int r = e / kBTW, c = e - r * kBTW;
+#pragma unroll
+            for (int pl = 0; pl < 4; pl++) tile[pl][r][c] = xr[k][pl];
         }
     }
     __syncthreads();
     // y pass and the next slab's phase operator
+    const C *cn = cf_next ? cf_next + (size_t)p.slot * cf_slot_size(g.NP) + CF_P * kCfTile : nullptr;
+    C *o = out + (size_t)prop0 * 2 * NP;
     for (int e = threadIdx.x; e < kBTH * kBTW; e += blockDim.x) {
         const int r = e / kBTW, c = e - r * kBTW;
         const int x = x0 + c, y = y0 + r;
         if (x >= g.Mx || y >= g.My) continue;
-        C v[2];
+        C v[4];
 #pragma unroll
-        for (int k = 0; k < 2; k++) {
-            C a = wy[0] * tmp[k][r][c];
+        for (int pl = 0; pl < 4; pl++) {
+            C a = wy[0] * tile[pl][r][c];
 #pragma unroll
-            for (int t = 1; t < 7; t++) a = a + wy[t] * tmp[k][r + t][c];
-            v[k] = a;
+            for (int t = 1; t < 7; t++) a = a + wy[t] * tile[pl][r + t][c];
+            v[pl] = a;
         }
-        const int64_t q = x + (int64_t)g.Mx * y;
-        if (cf_next) {
-            const C *cn = cf_next + (size_t)p.slot * cf_slot_size(NP) + CF_P * kCfTile + cf_point((unsigned)q);
-            const C pxx = cn[0], pyy = cn[kCfTile], pxy = cn[2 * kCfTile];
-            const C a = pxx * v[0] + pxy * v[1], b = pxy * v[0] + pyy * v[1];
-            v[0] = a;
-            v[1] = b;
+        const unsigned q = x + (unsigned)g.Mx * y;
+        if (cn) {
+            const C *c3 = cn + cf_point(q);
+            const C pxx = c3[0], pyy = c3[kCfTile], pxy = c3[2 * kCfTile];
+#pragma unroll
+            for (int pol = 0; pol < 2; pol++) {
+                const C a = pxx * v[2 * pol] + pxy * v[2 * pol + 1], b = pxy * v[2 * pol] + pyy * v[2 * pol + 1];
+                v[2 * pol] = a;
+                v[2 * pol + 1] = b;
+            }
         }
-        C *o0 = out + ((int64_t)prop * 2) * NP;
-        o0[q] = v[0];
-        o0[NP + q] = v[1];
+        o[q] = v[0];
+        o[NP + q] = v[1];
+        o[2 * NP + q] = v[2];
+        o[3 * NP + q] = v[3];
     }
 }
 
 template <class T>
 void launch_boundary(const C2<T> *in, C2<T> *out, const C2<T> *cf_prev, const C2<T> *cf_next, const Geo &g,
                      const PropParams *pp, int nprops, int nshift, hipStream_t st) {
-    dim3 grid((g.Mx + kBTW - 1) / kBTW, (g.My + kBTH - 1) / kBTH, nprops);
+    if (nprops % 2) throw std::runtime_error("slab boundary: propagations come in polarisation pairs");
+    dim3 grid((g.Mx + kBTW - 1) / kBTW, (g.My + kBTH - 1) / kBTH, nprops / 2);
     hipLaunchKernelGGL(k_boundary<T>, grid, dim3(256), 0, st, in, out, cf_prev, cf_next, g, pp, nshift);
     NMK_LAUNCH_CHECK();
 }
