@@ -65,7 +65,7 @@ typedef struct {
                               y components, bit 2 polarisation pairs adjacent
                               in the XCD work ranges                            */
     int32_t lean;        /*   1: register-lean solver variants (large batches)   */
-    int32_t reserved;
+    int32_t y_pair;      /*   1: y-line blocks hold both polarisations           */
 } nmk_bpm_options;
 
 typedef struct {

@@ -20,7 +20,7 @@ class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("wl_batch", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("profile", ctypes.c_int32), ("x_chunks", ctypes.c_int32),
                 ("x_rows", ctypes.c_int32), ("y_chunks", ctypes.c_int32), ("y_cols", ctypes.c_int32),
-                ("item_order", ctypes.c_int32), ("lean", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("item_order", ctypes.c_int32), ("lean", ctypes.c_int32), ("y_pair", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -123,7 +123,7 @@ def dptr(a):
 
 
 def options(precision=64, wl_batch=1, use_graph=False, device=-1, profile=False, tuning=None):
-    """tuning: None (built-in shapes) or dict(x_chunks, x_rows, y_chunks, y_cols[, order 0..7][, lean 0/1])"""
+    """tuning: None (built-in shapes) or dict(x_chunks, x_rows, y_chunks, y_cols[, order 0..7][, lean][, y_pair])"""
     o = Options()
     load().nmk_bpm_default_options(ctypes.byref(o))
     o.precision = int(precision)
@@ -137,6 +137,7 @@ def options(precision=64, wl_batch=1, use_graph=False, device=-1, profile=False,
         if tuning.get("order") is not None:
             o.item_order = 1 + int(tuning["order"])
         o.lean = 1 if tuning.get("lean") else 0
+        o.y_pair = 1 if tuning.get("y_pair") else 0
     return o
 
 

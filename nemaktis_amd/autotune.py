@@ -61,15 +61,16 @@ def x_candidates(Mx, precision):
     return out
 
 
-def y_candidates(My, precision):
+def y_candidates(My, precision, pair=False):
     c = 8 if precision == 32 else 16
+    k = 2 if pair else 1
     out = []
     for chunks in _pow2s(16, 512):
         P, lmax = _plan(My, chunks)
         if lmax == 0 or lmax > 16 or P != chunks:
             continue
         for cols in (1, 2, 4, 8, 16, 32):
-            if P * cols <= 512 and _lds(My, chunks, cols, c, 1, True) <= LDS_MAX:
+            if k * P * cols <= 512 and _lds(My, chunks, cols, c, k, True) <= LDS_MAX:
                 out.append((chunks, cols))
     return out
 
@@ -100,7 +101,7 @@ def _save_cache(c):
 
 
 def cache_key(precision, Nx, Ny, nprops):
-    return "gfx950|solver-v5|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
+    return "gfx950|solver-v6|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
 
 
 def _time(js, lc, n_E, precision, wl_batch, tuning, reps):
@@ -157,11 +158,21 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
         table.append((cand, ms))
         if ms < best_ms:
             best, best_ms = cand, ms
+    # y blocks holding both polarisations (one coefficient gather for the two)
+    for (yc, yw) in y_candidates(My, precision, pair=True):
+        if yw < 4:
+            continue
+        cand = dict(best, y_chunks=yc, y_cols=yw, y_pair=1)
+        ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+        table.append((cand, ms))
+        if ms < best_ms:
+            best, best_ms = cand, ms
     # register-lean variants (more waves per SIMD) with the shapes found so far
     # and with shorter chunks
     for (xc, yc) in [(best["x_chunks"], best["y_chunks"]), (2 * best["x_chunks"], 2 * best["y_chunks"])]:
         cand = dict(best, x_chunks=xc, y_chunks=yc, lean=1)
-        if (xc, cand["x_rows"]) not in xs or (yc, cand["y_cols"]) not in ys:
+        if (xc, cand["x_rows"]) not in xs or (yc, cand["y_cols"]) not in y_candidates(My, precision,
+                                                                                     pair=bool(cand.get("y_pair"))):
             continue
         ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
         table.append((cand, ms))

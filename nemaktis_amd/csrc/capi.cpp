@@ -38,6 +38,7 @@ Options to_options(const nmk_bpm_options *o) {
             r.tuning.y_cols = o->y_cols;
             if (o->item_order > 0) r.tuning.order = (o->item_order - 1) & 7;
             r.tuning.lean = o->lean ? 1 : 0;
+            r.tuning.y_pair = o->y_pair ? 1 : 0;
         }
     }
     return r;

@@ -503,12 +503,12 @@ private:
             // y half-steps: cur = (I - mu A_y)^-1 (I + mu A_x) other by its Neumann series
             if (g_.py) {
                 timed(KF_SOLVE_Y, [&] {
-                    launch_ysolve<T>(0, other, A, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, tune_.lean, st_);
+                    launch_ysolve<T>(0, other, A, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, tune_.lean, tune_.y_pair, st_);
                 });
                 C2<T> *a = A, *b = B;
                 for (int it = 0; it < nw; it++) {
                     timed(KF_SOLVE_Y, [&] {
-                        launch_ysolve<T>(1, a, b, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, tune_.lean, st_);
+                        launch_ysolve<T>(1, a, b, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, tune_.lean, tune_.y_pair, st_);
                     });
                     std::swap(a, b);
                 }
@@ -542,7 +542,8 @@ private:
                 throw std::runtime_error("x line solver tuning exceeds the block limits");
             xt_.alloc(2 * (size_t)sz_.nslots_max * 2 * 3 * g_.My * lpx_.lmax * lpx_.P);
         }
-        if (g_.py && (lpy_.P * t.y_cols > 512 || ysolve_lds_bytes<T>(lpy_, t.y_cols) > max_block_lds()))
+        if (g_.py && ((t.y_pair ? 2 : 1) * lpy_.P * t.y_cols > 512 ||
+                      ysolve_lds_bytes<T>(lpy_, t.y_cols, t.y_pair) > max_block_lds()))
             throw std::runtime_error("y line solver tuning exceeds the block limits");
         for (auto &kv : graphs_) {
             (void)hipGraphExecDestroy(kv.second.exec);
@@ -577,7 +578,7 @@ Tuning default_tuning(int precision, int Mx, int My) {
     };
     auto ylds = [&](int chunks, int cols) {
         const LinePlan lp = make_line_plan(My, chunks);
-        return precision == 32 ? ysolve_lds_bytes<float>(lp, cols) : ysolve_lds_bytes<double>(lp, cols);
+        return precision == 32 ? ysolve_lds_bytes<float>(lp, cols, 0) : ysolve_lds_bytes<double>(lp, cols, 0);
     };
     // chunks of at most 8 points (16 for lines longer than 1024)
     t.x_chunks = std::min(128, std::max(16, pow2_at_least((Mx + 7) / 8)));

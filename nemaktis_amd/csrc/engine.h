@@ -19,6 +19,7 @@ struct Tuning {
     int y_chunks = 0, y_cols = 0;  // y lines: chunks per column, columns per block
     int order = -1;                // XCD work order of the solvers (bits, see launch_xsolve); -1: by batch size
     int lean = 0;                  // 1: register-lean solver variants (see launch_xsolve)
+    int y_pair = 0;                // 1: y-line blocks hold both polarisations
 };
 Tuning default_tuning(int precision, int Mx, int My);
 

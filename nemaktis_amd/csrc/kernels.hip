@@ -1404,6 +1404,166 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     }
 }
 
+// ---- y lines, both polarisations per block (2 W P threads,
+// polarisation-major): the right-hand sides of the W x M tile are gathered for
+// the two polarisations together (one set of coefficient loads), each chunk
+// thread then solves one polarisation as in k_ysolve.
+template <class T, int LMAX, int NEU, int LEAN>
+__global__ __launch_bounds__(512) void k_ysolve_pair(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
+                                                      C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf, Geo g,
+                                                      const PropParams *__restrict__ pp, LinePlan lp, int W,
+                                                      int npairs, int order) {
+    using C = C2<T>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int P = lp.P, M = g.My, TL = LMAX * P * W, WP = W * P;
+    C *sd = reinterpret_cast<C *>(smem_raw);  // [pol][TL]
+    const RedLds<T> red(sd + 2 * TL, 2 * WP);
+    int *ymap = reinterpret_cast<int *>(sd + 2 * TL + RedLds<T>::kArrays * 2 * WP);  // row j -> i*P + p
+    // work items (component and strip in the order `order & 2` selects, pair
+    // outermost or, with `order & 4`, innermost), XCD-contiguous ranges
+    const int nstrip = (g.Mx + W - 1) / W;
+    const int item = xcd_tile(blockIdx.x, gridDim.x);
+    if (item >= nstrip * 2 * npairs) return;
+    int comp, lb, pair, rest;
+    if (order & 4) {
+        pair = item % npairs;
+        rest = item / npairs;
+    } else {
+        pair = item / (2 * nstrip);
+        rest = item % (2 * nstrip);
+    }
+    if (order & 2) {
+        comp = rest & 1;
+        lb = rest >> 1;
+    } else {
+        lb = rest % nstrip;
+        comp = rest / nstrip;
+    }
+    const int x0 = lb * W, prop0 = pair * 2;
+    const int slot = pp[prop0].slot;
+    const size_t NP = (size_t)g.NP;
+    const C *cfs = cf + (size_t)slot * cf_slot_size(g.NP);
+    const PairFields<T> fin(src, prop0, NP);
+    const int wv = min(W, g.Mx - x0), lw = __ffs(W) - 1;
+    if ((int)threadIdx.x < P) {
+        const int pc = threadIdx.x, s0c = chunk_start(lp, pc), lc = chunk_len(lp, pc);
+        for (int i = 0; i < lc; i++) ymap[s0c + i] = i * P + pc;
+    }
+    __syncthreads();
+    C *sd0 = sd, *sd1 = sd + TL;
+    // staging and output: a thread keeps column cc and steps 2P rows at a time
+    const int cc = threadIdx.x & (W - 1), jr = (int)threadIdx.x >> lw;
+    const bool col_ok = cc < wv;
+
+    if (col_ok) {
+        constexpr int KIND = NEU ? RHS_OY : RHS_XPLUS, NPR = LEAN ? 1 : rhs_points<T, LMAX>();
+        for (int j = jr; j < M; j += NPR * 2 * P) {
+            unsigned xs[NPR], ys[NPR];
+            bool v[NPR];
+#pragma unroll
+            for (int k = 0; k < NPR; k++) {
+                v[k] = j + k * 2 * P < M;
+                xs[k] = x0 + cc;
+                ys[k] = v[k] ? j + k * 2 * P : 0;
+            }
+            C d[NPR][2];
+            rhs_pair_n<T, KIND, NPR>(comp, xs, ys, v, cfs, fin, g, d);
+#pragma unroll
+            for (int k = 0; k < NPR; k++)
+                if (v[k]) {
+                    const int ia = ymap[ys[k]] * W + cc;
+                    sd0[ia] = d[k][0];
+                    sd1[ia] = d[k][1];
+                }
+        }
+    }
+    __syncthreads();
+
+    // one chunk of one column of one polarisation per thread
+    const int pol = threadIdx.x / WP, t = threadIdx.x - pol * WP, cofs = t & (W - 1), p = t >> lw, x = x0 + cofs;
+    const bool active = cofs < wv;
+    C *sdl = sd + pol * TL;
+    const int blk = comp == 0 ? 0 : 3;
+    const C *cdy = cfs + (CF_DY + blk * 3) * kCfTile;
+    const int s = chunk_start(lp, p), n = chunk_len(lp, p) - 1;
+    auto tri = [&](int j, C &a, C &b, C &c) {
+        j = min(j, M - 1);  // rows past the line end are fetched but not used
+        const C *c3 = cdy + cf_point((unsigned)(x + g.Mx * j));
+        const C va = c3[0], vb = c3[kCfTile], vc = c3[2 * kCfTile];
+        a = C{-va.x, -va.y};
+        b = C{(T)1 - vb.x, -vb.y};
+        c = C{-vc.x, -vc.y};
+    };
+    const int base = pol * WP + cofs, me = base + p * W, nx = base + ((p + 1) & (P - 1)) * W;
+    Chunk<T, LMAX> ck;
+    if (active) {
+        chunk_eliminate<T, LMAX, sizeof(T) == 4 && !LEAN>(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
+            tri(s + i, a, b, c);
+            d = sdl[(i * P + p) * W + cofs];
+        });
+        red.uf[me] = ck.uf;
+        red.gf[me] = ck.gf;
+        red.hf[me] = ck.hf;
+    }
+    __syncthreads();
+    C A{0, 0}, B{0, 0}, Cv{0, 0}, D{0, 0};
+    if (active) {
+        C ae, be, ce;
+        tri(s + n, ae, be, ce);
+        interface_row(ck, ae, be, ce, sdl[(n * P + p) * W + cofs], red, nx, A, B, Cv, D);
+    }
+    __syncthreads();
+    if (active) {
+        red.A[me] = A;
+        red.B[me] = B;
+        red.Cc[me] = Cv;
+        red.D[me] = D;
+    }
+    __syncthreads();
+    reduced_pcr(red, P, p, base, W, active);
+    if (active) {
+        const C xi = red.uf[me], xprev = red.uf[base + ((p - 1) & (P - 1)) * W];
+        chunk_rebuild(ck, n, xprev, xi, [&](int i, C v) { sdl[(i * P + p) * W + cofs] = v; });
+        sdl[(n * P + p) * W + cofs] = xi;
+    }
+    __syncthreads();
+
+    {
+        // a thread stores at most LMAX/2 points of its column for each
+        // polarisation (M <= P*LMAX, rows step 2P); accumulators in one round trip
+        const size_t o0 = ((size_t)prop0 * 2 + comp) * NP, o1 = o0 + 2 * NP;
+        C *d0 = dst + o0, *d1 = dst + o1, *a0 = acc + o0, *a1 = acc + o1;
+        constexpr int NO = (LMAX + 1) / 2;
+        C o0v[NO], o1v[NO];
+        if (NEU && col_ok) {
+#pragma unroll
+            for (int k = 0; k < NO; k++) {
+                const int j = jr + k * 2 * P;
+                if (j < M) {
+                    const unsigned q = x0 + cc + g.Mx * j;
+                    o0v[k] = a0[q];
+                    o1v[k] = a1[q];
+                }
+            }
+        }
+        if (col_ok) {
+#pragma unroll
+            for (int k = 0; k < NO; k++) {
+                const int j = jr + k * 2 * P;
+                if (j < M) {
+                    const int ia = ymap[j] * W + cc;
+                    const C v0 = sd0[ia], v1 = sd1[ia];
+                    const unsigned q = x0 + cc + g.Mx * j;
+                    d0[q] = v0;
+                    d1[q] = v1;
+                    a0[q] = NEU ? o0v[k] + v0 : v0;
+                    a1[q] = NEU ? o1v[k] + v1 : v1;
+                }
+            }
+        }
+    }
+}
+
 // dynamic LDS above the 64 KiB default needs an opt-in per kernel
 size_t max_block_lds() {
     int dev = 0, v = 0;
@@ -1449,8 +1609,8 @@ size_t xsolve_lds_bytes(const LinePlan &lp, int R) {
            (size_t)lp.M * sizeof(int);
 }
 template <class T>
-size_t ysolve_lds_bytes(const LinePlan &lp, int W) {
-    return (size_t)W * ((size_t)lp.lmax * lp.P + (size_t)RedLds<T>::kArrays * lp.P) * sizeof(C2<T>) +
+size_t ysolve_lds_bytes(const LinePlan &lp, int W, int pair) {
+    return (size_t)(pair ? 2 : 1) * W * ((size_t)lp.lmax * lp.P + (size_t)RedLds<T>::kArrays * lp.P) * sizeof(C2<T>) +
            (size_t)lp.M * sizeof(int);
 }
 
@@ -1479,11 +1639,38 @@ static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *s
 }
 
 template <class T, int NEU, int LEAN>
+static void ysolve_pair_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
+                                 const C2<T> *cf, const Geo &g, const PropParams *pp, int npairs, int order,
+                                 hipStream_t st) {
+    const dim3 block(2 * lp.P * W);
+    const size_t lds = ysolve_lds_bytes<T>(lp, W, 1);
+    switch (lp.lmax) {
+        case 4:
+            allow_lds(k_ysolve_pair<T, 4, NEU, LEAN>, lds);
+            hipLaunchKernelGGL((k_ysolve_pair<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W,
+                               npairs, order);
+            break;
+        case 8:
+            allow_lds(k_ysolve_pair<T, 8, NEU, LEAN>, lds);
+            hipLaunchKernelGGL((k_ysolve_pair<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W,
+                               npairs, order);
+            break;
+        case 16:
+            allow_lds(k_ysolve_pair<T, 16, NEU, LEAN>, lds);
+            hipLaunchKernelGGL((k_ysolve_pair<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp,
+                               W, npairs, order);
+            break;
+        default: throw std::runtime_error("y line solver: unsupported chunk length");
+    }
+    NMK_LAUNCH_CHECK();
+}
+
+template <class T, int NEU, int LEAN>
 static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
                             const C2<T> *cf, const Geo &g, const PropParams *pp, int npairs, int order,
                             hipStream_t st) {
     const dim3 block(lp.P * W);
-    const size_t lds = ysolve_lds_bytes<T>(lp, W);
+    const size_t lds = ysolve_lds_bytes<T>(lp, W, 0);
     switch (lp.lmax) {
         case 4:
             allow_lds(k_ysolve<T, 4, NEU, LEAN>, lds);
@@ -1523,12 +1710,24 @@ void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const 
 
 template <class T>
 void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
-                   const PropParams *pp, int nprops, const LinePlan &lp, int W, int order, int lean,
+                   const PropParams *pp, int nprops, const LinePlan &lp, int W, int order, int lean, int pair,
                    hipStream_t st) {
-    if (lp.P * W > 512 || lp.M != g.My || W < 1 || (W & (W - 1)) || lp.P * lp.lmax < lp.M)
+    if ((pair ? 2 : 1) * lp.P * W > 512 || lp.M != g.My || W < 1 || (W & (W - 1)) || lp.P * lp.lmax < lp.M)
         throw std::runtime_error("y line solver: bad launch shape");
     if (nprops % 2) throw std::runtime_error("y line solver: propagations come in polarisation pairs");
     const int npairs = nprops / 2;
+    if (pair) {
+        const int64_t items = (int64_t)((g.Mx + W - 1) / W) * 2 * npairs;
+        dim3 grid((unsigned)((items + 7) / 8 * 8));
+        if (lean) {
+            if (neumann) ysolve_pair_dispatch<T, 1, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+            else ysolve_pair_dispatch<T, 0, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+        } else {
+            if (neumann) ysolve_pair_dispatch<T, 1, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+            else ysolve_pair_dispatch<T, 0, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+        }
+        return;
+    }
     const int64_t items = (int64_t)((g.Mx + W - 1) / W) * 4 * npairs;
     dim3 grid((unsigned)((items + 7) / 8 * 8));
     if (lean) {
@@ -1795,9 +1994,9 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
     template void launch_xsolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const C2<T> *, const Geo &, \
                                    const PropParams *, int, const LinePlan &, int, int, int, hipStream_t);            \
     template void launch_ysolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const Geo &,                  \
-                                   const PropParams *, int, const LinePlan &, int, int, int, hipStream_t);            \
+                                   const PropParams *, int, const LinePlan &, int, int, int, int, hipStream_t);       \
     template size_t xsolve_lds_bytes<T>(const LinePlan &, int);                                                       \
-    template size_t ysolve_lds_bytes<T>(const LinePlan &, int);                                                       \
+    template size_t ysolve_lds_bytes<T>(const LinePlan &, int, int);                                                  \
     template void launch_output<T>(const C2<T> *, const double *, const uint8_t *, const Geo &, const SlotParams *,  \
                                    const PropParams *, int, int, c64 *, hipStream_t);
 NMK_INST(float)

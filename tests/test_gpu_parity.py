@@ -205,6 +205,7 @@ def test_random512_reduced_depth_fp32_vs_fp64():
     {"x_chunks": 8, "x_rows": 4, "y_chunks": 8, "y_cols": 4},     # chunks of 8-9 points (16-point solver)
     {"x_chunks": 32, "x_rows": 1, "y_chunks": 32, "y_cols": 1},   # chunks of 2-3 points
     {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8},   # chunks of 4-5 points
+    {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8, "y_pair": 1, "lean": 1},  # paired y, lean
 ])
 def test_line_solver_shapes_fp64(tuning):
     """Every autotuner candidate shape must give the oracle's answer."""
@@ -220,6 +221,7 @@ def test_xcd_work_orders_are_bitwise_equal():
     s, lc = _random_case((70, 66, 5), seed=19)
     base = {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8}
     outs = [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o)) for o in range(8)]
+    outs += [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o, y_pair=1)) for o in (0, 6)]
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
     want = oracle.propagate(s, lc)
