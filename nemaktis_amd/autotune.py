@@ -3,11 +3,13 @@
 The two cyclic line solvers (x lines and y lines) cut each periodic line into
 P chunks and put several lines in one block (rows for x lines, adjacent
 columns for y lines).  The best shapes depend on the mesh, the precision and
-how many propagations march together, so they are measured: every candidate
-shape that fits the block limits (<= 512 threads, <= 64 KiB LDS, chunks of at
-most 16 points) is timed on a shallow copy of the job (same transverse
-mesh, a few slabs), by coordinate descent (x shape first, then y shape), and
-the winner is cached per (device, precision, mesh, propagation count).
+how many propagations march together, so they are measured on a shallow copy
+of the job (same transverse mesh, a few slabs), by coordinate descent: every
+x shape that fits the block limits (<= 512 threads, the device's LDS per
+block, chunks of at most 16 points), then every y shape (one and two
+polarisations per block), x again, the register-lean variants, and the eight
+XCD work orders.  The winner is cached per (device, precision, mesh,
+propagation count).
 """
 import json
 import os
@@ -101,7 +103,7 @@ def _save_cache(c):
 
 
 def cache_key(precision, Nx, Ny, nprops):
-    return "gfx950|solver-v6|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
+    return "gfx950|solver-v7|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
 
 
 def _time(js, lc, n_E, precision, wl_batch, tuning, reps):
@@ -163,6 +165,15 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
         if yw < 4:
             continue
         cand = dict(best, y_chunks=yc, y_cols=yw, y_pair=1)
+        ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+        table.append((cand, ms))
+        if ms < best_ms:
+            best, best_ms = cand, ms
+    # second pass over the x shapes with the y shape found (coordinate descent)
+    for (xc, xr) in xs:
+        cand = dict(best, x_chunks=xc, x_rows=xr)
+        if cand == best:
+            continue
         ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
         table.append((cand, ms))
         if ms < best_ms:
