@@ -70,12 +70,35 @@ class SrvCounters(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
+class ViewGeometry(ctypes.Structure):
+    _fields_ = [("Nx", ctypes.c_int32), ("Ny", ctypes.c_int32), ("Lx", ctypes.c_double), ("Ly", ctypes.c_double),
+                ("n_wavelengths", ctypes.c_int32), ("wavelengths", c_double_p), ("n_out", c_double_p),
+                ("n_wavevectors", ctypes.c_int32), ("wavevectors", c_double_p),
+                ("max_NA_objective", ctypes.c_double), ("max_NA_condenser", ctypes.c_double),
+                ("N_radial_wavevectors", ctypes.c_int32), ("koehler_1D", ctypes.c_int32),
+                ("zfoc_NA_corr", ctypes.c_double), ("device", ctypes.c_int32)]
+
+
+class ViewOptics(ctypes.Structure):
+    _fields_ = [("polariser", ctypes.c_int32), ("analyser", ctypes.c_int32),
+                ("polariser_angle", ctypes.c_double), ("analyser_angle", ctypes.c_double),
+                ("lower_waveplate", ctypes.c_int32), ("upper_waveplate", ctypes.c_int32),
+                ("lower_waveplate_angle", ctypes.c_double), ("upper_waveplate_angle", ctypes.c_double),
+                ("intensity", ctypes.c_double), ("NA_condenser", ctypes.c_double),
+                ("grayscale", ctypes.c_int32), ("gamma", ctypes.c_int32)]
+
+
+WAVEPLATES = {"No": 0, "Quarter-wave": 1, "Half-wave": 2, "Tint-sensitive": 3}
+
 EXPORTED = [
     "nmk_bpm_run_backend_without_mask", "nmk_bpm_run_backend_with_mask", "nmk_bpm_default_options",
     "nmk_bpm_run", "nmk_bpm_create", "nmk_bpm_upload_fields", "nmk_bpm_propagate", "nmk_bpm_download",
     "nmk_bpm_destroy", "nmk_bpm_estimate_footprint", "nmk_bpm_eval_expression", "nmk_bpm_version",
     "nmk_srv_default_config", "nmk_srv_create", "nmk_srv_submit", "nmk_srv_wait", "nmk_srv_cancel",
     "nmk_srv_stats", "nmk_srv_destroy",
+    "nmk_view_default_optics", "nmk_view_create", "nmk_view_from_ctx", "nmk_view_set_fields", "nmk_view_focus",
+    "nmk_view_get_focused", "nmk_view_render", "nmk_view_get_spectrum", "nmk_view_get_image", "nmk_view_state",
+    "nmk_view_timings", "nmk_view_destroy",
 ]
 
 
@@ -114,6 +137,20 @@ def load():
     lib.nmk_srv_cancel.argtypes = [vp, i64]
     lib.nmk_srv_stats.argtypes = [vp, ctypes.POINTER(SrvCounters)]
     lib.nmk_srv_destroy.argtypes = [vp]
+    lib.nmk_view_default_optics.argtypes = [ctypes.POINTER(ViewOptics)]
+    lib.nmk_view_create.argtypes = [ctypes.POINTER(ViewGeometry), c_double_p, i64, cp, sz]
+    lib.nmk_view_create.restype = vp
+    lib.nmk_view_from_ctx.argtypes = [vp, ctypes.POINTER(ViewGeometry), cp, sz]
+    lib.nmk_view_from_ctx.restype = vp
+    lib.nmk_view_set_fields.argtypes = [vp, c_double_p, i64, cp, sz]
+    lib.nmk_view_focus.argtypes = [vp, ctypes.c_double, ctypes.c_double, cp, sz]
+    lib.nmk_view_get_focused.argtypes = [vp, c_double_p, i64, cp, sz]
+    lib.nmk_view_render.argtypes = [vp, ctypes.POINTER(ViewOptics), c_double_p, i64, cp, sz]
+    lib.nmk_view_get_spectrum.argtypes = [vp, c_double_p, i64, cp, sz]
+    lib.nmk_view_get_image.argtypes = [vp, c_double_p, i64, cp, sz]
+    lib.nmk_view_state.argtypes = [vp, c_double_p, c_double_p]
+    lib.nmk_view_timings.argtypes = [vp, c_double_p, ctypes.c_int32]
+    lib.nmk_view_destroy.argtypes = [vp]
     _lib = lib
     return lib
 

@@ -127,6 +127,7 @@ class Context:
         lc = _lib.as_lc(lc_field_vals)
         mask = None if mask_vals is None else _lib.as_lc(mask_vals)
         self.n_E = int(n_E)
+        self.json_str = json_str
         opts = _lib.options(precision=precision, wl_batch=wl_batch, use_graph=use_graph, profile=profile,
                             device=device, tuning=tuning)
         err = _lib.err_buf()

@@ -22,8 +22,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + 
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 
-HIP_SOURCES = ["kernels.hip"]
-CXX_SOURCES = ["json.cpp", "expr.cpp", "settings.cpp", "engine.cpp", "server.cpp", "capi.cpp"]
+HIP_SOURCES = ["kernels.hip", "viewer_kernels.hip"]
+CXX_SOURCES = ["json.cpp", "expr.cpp", "settings.cpp", "engine.cpp", "server.cpp", "viewer.cpp", "capi.cpp"]
 
 
 def _run(cmd):

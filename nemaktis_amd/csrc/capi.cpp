@@ -1,6 +1,9 @@
 // extern "C" entry points of libnemaktis_bpm.so (declared in include/nemaktis_bpm.h).
 #include "../../include/nemaktis_bpm.h"
+#include "../../include/nemaktis_view.h"
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -11,6 +14,7 @@
 #include "expr.h"
 #include "server.h"
 #include "settings.h"
+#include "viewer.h"
 
 using namespace nmk;
 
@@ -95,7 +99,58 @@ Problem problem_from(const char *json, const char *formula, const double *lc, in
     return build_problem(s, lc, n_lc, mask, n_mask, mask ? (formula ? formula : "") : nullptr, n_E, warning);
 }
 
+ViewGeometry view_geometry(const nmk_view_geometry *g) {
+    if (!g) throw BpmError("null geometry");
+    ViewGeometry v;
+    v.Nx = g->Nx;
+    v.Ny = g->Ny;
+    v.Lx = g->Lx;
+    v.Ly = g->Ly;
+    if (g->n_wavelengths < 1 || !g->wavelengths) throw BpmError("at least one wavelength is required");
+    v.wl.assign(g->wavelengths, g->wavelengths + g->n_wavelengths);
+    if (g->n_out) v.n_out.assign(g->n_out, g->n_out + g->n_wavelengths);
+    if (g->n_wavevectors > 0) {
+        if (!g->wavevectors) throw BpmError("null wavevectors");
+        for (int q = 0; q < g->n_wavevectors; q++) {
+            v.qx.push_back(g->wavevectors[2 * q]);
+            v.qy.push_back(g->wavevectors[2 * q + 1]);
+        }
+    }
+    v.max_NA_objective = g->max_NA_objective;
+    v.max_NA_condenser = g->max_NA_condenser;
+    v.Nr = g->N_radial_wavevectors;
+    v.koehler_1D = g->koehler_1D != 0;
+    v.zfoc_NA_corr = g->zfoc_NA_corr;
+    v.device = g->device;
+    return v;
+}
+
+ViewOptics view_optics(const nmk_view_optics *o) {
+    ViewOptics r;
+    if (!o) return r;
+    r.polariser = o->polariser != 0;
+    r.analyser = o->analyser != 0;
+    r.polariser_angle = o->polariser_angle;
+    r.analyser_angle = o->analyser_angle;
+    r.lower_waveplate = o->lower_waveplate;
+    r.upper_waveplate = o->upper_waveplate;
+    for (int k : {r.lower_waveplate, r.upper_waveplate})
+        if (k < NMK_VIEW_NO_WAVEPLATE || k > NMK_VIEW_TINT_SENSITIVE)
+            throw BpmError("unknown waveplate type " + std::to_string(k));
+    r.lower_waveplate_angle = o->lower_waveplate_angle;
+    r.upper_waveplate_angle = o->upper_waveplate_angle;
+    r.intensity = o->intensity;
+    r.NA_condenser = o->NA_condenser;
+    r.grayscale = o->grayscale != 0;
+    r.gamma = o->gamma != 0;
+    return r;
+}
+
 }  // namespace
+
+struct nmk_view {
+    std::unique_ptr<Viewer> v;
+};
 
 struct nmk_bpm_ctx {
     Problem problem;
@@ -314,5 +369,145 @@ int nmk_srv_stats(nmk_srv *srv, nmk_srv_counters *out) {
 }
 
 void nmk_srv_destroy(nmk_srv *srv) { delete reinterpret_cast<Server *>(srv); }
+
+/* ---- microscope imaging ---------------------------------------------------- */
+
+void nmk_view_default_optics(nmk_view_optics *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->polariser = 1;
+    o->analyser = 1;
+    o->polariser_angle = 0;
+    o->analyser_angle = 90;
+    o->intensity = 1;
+    o->gamma = 1;
+}
+
+nmk_view *nmk_view_create(const nmk_view_geometry *geometry, const double *E, int64_t n_E, char *err,
+                          size_t errlen) {
+    nmk_view *out = nullptr;
+    guarded(err, errlen, [&] {
+        std::unique_ptr<nmk_view> h(new nmk_view);
+        h->v.reset(new Viewer(view_geometry(geometry)));
+        if (E) {
+            if (n_E != h->v->geometry().n_values()) throw BpmError("Wrong shape for the optical field ndarray");
+            h->v->set_fields(reinterpret_cast<const c64 *>(E));
+        }
+        out = h.release();
+    });
+    return out;
+}
+
+nmk_view *nmk_view_from_ctx(nmk_bpm_ctx *ctx, const nmk_view_geometry *geometry, char *err, size_t errlen) {
+    nmk_view *out = nullptr;
+    guarded(err, errlen, [&] {
+        if (!ctx) throw BpmError("null context");
+        const Problem &p = ctx->problem;
+        ViewGeometry g;
+        if (geometry) {
+            g = view_geometry(geometry);
+        } else {
+            // as LightPropagator._bpm_propagation builds its OpticalFields
+            g.Nx = p.Nx;
+            g.Ny = p.Ny;
+            g.Lx = p.dx * std::max(1, p.Nx - 1);
+            g.Ly = p.dy * std::max(1, p.Ny - 1);
+            g.wl = p.wl;
+            g.n_out = p.nout;
+            g.qx = p.qx;
+            g.qy = p.qy;
+            g.max_NA_objective = p.NA;
+            const double n0 = p.nout_ref;
+            for (size_t i = 0; i < p.iso_thick.size(); i++) {
+                const double nk = p.iso_index[i], ek = p.iso_thick[i];
+                g.zfoc_NA_corr += 3 * n0 * ek / (14 * nk) * (1 / (n0 * n0) - 1 / (nk * nk));
+            }
+            // rings and condenser aperture from the wavevectors, as the VTI import
+            const int Nq = (int)p.qx.size();
+            g.Nr = (int)std::lround((1 + std::sqrt((4.0 * Nq - 1) / 3)) / 2);
+            g.max_NA_condenser = std::sqrt(p.qx.back() * p.qx.back() + p.qy.back() * p.qy.back());
+        }
+        if (g.Nx != p.Nx || g.Ny != p.Ny || g.wl.size() != p.wl.size() ||
+            (!g.qx.empty() && g.qx.size() != p.qx.size()))
+            throw BpmError("the geometry does not match the context's fields");
+        std::unique_ptr<nmk_view> h(new nmk_view);
+        h->v.reset(new Viewer(g));
+        if (h->v->geometry().n_values() != p.n_out_values())
+            throw BpmError("the geometry does not match the context's fields");
+        h->v->set_fields_device(ctx->runner->device_output(), ctx->runner->stream());
+        out = h.release();
+    });
+    return out;
+}
+
+int nmk_view_set_fields(nmk_view *v, const double *E, int64_t n_E, char *err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!v) throw BpmError("null view");
+        if (!E || n_E != v->v->geometry().n_values()) throw BpmError("Wrong shape for the optical field ndarray");
+        v->v->set_fields(reinterpret_cast<const c64 *>(E));
+    });
+}
+
+int nmk_view_focus(nmk_view *v, double z_focus, double NA_objective, char *err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!v) throw BpmError("null view");
+        v->v->focus(z_focus, NA_objective);
+    });
+}
+
+int nmk_view_get_focused(nmk_view *v, double *E, int64_t n_E, char *err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!v) throw BpmError("null view");
+        if (!E || n_E != v->v->geometry().n_values()) throw BpmError("Wrong shape for the optical field ndarray");
+        v->v->download_focused(reinterpret_cast<c64 *>(E));
+    });
+}
+
+int nmk_view_render(nmk_view *v, const nmk_view_optics *optics, const double *cmf, int64_t n_cmf, char *err,
+                    size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!v) throw BpmError("null view");
+        if (cmf && n_cmf != 3 * (int64_t)v->v->geometry().wl.size())
+            throw BpmError("the colour matching table needs 3 values per wavelength");
+        nmk_view_optics d;
+        nmk_view_default_optics(&d);
+        v->v->render(view_optics(optics ? optics : &d), cmf);
+    });
+}
+
+int nmk_view_get_spectrum(nmk_view *v, double *spec, int64_t n, char *err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!v) throw BpmError("null view");
+        const ViewGeometry &g = v->v->geometry();
+        if (!spec || n != (int64_t)g.Nx * g.Ny * (int64_t)g.wl.size())
+            throw BpmError("the spectrum array needs Ny * Nx * N_wavelengths values");
+        v->v->download_spectrum(spec);
+    });
+}
+
+int nmk_view_get_image(nmk_view *v, double *img, int64_t n, char *err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!v) throw BpmError("null view");
+        const ViewGeometry &g = v->v->geometry();
+        if (!img || n != (int64_t)g.Nx * g.Ny * 3) throw BpmError("the image array needs Ny * Nx * 3 values");
+        v->v->download_image(img);
+    });
+}
+
+int nmk_view_state(const nmk_view *v, double *z_focus, double *NA_objective) {
+    if (!v) return NMK_BPM_EINPUT;
+    if (z_focus) *z_focus = v->v->z_focus();
+    if (NA_objective) *NA_objective = v->v->NA_objective();
+    return NMK_BPM_OK;
+}
+
+int nmk_view_timings(const nmk_view *v, double *ms, int32_t n) {
+    if (!v || !ms) return 0;
+    const int k = std::min<int>(n, Viewer::T_COUNT);
+    for (int i = 0; i < k; i++) ms[i] = v->v->timings()[i];
+    return k;
+}
+
+void nmk_view_destroy(nmk_view *v) { delete v; }
 
 }  // extern "C"

@@ -73,6 +73,12 @@ __host__ __device__ __forceinline__ C2<T> cast2(C2<U> a) {
     return {(T)a.x, (T)a.y};
 }
 
+// Under round-robin dispatch XCD k receives blocks k, k+8, k+16, ...; this
+// remap gives them consecutive work items (nbx = gridDim.x, a multiple of 8),
+// so items that read the same data (neighbouring rows, the components of one
+// plane) run on one XCD and share its L2.
+__device__ __forceinline__ int xcd_tile(int bx, int nbx) { return (bx & 7) * (nbx >> 3) + (bx >> 3); }
+
 // ---- launch-time parameter blocks ----------------------------------------
 
 // geometry of the run: full grid (with the duplicate column/row) and the

@@ -1,0 +1,92 @@
+// Device side of the mixed-radix FFT used by the screen filter and the
+// objective refocus: Stockham autosort stages over C lines held together in
+// LDS, ping-ponging between two buffers.  Element k of line i sits at
+// k*C + i, so the C lines of a block are read from HBM as rows of C
+// consecutive complex values (columns of a plane) and the threads of a wave
+// touch consecutive LDS words.  Stage with radix r after sub-transforms of
+// length L:
+//   y'[j + L p + L r k] = sum_q w_r^{pq} ( w_{Lr}^{jq} y[j + L k + (n/r) q] )
+// Twiddles tw[t] = exp(-2 pi i t / n); sign < 0 is the forward transform
+// (exp(-i)), sign > 0 the unnormalised backward one.
+#pragma once
+
+#include "device_common.h"
+#include "kernels.h"
+
+namespace nmk {
+
+// one stage of radix R: butterflies b (and lines i) of this thread
+template <int C, int R>
+__device__ __forceinline__ void fft_stage(const c64 *__restrict__ b0, c64 *__restrict__ b1, int n, int L,
+                                          const c64 *__restrict__ tw, int sign) {
+    const int Lr = L * R, nb = n / R, tstep = n / Lr, rstep = n / R;
+    auto W = [&](int idx) -> c64 {
+        const c64 w = tw[idx];
+        return sign > 0 ? conj(w) : w;
+    };
+    // R-th roots of unity of the butterfly (uniform across the block)
+    c64 wr[R];
+#pragma unroll
+    for (int m = 0; m < R; m++) wr[m] = (R == 2 || R == 4) ? mk(1.0, 0.0) : W(m * rstep);
+    for (int t = threadIdx.x; t < nb * C; t += blockDim.x) {
+        const int i = t % C, b = t / C;
+        const int j = b % L, k = b / L;
+        c64 u[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            const c64 v = b0[(j + L * k + rstep * q) * C + i];
+            u[q] = (q == 0 || j == 0) ? v : v * W(j * q * tstep);
+        }
+        c64 *o = b1 + (j + Lr * k) * C + i;
+        const int os = L * C;
+        if constexpr (R == 2) {
+            o[0] = u[0] + u[1];
+            o[os] = u[0] - u[1];
+        } else if constexpr (R == 4) {
+            // w_4 = -i (forward) or +i (backward)
+            const c64 t0 = u[0] + u[2], t1 = u[0] - u[2], t2 = u[1] + u[3], d = u[1] - u[3];
+            const c64 t3 = sign > 0 ? mk(-d.y, d.x) : mk(d.y, -d.x);
+            o[0] = t0 + t2;
+            o[os] = t1 + t3;
+            o[2 * os] = t0 - t2;
+            o[3 * os] = t1 - t3;
+        } else {
+#pragma unroll
+            for (int pq = 0; pq < R; pq++) {
+                c64 a = u[0];
+#pragma unroll
+                for (int q = 1; q < R; q++) a = cmad(a, u[q], wr[(pq * q) % R]);
+                o[pq * os] = a;
+            }
+        }
+    }
+}
+
+// runs every stage of plan pl on the C lines in b0 (all threads of the block
+// take part; the caller has synchronised after filling b0) and returns the
+// buffer that holds the result
+template <int C>
+__device__ c64 *fft_stages(c64 *b0, c64 *b1, const FftPlan &pl, const c64 *__restrict__ tw, int sign) {
+    const int n = pl.n;
+    int L = 1;
+    for (int st = 0; st < pl.nst; st++) {
+        const int r = (int)((pl.radices >> (4 * st)) & 15);
+        switch (r) {
+        case 4: fft_stage<C, 4>(b0, b1, n, L, tw, sign); break;
+        case 2: fft_stage<C, 2>(b0, b1, n, L, tw, sign); break;
+        case 3: fft_stage<C, 3>(b0, b1, n, L, tw, sign); break;
+        case 5: fft_stage<C, 5>(b0, b1, n, L, tw, sign); break;
+        case 7: fft_stage<C, 7>(b0, b1, n, L, tw, sign); break;
+        case 11: fft_stage<C, 11>(b0, b1, n, L, tw, sign); break;
+        default: fft_stage<C, 13>(b0, b1, n, L, tw, sign); break;
+        }
+        __syncthreads();
+        c64 *tmp = b0;
+        b0 = b1;
+        b1 = tmp;
+        L *= r;
+    }
+    return b0;
+}
+
+}  // namespace nmk

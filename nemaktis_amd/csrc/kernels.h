@@ -74,14 +74,22 @@ void launch_output(const C2<T> *F, const double *lc, const uint8_t *mask, const 
 
 // screen filter: batched 2-D FFT of howmany planes (Ny x Nx) and the per-k
 // 2x2 transfer filter
+// radices of the stages, 4 bits each from the lowest (at most 16 stages)
 struct FftPlan {
     int n, nst;
-    int radix[24];
+    uint64_t radices;
 };
 FftPlan make_fft_plan(int n);
 bool fft_supported(int n);
 void launch_fft2(c64 *data, int Nx, int Ny, int planes, int sign, const c64 *tw_x, const c64 *tw_y,
                  const FftPlan &px, const FftPlan &py, hipStream_t st);
+// the two passes of launch_fft2: rows (src may equal dst) and columns in place
+void launch_fft_rows(const c64 *src, c64 *dst, int Nx, int Ny, int planes, int sign, const c64 *tw_x,
+                     const FftPlan &px, hipStream_t st);
+void launch_fft_cols(c64 *data, int Nx, int Ny, int planes, int sign, const c64 *tw_y, const FftPlan &py,
+                     hipStream_t st);
+// adjacent columns per workgroup of the column pass (LDS 2*n*C complex128)
+int fft_cols_per_block(int n);
 void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const double *qxy, int nq,
                    const double *iso_thick, const double *iso_index, int n_iso, double dx, double dy,
                    double z_foc, double NA, hipStream_t st);

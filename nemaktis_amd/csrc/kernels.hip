@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "fft_device.h"
 
 #include <map>
 #include <mutex>
@@ -204,10 +205,6 @@ void launch_slab_material(const double *lc, const uint8_t *mask, const double *w
 //   R  = i W + (D S^-1 + S^-1 D) / 4,    W = (v_a d_b + d_a v_b) / 2
 // for the three stencil families DX, DY, DXDY and their 2x2 blocks.
 // ---------------------------------------------------------------------------
-// Under round-robin dispatch XCD k receives blocks k, k+8, k+16, ...; this
-// remap gives them consecutive tiles (nbx = gridDim.x, a multiple of 8), so
-// the neighbouring rows a tile reads were just read into the same L2.
-__device__ __forceinline__ int xcd_tile(int bx, int nbx) { return (bx & 7) * (nbx >> 3) + (bx >> 3); }
 
 template <class T>
 __global__ __launch_bounds__(256) void k_coefs(const double *__restrict__ sm, Geo g, const SlotParams *__restrict__ sp,
@@ -1778,13 +1775,10 @@ void launch_output(const C2<T> *F, const double *lc, const uint8_t *mask, const 
 }
 
 // ---------------------------------------------------------------------------
-// screen filter FFT: mixed-radix Stockham autosort transform of one line per
-// workgroup, ping-ponging between two LDS buffers.  Stage with radix r after
-// sub-transforms of length L:
-//   y'[j + L p + L r k] = sum_q w_r^{pq} ( w_{Lr}^{jq} y[j + L k + (n/r) q] )
+// screen filter FFT: rows as one line per workgroup, columns as C adjacent
+// columns per workgroup (rows of C consecutive values in HBM), both with the
+// Stockham stages of fft_device.h
 // ---------------------------------------------------------------------------
-constexpr int kFftMaxRadix = 13;
-
 bool fft_supported(int n) {
     if (n < 1) return false;
     int m = n;
@@ -1800,85 +1794,78 @@ FftPlan make_fft_plan(int n) {
     pl.nst = 0;
     for (int r : {4, 2, 3, 5, 7, 11, 13})
         while (m % r == 0 && m > 1) {
-            if (pl.nst >= 24) throw std::runtime_error("FFT plan too deep");
-            pl.radix[pl.nst++] = r;
+            if (pl.nst >= 16) throw std::runtime_error("FFT plan too deep");
+            pl.radices |= (uint64_t)r << (4 * pl.nst++);
             m /= r;
         }
     if (m != 1) throw std::runtime_error("FFT length " + std::to_string(n) + " has a prime factor above 13");
     return pl;
 }
 
-__global__ __launch_bounds__(256) void k_fft_lines(c64 *__restrict__ data, int64_t line_stride, int64_t elem_stride,
-                                                   int64_t plane_stride, FftPlan pl, const c64 *__restrict__ tw,
-                                                   int sign) {
+int fft_cols_per_block(int n) { return n <= 256 ? 8 : n <= 512 ? 4 : n <= 1024 ? 2 : 1; }
+
+// rows: src and dst may be the same buffer (the line is held in LDS)
+__global__ __launch_bounds__(256) void k_fft_lines(const c64 *src, c64 *dst, int64_t plane_stride, FftPlan pl,
+                                                   const c64 *__restrict__ tw, int sign) {
     extern __shared__ c64 fbuf[];
     const int n = pl.n;
-    c64 *b0 = fbuf, *b1 = fbuf + n;
-    const int64_t base = (int64_t)blockIdx.y * plane_stride + (int64_t)blockIdx.x * line_stride;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) b0[i] = data[base + (int64_t)i * elem_stride];
+    const int64_t base = (int64_t)blockIdx.y * plane_stride + (int64_t)blockIdx.x * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) fbuf[i] = src[base + i];
     __syncthreads();
-    auto W = [&](int idx) -> c64 {
-        c64 w = tw[idx];
-        return sign > 0 ? conj(w) : w;
-    };
-    int L = 1;
-    for (int st = 0; st < pl.nst; st++) {
-        const int r = pl.radix[st];
-        const int Lr = L * r, nb = n / r, tstep = n / Lr, rstep = n / r;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-            const int j = b % L, k = b / L;
-            c64 u[kFftMaxRadix];
-#pragma unroll
-            for (int q = 0; q < kFftMaxRadix; q++)
-                if (q < r) {
-                    c64 v = b0[j + L * k + rstep * q];
-                    u[q] = (q == 0 || j == 0) ? v : v * W(j * q * tstep);
-                }
-            if (r == 2) {
-                b1[j + L * 0 + Lr * k] = u[0] + u[1];
-                b1[j + L * 1 + Lr * k] = u[0] - u[1];
-            } else if (r == 4) {
-                // w_4 = -i (forward) or +i (backward)
-                const c64 t0 = u[0] + u[2], t1 = u[0] - u[2], t2 = u[1] + u[3], d = u[1] - u[3];
-                const c64 t3 = sign > 0 ? mk(-d.y, d.x) : mk(d.y, -d.x);
-                b1[j + L * 0 + Lr * k] = t0 + t2;
-                b1[j + L * 1 + Lr * k] = t1 + t3;
-                b1[j + L * 2 + Lr * k] = t0 - t2;
-                b1[j + L * 3 + Lr * k] = t1 - t3;
-            } else {
-#pragma unroll
-                for (int pq = 0; pq < kFftMaxRadix; pq++)
-                    if (pq < r) {
-                        c64 a = u[0];
-#pragma unroll
-                        for (int q = 1; q < kFftMaxRadix; q++)
-                            if (q < r) a = cmad(a, u[q], W(((pq * q) % r) * rstep));
-                        b1[j + L * pq + Lr * k] = a;
-                    }
-            }
-        }
-        __syncthreads();
-        c64 *t = b0;
-        b0 = b1;
-        b1 = t;
-        L = Lr;
+    const c64 *res = fft_stages<1>(fbuf, fbuf + n, pl, tw, sign);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[base + i] = res[i];
+}
+
+// columns, in place: block (group g, plane) transforms columns C*g .. C*g+C-1
+template <int C>
+__global__ __launch_bounds__(256) void k_fft_cols(c64 *__restrict__ data, int Nx, int ngroups, int64_t nblocks,
+                                                  FftPlan pl, const c64 *__restrict__ tw, int sign) {
+    extern __shared__ c64 fbuf[];
+    const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
+    if (blk >= nblocks) return;
+    const int n = pl.n;
+    const int g = (int)(blk % ngroups);
+    c64 *plane = data + (blk / ngroups) * (int64_t)Nx * n;
+    const int x0 = g * C;
+    for (int t = threadIdx.x; t < n * C; t += blockDim.x) {
+        const int i = t % C, y = t / C;
+        fbuf[t] = (x0 + i < Nx) ? plane[(int64_t)y * Nx + x0 + i] : mk(0.0, 0.0);
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) data[base + (int64_t)i * elem_stride] = b0[i];
+    __syncthreads();
+    const c64 *res = fft_stages<C>(fbuf, fbuf + n * C, pl, tw, sign);
+    for (int t = threadIdx.x; t < n * C; t += blockDim.x) {
+        const int i = t % C, y = t / C;
+        if (x0 + i < Nx) plane[(int64_t)y * Nx + x0 + i] = res[t];
+    }
+}
+
+void launch_fft_rows(const c64 *src, c64 *dst, int Nx, int Ny, int planes, int sign, const c64 *tw_x,
+                     const FftPlan &px, hipStream_t st) {
+    hipLaunchKernelGGL(k_fft_lines, dim3(Ny, planes), dim3(256), 2 * Nx * sizeof(c64), st, src, dst,
+                       (int64_t)Nx * Ny, px, tw_x, sign);
+    NMK_LAUNCH_CHECK();
+}
+
+void launch_fft_cols(c64 *data, int Nx, int Ny, int planes, int sign, const c64 *tw_y, const FftPlan &py,
+                     hipStream_t st) {
+    const int C = fft_cols_per_block(Ny);
+    const int ngroups = (Nx + C - 1) / C;
+    const int64_t nblocks = (int64_t)ngroups * planes;
+    const unsigned grid = (unsigned)((nblocks + 7) / 8 * 8);
+    const size_t lds = 2 * (size_t)Ny * C * sizeof(c64);
+    switch (C) {
+    case 8: hipLaunchKernelGGL(k_fft_cols<8>, dim3(grid), dim3(256), lds, st, data, Nx, ngroups, nblocks, py, tw_y, sign); break;
+    case 4: hipLaunchKernelGGL(k_fft_cols<4>, dim3(grid), dim3(256), lds, st, data, Nx, ngroups, nblocks, py, tw_y, sign); break;
+    case 2: hipLaunchKernelGGL(k_fft_cols<2>, dim3(grid), dim3(256), lds, st, data, Nx, ngroups, nblocks, py, tw_y, sign); break;
+    default: hipLaunchKernelGGL(k_fft_cols<1>, dim3(grid), dim3(256), lds, st, data, Nx, ngroups, nblocks, py, tw_y, sign); break;
+    }
+    NMK_LAUNCH_CHECK();
 }
 
 void launch_fft2(c64 *data, int Nx, int Ny, int planes, int sign, const c64 *tw_x, const c64 *tw_y,
                  const FftPlan &px, const FftPlan &py, hipStream_t st) {
-    const int64_t plane = (int64_t)Nx * Ny;
-    if (Nx > 1) {
-        hipLaunchKernelGGL(k_fft_lines, dim3(Ny, planes), dim3(256), 2 * Nx * sizeof(c64), st, data, (int64_t)Nx,
-                           (int64_t)1, plane, px, tw_x, sign);
-        NMK_LAUNCH_CHECK();
-    }
-    if (Ny > 1) {
-        hipLaunchKernelGGL(k_fft_lines, dim3(Nx, planes), dim3(256), 2 * Ny * sizeof(c64), st, data, (int64_t)1,
-                           (int64_t)Nx, plane, py, tw_y, sign);
-        NMK_LAUNCH_CHECK();
-    }
+    if (Nx > 1) launch_fft_rows(data, data, Nx, Ny, planes, sign, tw_x, px, st);
+    if (Ny > 1) launch_fft_cols(data, Nx, Ny, planes, sign, tw_y, py, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,422 @@
+"""Microscope imaging of the bpm output on the device: the ``OpticalFields``
+and ``FieldViewer`` classes of Nemaktis with their compute moved to HIP.
+
+Mirrors ``HighLevelPythonInterface/nemaktis/light_propagator.py:414-846``
+(``OpticalFields``: objective filter, ``update_NA_objective``,
+``focus_fields``, ``focused_vals``) and
+``HighLevelPythonInterface/nemaktis/field_viewer.py:16-517``
+(``FieldViewer``: optical elements, ``update_image``, ``get_image``,
+``get_spectrum``).  The refocus (2-D FFT, objective mask and defocus phase,
+inverse FFT) and the micrograph synthesis (Jones matrices, q-integration,
+colour conversion) run through the C ABI of ``include/nemaktis_view.h``; the
+fields stay in HBM between calls and only the image (or what the caller asks
+for) comes back to the host.  There is no CPU fallback.
+
+Differences from the reference, by construction:
+  * VTI import/export (``OpticalFields(vti_file=...)``, ``save_to_vti``) and
+    the GUI (``FieldViewer.plot``) are not provided.
+  * ``cmf`` must be given for colour images: the default table comes from
+    dtmm's data files (``dtmm.color.load_tcmf``), which are not shipped here.
+    Without it the spectrum is still computed and ``get_image`` raises.
+  * ``FieldViewer.NA_objective`` returns the objective NA (the reference
+    declares its setter with ``@z_focus.setter``, so its getter returns the
+    focus altitude).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class ViewerError(RuntimeError):
+    pass
+
+
+def _check(rc, err):
+    if rc:
+        raise ViewerError(err.value.decode())
+
+
+def _geometry(Nx, Ny, Lx, Ly, wavelengths, n_out, wavevectors, max_NA_objective, max_NA_condenser, Nr,
+              koehler_1D, zfoc_NA_corr, device):
+    wl = np.ascontiguousarray(np.atleast_1d(np.asarray(wavelengths, dtype=np.float64)))
+    nout = np.ascontiguousarray(np.broadcast_to(np.asarray(n_out, dtype=np.float64), wl.shape))
+    g = _lib.ViewGeometry()
+    g.Nx, g.Ny, g.Lx, g.Ly = int(Nx), int(Ny), float(Lx), float(Ly)
+    g.n_wavelengths = wl.size
+    g.wavelengths = _lib.dptr(wl)
+    g.n_out = _lib.dptr(nout)
+    keep = [wl, nout]
+    if wavevectors is not None:
+        q = np.ascontiguousarray(np.asarray(wavevectors, dtype=np.float64).reshape(-1, 2))
+        g.n_wavevectors = q.shape[0]
+        g.wavevectors = _lib.dptr(q)
+        keep.append(q)
+    g.max_NA_objective = float(max_NA_objective)
+    g.max_NA_condenser = float(max_NA_condenser)
+    g.N_radial_wavevectors = int(Nr)
+    g.koehler_1D = 1 if koehler_1D else 0
+    g.zfoc_NA_corr = float(zfoc_NA_corr)
+    g.device = int(device)
+    return g, keep
+
+
+def condenser_wavevectors(mesh_dimensions, max_NA_condenser, N_radial_wavevectors, koehler_1D=False):
+    """Wavevectors (qx, qy) of the condenser rings (light_propagator.py:578-599)."""
+    Nr = N_radial_wavevectors
+    if mesh_dimensions[0] == 1 and koehler_1D:
+        q = np.zeros((2 * Nr - 1, 2))
+        for ir in range(1, Nr):
+            q[Nr - 1 + ir, 1] = ir * max_NA_condenser / (Nr - 1)
+            q[Nr - 1 - ir, 1] = -ir * max_NA_condenser / (Nr - 1)
+    elif mesh_dimensions[1] == 1 and koehler_1D:
+        q = np.zeros((2 * Nr - 1, 2))
+        for ir in range(1, Nr):
+            q[Nr - 1 + ir, 0] = ir * max_NA_condenser / (Nr - 1)
+            q[Nr - 1 - ir, 0] = -ir * max_NA_condenser / (Nr - 1)
+    else:
+        q = np.zeros((1 + 3 * Nr * (Nr - 1), 2))
+        for ir in range(1, Nr):
+            beta = ir * max_NA_condenser / (Nr - 1)
+            for iphi in range(6 * ir):
+                phi = iphi * np.pi / (3 * ir)
+                q[1 + 3 * ir * (ir - 1) + iphi] = beta * np.cos(phi), beta * np.sin(phi)
+    return q
+
+
+class OpticalFields:
+    """Transverse output fields of shape (N_wavelengths, N_wavevectors, 4, Ny, Nx) and their refocus
+    through the microscope objective (reference: light_propagator.py:414-846).
+
+    ``OpticalFields(wavelengths=..., max_NA_objective=..., mesh_lengths=(Lx, Ly), mesh_dimensions=(Nx, Ny),
+    max_NA_condenser=0, N_radial_wavevectors=1, koehler_1D=False, zfoc_NA_corr=0, n_out=1, device=-1)``
+    """
+
+    def __init__(self, **kwargs):
+        if "vti_file" in kwargs:
+            raise NotImplementedError("VTI import is not provided by nemaktis_amd")
+        if len(kwargs["mesh_dimensions"]) != 2:
+            raise Exception("mesh_dimensions should be an array-like object of size 2")
+        if len(kwargs["mesh_lengths"]) != 2:
+            raise Exception("mesh_lengths should be an array-like object of size 2")
+        self._n_out = kwargs.get("n_out", 1)
+        self._zfoc_NA_corr = kwargs.get("zfoc_NA_corr", 0)
+        self._wavelengths = np.array(kwargs["wavelengths"], dtype=np.float64).reshape(-1)
+        self._max_NA_objective = kwargs["max_NA_objective"]
+        self._max_NA_condenser = kwargs.get("max_NA_condenser", 0)
+        self._N_radial_wavevectors = kwargs.get("N_radial_wavevectors", 1)
+        self._koehler_1D = kwargs.get("koehler_1D", False)
+        self._NA = self._max_NA_objective
+        self._device = kwargs.get("device", -1)
+        dims = np.array(kwargs["mesh_dimensions"])
+        lengths = np.array(kwargs["mesh_lengths"], dtype=np.float64)
+        self._wavevectors = condenser_wavevectors(dims, self._max_NA_condenser, self._N_radial_wavevectors,
+                                                  self._koehler_1D)
+        (self._Nx, self._Ny) = (int(dims[0]), int(dims[1]))
+        (self._Lx, self._Ly) = (float(lengths[0]), float(lengths[1]))
+        (self._dx, self._dy) = tuple(lengths / np.maximum(np.ones(2), dims - 1))
+        shape = (len(self._wavelengths), len(self._wavevectors), 4, self._Ny, self._Nx)
+        self._vals = np.zeros(shape, dtype=np.complex128)
+        self._init_common()
+
+    def _init_common(self):
+        if self._N_radial_wavevectors > 1 and self._max_NA_condenser > 0:
+            self._delta_qr = self._max_NA_condenser / (self._N_radial_wavevectors - 1)
+        else:
+            self._delta_qr = 1
+        self._z = 0
+        self._lib = _lib.load()
+        self._h = None
+        self._ctx = None
+        self._upload = True
+        self._focused = None
+
+    @classmethod
+    def from_context(cls, ctx):
+        """Fields of the last ``propagate()`` of a ``bpm_backend.Context``, copied on the device: the settings
+        give the mesh, wavelengths, n_out, wavevectors, objective NA and focus correction as
+        ``LightPropagator._bpm_propagation`` passes them (light_propagator.py:196-219)."""
+        self = cls.__new__(cls)
+        lib = _lib.load()
+        err = _lib.err_buf()
+        h = lib.nmk_view_from_ctx(ctx.h, None, err, len(err))
+        if not h:
+            raise ViewerError(err.value.decode())
+        import json
+        s = json.loads(ctx.json_str)
+        ic = s["Physics settings"]["Initial conditions"]
+        co = s["Physics settings"]["Coefficients"]
+        scr = s["Postprocessor settings"]["Screen output"]
+        dims, sp = ic["Mesh dimensions"], ic["Mesh spacings"]
+        self._wavelengths = np.array(co["Wavelengths"], dtype=np.float64).reshape(-1)
+        self._wavevectors = np.array(co["Wavevectors"], dtype=np.float64).reshape(-1, 2)
+        self._Nx, self._Ny = int(dims[0]), int(dims[1])
+        self._dx, self._dy = float(sp[0]), float(sp[1])
+        self._Lx, self._Ly = self._dx * max(1, self._Nx - 1), self._dy * max(1, self._Ny - 1)
+        self._max_NA_objective = float(scr["Numerical aperture"])
+        self._NA = self._max_NA_objective
+        Nq = len(self._wavevectors)
+        self._N_radial_wavevectors = int(np.round((1 + np.sqrt((4 * Nq - 1) / 3)) / 2))
+        self._max_NA_condenser = float(np.sqrt(np.sum(self._wavevectors ** 2, axis=1))[-1])
+        self._koehler_1D = False
+        self._n_out = None
+        self._zfoc_NA_corr = None
+        self._device = -1
+        self._vals = None
+        self._init_common()
+        self._h = h
+        self._ctx = ctx
+        self._upload = False
+        return self
+
+    # ---- device handle -------------------------------------------------------------------------------------
+    def _handle(self):
+        if self._h is None:
+            g, keep = _geometry(self._Nx, self._Ny, self._Lx, self._Ly, self._wavelengths, self._n_out,
+                                self._wavevectors, self._max_NA_objective, self._max_NA_condenser,
+                                self._N_radial_wavevectors, self._koehler_1D, self._zfoc_NA_corr, self._device)
+            err = _lib.err_buf()
+            self._h = self._lib.nmk_view_create(ctypes.byref(g), None, 0, err, len(err))
+            del keep
+            if not self._h:
+                raise ViewerError(err.value.decode())
+            self._upload = True
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nmk_view_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- reference interface -------------------------------------------------------------------------------
+    def copy(self):
+        """Returns a hard copy of this OpticalFields object"""
+        new_fields = OpticalFields(
+            wavelengths=self._wavelengths, max_NA_objective=self._max_NA_objective,
+            max_NA_condenser=self._max_NA_condenser, N_radial_wavevectors=self._N_radial_wavevectors,
+            koehler_1D=self._koehler_1D, zfoc_NA_corr=self.zfoc_NA_corr, n_out=self.n_out,
+            mesh_dimensions=(self._Nx, self._Ny), mesh_lengths=(self._Lx, self._Ly), device=self._device)
+        new_fields.vals = self.vals
+        return new_fields
+
+    @property
+    def n_out(self):
+        if self._n_out is None:
+            raise ViewerError("n_out is held on the device for fields taken from a context")
+        return self._n_out
+
+    @property
+    def zfoc_NA_corr(self):
+        if self._zfoc_NA_corr is None:
+            raise ViewerError("zfoc_NA_corr is held on the device for fields taken from a context")
+        return self._zfoc_NA_corr
+
+    @property
+    def focused_vals(self):
+        """Fields after focalisation by the microscope objective, shape (N_wavelengths,N_wavevectors,4,Ny,Nx)."""
+        if self._focused is None:
+            out = np.empty(self._shape(), dtype=np.complex128)
+            err = _lib.err_buf()
+            _check(self._lib.nmk_view_get_focused(self._handle(), _lib.dptr(out.reshape(-1).view(np.float64)),
+                                                  out.size, err, len(err)), err)
+            self._focused = out
+        return self._focused
+
+    def _shape(self):
+        return (len(self._wavelengths), len(self._wavevectors), 4, self._Ny, self._Nx)
+
+    @property
+    def vals(self):
+        """Fields of shape (N_wavelengths,N_wavevectors,4,Ny,Nx); components Ex, Ey for an input polarised
+        along x, then Ex, Ey for an input polarised along y."""
+        if self._vals is None:
+            self._vals = np.empty(self._shape(), dtype=np.complex128)
+            self._ctx.download(self._vals)
+        # the caller may write into the array: upload it again before the next focus
+        self._upload = True
+        return self._vals
+
+    @vals.setter
+    def vals(self, fields_ndarray):
+        if self._shape() == fields_ndarray.shape:
+            if self._vals is None:
+                self._vals = np.empty(self._shape(), dtype=np.complex128)
+            self._vals[:] = fields_ndarray
+            self._upload = True
+        else:
+            raise Exception("Wrong shape for the optical field ndarray")
+
+    @property
+    def z_focus(self):
+        return self._z
+
+    def update_NA_objective(self, new_NA):
+        self._NA = max(0, min(self._max_NA_objective, new_NA))
+
+    def focus_fields(self, z_focus=None):
+        """Propagate the optical fields through the objective lens to the screen conjugate to the focusing
+        plane (whose altitude inside the sample is set with the parameter z_focus)."""
+        if z_focus is not None:
+            self._z = z_focus
+        h = self._handle()
+        err = _lib.err_buf()
+        if self._upload:
+            v = np.ascontiguousarray(self._vals, dtype=np.complex128)
+            _check(self._lib.nmk_view_set_fields(h, _lib.dptr(v.reshape(-1).view(np.float64)), v.size, err,
+                                                 len(err)), err)
+            self._upload = False
+        _check(self._lib.nmk_view_focus(h, float(self._z), float(self._NA), err, len(err)), err)
+        self._focused = None
+
+    def timings(self):
+        """Device ms of the last focus / render: filter, rows forward, columns, rows backward, spectrum, colour."""
+        ms = (ctypes.c_double * 6)()
+        self._lib.nmk_view_timings(self._handle(), ms, 6)
+        return dict(zip(["filter", "rows_fwd", "columns", "rows_bwd", "spectrum", "colour"], list(ms)))
+
+    def get_pos(self, ix, iy):
+        return np.array([(ix - (self._Nx - 1) / 2) * self._dx, (iy - (self._Ny - 1) / 2) * self._dy])
+
+    def get_wavelengths(self):
+        return self._wavelengths
+
+    def get_wavevectors(self):
+        return self._wavevectors
+
+    def get_qr_index(self, NA_condenser):
+        return int(np.ceil(NA_condenser / self._delta_qr))
+
+    def get_delta_qr(self):
+        return self._delta_qr
+
+    def get_mesh_dimensions(self):
+        return (self._Nx, self._Ny)
+
+    def get_mesh_lengths(self):
+        return (self._Lx, self._Ly)
+
+    def get_mesh_spacings(self):
+        return (self._dx, self._dy)
+
+    def get_n_vertices(self):
+        return self._Nx * self._Ny
+
+    def save_to_vti(self, filename):
+        raise NotImplementedError("VTI export is not provided by nemaktis_amd")
+
+
+class FieldViewer:
+    """Recombines optical fields into micrographs as in a real microscope (reference: field_viewer.py:16-470).
+
+    ``FieldViewer(optical_fields, cmf=None, gamma=True)``; ``cmf`` is the (N_wavelengths, 3) colour matching
+    table of the wavelengths (the ``cmf`` argument of the reference)."""
+
+    polariser = True
+    analyser = True
+    upper_waveplate = "No"
+    lower_waveplate = "No"
+    polariser_angle = 0
+    analyser_angle = 90
+    upper_waveplate_angle = 0
+    lower_waveplate_angle = 0
+    angle_lock = False
+    intensity = 1
+    NA_condenser = 0
+    n_tiles_x = 1
+    n_tiles_y = 1
+    grayscale = False
+
+    def __init__(self, optical_fields, cmf=None, gamma=True):
+        if not isinstance(optical_fields, OpticalFields):
+            raise TypeError("optical_fields should be an OpticalFields object")
+        self._optical_fields = optical_fields
+        optical_fields.focus_fields(0)
+        self._NA_objective = optical_fields._max_NA_objective
+        self._gamma = gamma
+        Nl = len(optical_fields.get_wavelengths())
+        Nx, Ny = optical_fields.get_mesh_dimensions()
+        self._q_averaged_specter = np.zeros((Ny, Nx, Nl))
+        self._image = np.zeros((Ny, Nx, 3))
+        self._cmf = None if cmf is None else np.ascontiguousarray(np.asarray(cmf, dtype=np.float64).reshape(Nl, 3))
+        self._koehler_1D = optical_fields._koehler_1D
+        self._N_radial_wavevectors = optical_fields._N_radial_wavevectors
+
+    def plot(self):
+        raise NotImplementedError("the graphical interface is not provided by nemaktis_amd")
+
+    @property
+    def z_focus(self):
+        """Current vertical position of the focal plane"""
+        return self._optical_fields.z_focus
+
+    @z_focus.setter
+    def z_focus(self, new_z):
+        self._optical_fields.focus_fields(new_z)
+
+    @property
+    def NA_objective(self):
+        """Numerical aperture of the microscope's objective"""
+        return self._NA_objective
+
+    @NA_objective.setter
+    def NA_objective(self, new_NA):
+        self._NA_objective = new_NA
+        self._optical_fields.update_NA_objective(new_NA)
+        self._optical_fields.focus_fields()
+
+    def _optics(self):
+        o = _lib.ViewOptics()
+        lib = _lib.load()
+        lib.nmk_view_default_optics(ctypes.byref(o))
+        o.polariser = 1 if self.polariser else 0
+        o.analyser = 1 if self.analyser else 0
+        o.polariser_angle = float(self.polariser_angle)
+        o.analyser_angle = float(self.analyser_angle)
+        if self.lower_waveplate not in _lib.WAVEPLATES or self.upper_waveplate not in _lib.WAVEPLATES:
+            raise ValueError("waveplates are 'No', 'Quarter-wave', 'Half-wave' or 'Tint-sensitive'")
+        o.lower_waveplate = _lib.WAVEPLATES[self.lower_waveplate]
+        o.upper_waveplate = _lib.WAVEPLATES[self.upper_waveplate]
+        o.lower_waveplate_angle = float(self.lower_waveplate_angle)
+        o.upper_waveplate_angle = float(self.upper_waveplate_angle)
+        o.intensity = float(self.intensity)
+        o.NA_condenser = float(self.NA_condenser)
+        o.grayscale = 1 if self.grayscale else 0
+        o.gamma = 1 if self._gamma else 0
+        return o
+
+    def render(self):
+        """Computes spectrum (and image when a colour table is set) on the device without downloading."""
+        of = self._optical_fields
+        o = self._optics()
+        err = _lib.err_buf()
+        cmf = self._cmf
+        _check(of._lib.nmk_view_render(of._handle(), ctypes.byref(o), _lib.dptr(cmf),
+                                       0 if cmf is None else cmf.size, err, len(err)), err)
+
+    def update_image(self):
+        """Recompute the micrograph from the optical fields data"""
+        self.render()
+        of = self._optical_fields
+        err = _lib.err_buf()
+        _check(of._lib.nmk_view_get_spectrum(of._handle(), _lib.dptr(self._q_averaged_specter),
+                                             self._q_averaged_specter.size, err, len(err)), err)
+        if self._cmf is not None:
+            _check(of._lib.nmk_view_get_image(of._handle(), _lib.dptr(self._image), self._image.size, err,
+                                              len(err)), err)
+
+    def get_image(self):
+        """Returns the current micrograph as a numpy array of shape (Ny,Nx,3)."""
+        if self._cmf is None:
+            raise ViewerError("a colour matching table (cmf) is required for images")
+        return np.flip(np.tile(self._image, (self.n_tiles_y, self.n_tiles_x, 1)), axis=0)
+
+    def get_spectrum(self):
+        """Returns the q-averaged spectrum as a numpy array of shape (Ny,Nx,Nl)."""
+        return np.flip(np.tile(self._q_averaged_specter, (self.n_tiles_y, self.n_tiles_x, 1)), axis=0)
