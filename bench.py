@@ -93,6 +93,17 @@ def probe(kind, steps):
         torch.cuda.synchronize()
         log("copy probe: %d copies of %d bytes" % (steps, 4 * n))
         return
+    if kind == "imaging":
+        of, cmf = imaging_workload()
+        from nemaktis_amd import viewer
+        fv = viewer.FieldViewer(of, cmf=cmf)
+        fv.NA_condenser = IMAGING["NA_condenser"]
+        for _ in range(steps):
+            of.focus_fields(IMAGING["z_focus"])
+            fv.render()
+        log("imaging probe: %d refocus + render steps, last %s" % (steps, of.timings()))
+        of.close()
+        return
     from nemaktis_amd import fields, autotune, bpm_backend
     s, lc, _, _ = fields.workload("random512")
     tuning, _ = autotune.tune(s, lc, precision=32, wl_batch=0, use_cache=True, log=log)
@@ -119,6 +130,98 @@ def serve_leg(n_jobs, rate_hz, seed, max_active):
                          "1024^2x200, 1-21 wavelengths x 1-7 wavevectors x 2 polarisations, complex64"
                          % (n_jobs, rate_hz, seed), "max_active": max_active})
     return res
+
+
+IMAGING = {"Nx": 256, "Ny": 256, "L": 31.875, "wavelengths": list(np.linspace(0.40, 0.80, 21)), "Nr": 2,
+           "NA_condenser": 0.2, "NA_objective": 0.4, "z_focus": 1.0}
+
+
+def imaging_bytes(kernel, Nl, Nq, N, nq_used):
+    """Unique HBM bytes of one launch of an imaging kernel (complex128 = 16 B):
+      filter   writes one plane per (wavelength, wavevector)
+      rows     read and write the 4 field planes of every (wavelength, wavevector)
+      columns  the same plus one read of the filter plane
+      spectrum reads the 4 planes of every used (wavelength, wavevector), writes Nl planes of float64
+      colour   reads Nl float64 planes, writes the spectrum (Nl) and image (3) float64 planes"""
+    return {"filter": 16 * Nl * Nq * N, "rows_fwd": 128 * Nl * Nq * N, "rows_bwd": 128 * Nl * Nq * N,
+            "columns": 144 * Nl * Nq * N, "spectrum": 64 * Nl * nq_used * N + 8 * Nl * N,
+            "colour": 16 * Nl * N + 24 * N}[kernel]
+
+
+def imaging_workload(seed=7):
+    from nemaktis_amd import viewer
+    w = IMAGING
+    of = viewer.OpticalFields(wavelengths=w["wavelengths"], max_NA_objective=w["NA_objective"],
+                              max_NA_condenser=w["NA_condenser"], N_radial_wavevectors=w["Nr"],
+                              mesh_lengths=(w["L"], w["L"]), mesh_dimensions=(w["Nx"], w["Ny"]))
+    rng = np.random.default_rng(seed)
+    v = of.vals
+    v.real[...] = rng.standard_normal(v.shape)
+    v.imag[...] = rng.standard_normal(v.shape)
+    Nl = len(w["wavelengths"])
+    cmf = np.stack([np.exp(-((np.array(w["wavelengths"]) - c) / 0.05) ** 2) for c in (0.6, 0.55, 0.45)], -1)
+    return of, cmf / cmf[:, 1].sum() * 2
+
+
+def imaging_leg(steps, warmup, with_cpu):
+    """Objective refocus + micrograph synthesis of a 21-wavelength x 7-wavevector 256^2 output on the device
+    (fields resident in HBM; only the device work is timed)."""
+    from nemaktis_amd import viewer
+    w = IMAGING
+    of, cmf = imaging_workload()
+    fv = viewer.FieldViewer(of, cmf=cmf)
+    fv.NA_condenser = w["NA_condenser"]
+    fv.lower_waveplate = "Tint-sensitive"
+    fv.lower_waveplate_angle = 45
+    for _ in range(warmup):
+        of.focus_fields(w["z_focus"])
+        fv.render()
+    per = {k: 0.0 for k in ("filter", "rows_fwd", "columns", "rows_bwd", "spectrum", "colour")}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        of.focus_fields(w["z_focus"])
+        fv.render()
+        for k, v in of.timings().items():
+            per[k] += v
+    wall = time.perf_counter() - t0
+    Nl, Nq, N = len(w["wavelengths"]), 7, w["Nx"] * w["Ny"]
+    avg = {k: v / steps for k, v in per.items()}
+    dom = max(avg, key=avg.get)
+    algo = imaging_bytes(dom, Nl, Nq, N, Nq)
+    achieved = algo / (avg[dom] * 1e-3) / 1e9
+    out = {"metric": "micrographs/s (refocus + 21-wavelength colour micrograph)", "value": steps / wall,
+           "unit": "micrographs/s", "ms_per_step": 1e3 * wall / steps,
+           "device_ms": {k: round(v, 4) for k, v in avg.items()},
+           "workload": "%d wavelengths x %d wavevectors x 4 planes of %dx%d complex128 (the droplet256 output "
+                       "shape), z_focus %.1f, NA %.1f / condenser %.1f, tint plate, crossed polarisers"
+                       % (Nl, Nq, w["Nx"], w["Ny"], w["z_focus"], w["NA_objective"], w["NA_condenser"]),
+           "data": "synthetic (seeded normal fields)",
+           "roofline": {"bound": "hbm", "kernel": dom, "avg_launch_ms": avg[dom], "achieved": round(achieved, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "algorithmic_bytes_per_launch": algo,
+                        "all": {k: round(imaging_bytes(k, Nl, Nq, N, Nq) / (avg[k] * 1e-3) / 1e9, 1)
+                                for k in avg if avg[k] > 0}}}
+    of.close()
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import viewer_oracle as vo
+        nl = 3
+        vals = of.vals[:nl]
+        d = w["L"] / (w["Nx"] - 1)
+        t0 = time.perf_counter()
+        foc = vo.focus(vals, d, d, w["wavelengths"][:nl], 1.0, of.get_wavevectors(), w["NA_objective"],
+                       w["z_focus"])
+        spec = vo.spectrum(foc, w["wavelengths"][:nl], dict(lower_waveplate="Tint-sensitive",
+                                                             lower_waveplate_angle=45,
+                                                             NA_condenser=w["NA_condenser"]),
+                           w["Nr"], w["NA_condenser"])
+        vo.colour(spec, cmf[:nl])
+        dt = (time.perf_counter() - t0) * Nl / nl
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "micrographs/s", "cores": 1, "kind": "port",
+                               "sample": "numpy FP64 oracle on %d of the %d wavelengths (all 7 wavevectors), "
+                                         "scaled by %d/%d" % (nl, Nl, Nl, nl)}
+        out["vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
 
 
 def rel_errors(a, b):
@@ -185,7 +288,8 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=8)
     ap.add_argument("--serve-rate", type=float, default=1.0, help="mean job arrivals per second")
     ap.add_argument("--serve-active", type=int, default=4, help="jobs on the device at once")
-    ap.add_argument("--probe", choices=["copy", "march"], help="short run for a hardware-counter pass")
+    ap.add_argument("--no-imaging", action="store_true", help="skip the refocus + micrograph leg")
+    ap.add_argument("--probe", choices=["copy", "march", "imaging"], help="short run for a hardware-counter pass")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -263,6 +367,11 @@ def main():
                "sample": "FP64 oracle, 512x512x%d slice of the same director (%d slabs, 2 polarisations), "
                          "%.1f s wall" % (args.cpu_nz, args.cpu_nz - 1, dt)}
 
+    imaging = None
+    if not args.no_imaging:
+        log("refocus + micrograph leg ...")
+        imaging = imaging_leg(args.steps, args.warmup, not args.no_cpu)
+
     serve = None
     if not args.no_serve:
         log("serving loop trace ...")
@@ -281,6 +390,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serve,
+        "imaging": imaging,
     }
     if cpu:
         line["vs_cpu"] = value / cpu["value"]

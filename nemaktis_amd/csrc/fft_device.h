@@ -15,8 +15,10 @@
 
 namespace nmk {
 
-// one stage of radix R: butterflies b (and lines i) of this thread
-template <int C, int R>
+// one stage of radix R over the C lines of the block.  LM selects the LDS
+// layout: line-major (element k of line i at i*n + k, rows) or interleaved
+// (k*C + i, column tiles); work items run along the contiguous direction.
+template <int C, bool LM, int R>
 __device__ __forceinline__ void fft_stage(const c64 *__restrict__ b0, c64 *__restrict__ b1, int n, int L,
                                           const c64 *__restrict__ tw, int sign) {
     const int Lr = L * R, nb = n / R, tstep = n / Lr, rstep = n / R;
@@ -24,21 +26,24 @@ __device__ __forceinline__ void fft_stage(const c64 *__restrict__ b0, c64 *__res
         const c64 w = tw[idx];
         return sign > 0 ? conj(w) : w;
     };
-    // R-th roots of unity of the butterfly (uniform across the block)
+    // R-th roots of unity of the butterfly
     c64 wr[R];
 #pragma unroll
     for (int m = 0; m < R; m++) wr[m] = (R == 2 || R == 4) ? mk(1.0, 0.0) : W(m * rstep);
+    const int step = LM ? 1 : C;          // LDS distance of consecutive elements of a line
     for (int t = threadIdx.x; t < nb * C; t += blockDim.x) {
-        const int i = t % C, b = t / C;
+        const int i = LM ? t / nb : t % C;
+        const int b = LM ? t - i * nb : t / C;
+        const int base = LM ? i * n : i;
         const int j = b % L, k = b / L;
         c64 u[R];
 #pragma unroll
         for (int q = 0; q < R; q++) {
-            const c64 v = b0[(j + L * k + rstep * q) * C + i];
+            const c64 v = b0[base + (j + L * k + rstep * q) * step];
             u[q] = (q == 0 || j == 0) ? v : v * W(j * q * tstep);
         }
-        c64 *o = b1 + (j + Lr * k) * C + i;
-        const int os = L * C;
+        c64 *o = b1 + base + (j + Lr * k) * step;
+        const int os = L * step;
         if constexpr (R == 2) {
             o[0] = u[0] + u[1];
             o[os] = u[0] - u[1];
@@ -63,22 +68,26 @@ __device__ __forceinline__ void fft_stage(const c64 *__restrict__ b0, c64 *__res
 }
 
 // runs every stage of plan pl on the C lines in b0 (all threads of the block
-// take part; the caller has synchronised after filling b0) and returns the
-// buffer that holds the result
-template <int C>
+// take part; the caller has synchronised after filling b0 and the twiddle
+// table tw, normally an LDS copy) and returns the buffer that holds the result
+// (ODD: the plan may hold radices 3 to 13; without it only 2 and 4 are
+// compiled, which keeps the power-of-two kernels small)
+template <int C, bool LM, bool ODD>
 __device__ c64 *fft_stages(c64 *b0, c64 *b1, const FftPlan &pl, const c64 *__restrict__ tw, int sign) {
     const int n = pl.n;
     int L = 1;
     for (int st = 0; st < pl.nst; st++) {
         const int r = (int)((pl.radices >> (4 * st)) & 15);
-        switch (r) {
-        case 4: fft_stage<C, 4>(b0, b1, n, L, tw, sign); break;
-        case 2: fft_stage<C, 2>(b0, b1, n, L, tw, sign); break;
-        case 3: fft_stage<C, 3>(b0, b1, n, L, tw, sign); break;
-        case 5: fft_stage<C, 5>(b0, b1, n, L, tw, sign); break;
-        case 7: fft_stage<C, 7>(b0, b1, n, L, tw, sign); break;
-        case 11: fft_stage<C, 11>(b0, b1, n, L, tw, sign); break;
-        default: fft_stage<C, 13>(b0, b1, n, L, tw, sign); break;
+        if (r == 4) fft_stage<C, LM, 4>(b0, b1, n, L, tw, sign);
+        else if (r == 2) fft_stage<C, LM, 2>(b0, b1, n, L, tw, sign);
+        else if constexpr (ODD) {
+            switch (r) {
+            case 3: fft_stage<C, LM, 3>(b0, b1, n, L, tw, sign); break;
+            case 5: fft_stage<C, LM, 5>(b0, b1, n, L, tw, sign); break;
+            case 7: fft_stage<C, LM, 7>(b0, b1, n, L, tw, sign); break;
+            case 11: fft_stage<C, LM, 11>(b0, b1, n, L, tw, sign); break;
+            default: fft_stage<C, LM, 13>(b0, b1, n, L, tw, sign); break;
+            }
         }
         __syncthreads();
         c64 *tmp = b0;
@@ -88,5 +97,24 @@ __device__ c64 *fft_stages(c64 *b0, c64 *b1, const FftPlan &pl, const c64 *__res
     }
     return b0;
 }
+
+// true when every stage of the plan has radix 2 or 4
+__host__ __device__ inline bool fft_pow2(const FftPlan &pl) {
+    for (int st = 0; st < pl.nst; st++) {
+        const int r = (int)((pl.radices >> (4 * st)) & 15);
+        if (r != 2 && r != 4) return false;
+    }
+    return true;
+}
+
+// lines per block: at least 1024 values per block of 256 threads (one
+// radix-4 butterfly per thread and stage)
+__host__ __device__ constexpr int fft_lines_per_block(int n, int cap) {
+    int c = 1;
+    while (2 * c <= cap && 2 * c * n <= 1024) c *= 2;
+    return c;
+}
+// LDS bytes of a block: two line buffers and the twiddle table
+__host__ __device__ constexpr size_t fft_lds_bytes(int n, int C) { return (size_t)(2 * C + 1) * n * 16; }
 
 }  // namespace nmk

@@ -54,6 +54,8 @@ void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, cons
 LinePlan make_line_plan(int M, int chunks);
 // LDS bytes one block may hold on the current device
 size_t max_block_lds();
+// raises the dynamic LDS limit of kernel f to lds bytes when above 64 KiB
+void allow_kernel_lds(const void *f, size_t lds);
 template <class T>
 size_t xsolve_lds_bytes(const LinePlan &lp, int R);
 template <class T>

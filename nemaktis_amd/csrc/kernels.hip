@@ -1570,18 +1570,21 @@ size_t max_block_lds() {
     return (size_t)v;
 }
 
-template <class K>
-static void allow_lds(K kernel, size_t lds) {
+void allow_kernel_lds(const void *f, size_t lds) {
     if (lds <= 64 * 1024) return;
     static std::mutex mu;
     static std::map<const void *, size_t> granted;
     std::lock_guard<std::mutex> lk(mu);
-    const void *f = reinterpret_cast<const void *>(kernel);
     auto it = granted.find(f);
     if (it != granted.end() && it->second >= lds) return;
     if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-        throw std::runtime_error("line solver: the device refuses " + std::to_string(lds) + " bytes of LDS per block");
+        throw std::runtime_error("the device refuses " + std::to_string(lds) + " bytes of LDS per block");
     granted[f] = lds;
+}
+
+template <class K>
+static void allow_lds(K kernel, size_t lds) {
+    allow_kernel_lds(reinterpret_cast<const void *>(kernel), lds);
 }
 
 LinePlan make_line_plan(int M, int chunks) {
@@ -1802,28 +1805,35 @@ FftPlan make_fft_plan(int n) {
     return pl;
 }
 
-int fft_cols_per_block(int n) { return n <= 256 ? 8 : n <= 512 ? 4 : n <= 1024 ? 2 : 1; }
+int fft_cols_per_block(int n) { return fft_lines_per_block(n, 8); }
 
-// rows: src and dst may be the same buffer (the line is held in LDS)
-__global__ __launch_bounds__(256) void k_fft_lines(const c64 *src, c64 *dst, int64_t plane_stride, FftPlan pl,
-                                                   const c64 *__restrict__ tw, int sign) {
+// rows: block b transforms lines C*b .. C*b+C-1 of the whole batch (each a
+// contiguous run of n values); src and dst may be the same buffer
+template <int C, bool ODD>
+__global__ __launch_bounds__(256) void k_fft_rows(const c64 *src, c64 *dst, int64_t nlines, FftPlan pl,
+                                                  const c64 *__restrict__ tw, int sign) {
     extern __shared__ c64 fbuf[];
     const int n = pl.n;
-    const int64_t base = (int64_t)blockIdx.y * plane_stride + (int64_t)blockIdx.x * n;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) fbuf[i] = src[base + i];
+    c64 *stw = fbuf + 2 * C * n;
+    const int64_t l0 = (int64_t)blockIdx.x * C;
+    const int nl = (int)(nlines - l0 < C ? nlines - l0 : C);
+    const int64_t base = l0 * n;
+    for (int t = threadIdx.x; t < C * n; t += blockDim.x) fbuf[t] = t < nl * n ? src[base + t] : mk(0.0, 0.0);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
     __syncthreads();
-    const c64 *res = fft_stages<1>(fbuf, fbuf + n, pl, tw, sign);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[base + i] = res[i];
+    const c64 *res = fft_stages<C, true, ODD>(fbuf, fbuf + C * n, pl, stw, sign);
+    for (int t = threadIdx.x; t < nl * n; t += blockDim.x) dst[base + t] = res[t];
 }
 
 // columns, in place: block (group g, plane) transforms columns C*g .. C*g+C-1
-template <int C>
+template <int C, bool ODD>
 __global__ __launch_bounds__(256) void k_fft_cols(c64 *__restrict__ data, int Nx, int ngroups, int64_t nblocks,
                                                   FftPlan pl, const c64 *__restrict__ tw, int sign) {
     extern __shared__ c64 fbuf[];
     const int64_t blk = xcd_tile(blockIdx.x, gridDim.x);
     if (blk >= nblocks) return;
     const int n = pl.n;
+    c64 *stw = fbuf + 2 * C * n;
     const int g = (int)(blk % ngroups);
     c64 *plane = data + (blk / ngroups) * (int64_t)Nx * n;
     const int x0 = g * C;
@@ -1831,8 +1841,9 @@ __global__ __launch_bounds__(256) void k_fft_cols(c64 *__restrict__ data, int Nx
         const int i = t % C, y = t / C;
         fbuf[t] = (x0 + i < Nx) ? plane[(int64_t)y * Nx + x0 + i] : mk(0.0, 0.0);
     }
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
     __syncthreads();
-    const c64 *res = fft_stages<C>(fbuf, fbuf + n * C, pl, tw, sign);
+    const c64 *res = fft_stages<C, false, ODD>(fbuf, fbuf + n * C, pl, stw, sign);
     for (int t = threadIdx.x; t < n * C; t += blockDim.x) {
         const int i = t % C, y = t / C;
         if (x0 + i < Nx) plane[(int64_t)y * Nx + x0 + i] = res[t];
@@ -1841,8 +1852,31 @@ __global__ __launch_bounds__(256) void k_fft_cols(c64 *__restrict__ data, int Nx
 
 void launch_fft_rows(const c64 *src, c64 *dst, int Nx, int Ny, int planes, int sign, const c64 *tw_x,
                      const FftPlan &px, hipStream_t st) {
-    hipLaunchKernelGGL(k_fft_lines, dim3(Ny, planes), dim3(256), 2 * Nx * sizeof(c64), st, src, dst,
-                       (int64_t)Nx * Ny, px, tw_x, sign);
+    const int C = fft_lines_per_block(Nx, 16);
+    const int64_t nlines = (int64_t)Ny * planes;
+    const unsigned grid = (unsigned)((nlines + C - 1) / C);
+    const size_t lds = fft_lds_bytes(Nx, C);
+    switch (C) {
+#define NMK_ROWS(CC)                                                                                          \
+    case CC:                                                                                                  \
+        if (fft_pow2(px)) {                                                                                   \
+            allow_kernel_lds(reinterpret_cast<const void *>(k_fft_rows<CC, false>), lds);                    \
+            hipLaunchKernelGGL((k_fft_rows<CC, false>), dim3(grid), dim3(256), lds, st, src, dst, nlines, px, \
+                               tw_x, sign);                                                                   \
+        } else {                                                                                              \
+            allow_kernel_lds(reinterpret_cast<const void *>(k_fft_rows<CC, true>), lds);                     \
+            hipLaunchKernelGGL((k_fft_rows<CC, true>), dim3(grid), dim3(256), lds, st, src, dst, nlines, px,  \
+                               tw_x, sign);                                                                   \
+        }                                                                                                     \
+        break;
+        NMK_ROWS(16)
+        NMK_ROWS(8)
+        NMK_ROWS(4)
+        NMK_ROWS(2)
+        NMK_ROWS(1)
+#undef NMK_ROWS
+    default: throw std::runtime_error("FFT rows: bad block shape");
+    }
     NMK_LAUNCH_CHECK();
 }
 
@@ -1852,12 +1886,26 @@ void launch_fft_cols(c64 *data, int Nx, int Ny, int planes, int sign, const c64 
     const int ngroups = (Nx + C - 1) / C;
     const int64_t nblocks = (int64_t)ngroups * planes;
     const unsigned grid = (unsigned)((nblocks + 7) / 8 * 8);
-    const size_t lds = 2 * (size_t)Ny * C * sizeof(c64);
+    const size_t lds = fft_lds_bytes(Ny, C);
     switch (C) {
-    case 8: hipLaunchKernelGGL(k_fft_cols<8>, dim3(grid), dim3(256), lds, st, data, Nx, ngroups, nblocks, py, tw_y, sign); break;
-    case 4: hipLaunchKernelGGL(k_fft_cols<4>, dim3(grid), dim3(256), lds, st, data, Nx, ngroups, nblocks, py, tw_y, sign); break;
-    case 2: hipLaunchKernelGGL(k_fft_cols<2>, dim3(grid), dim3(256), lds, st, data, Nx, ngroups, nblocks, py, tw_y, sign); break;
-    default: hipLaunchKernelGGL(k_fft_cols<1>, dim3(grid), dim3(256), lds, st, data, Nx, ngroups, nblocks, py, tw_y, sign); break;
+#define NMK_COLS(CC)                                                                                          \
+    case CC:                                                                                                  \
+        if (fft_pow2(py)) {                                                                                   \
+            allow_kernel_lds(reinterpret_cast<const void *>(k_fft_cols<CC, false>), lds);                    \
+            hipLaunchKernelGGL((k_fft_cols<CC, false>), dim3(grid), dim3(256), lds, st, data, Nx, ngroups,    \
+                               nblocks, py, tw_y, sign);                                                      \
+        } else {                                                                                              \
+            allow_kernel_lds(reinterpret_cast<const void *>(k_fft_cols<CC, true>), lds);                     \
+            hipLaunchKernelGGL((k_fft_cols<CC, true>), dim3(grid), dim3(256), lds, st, data, Nx, ngroups,     \
+                               nblocks, py, tw_y, sign);                                                      \
+        }                                                                                                     \
+        break;
+        NMK_COLS(8)
+        NMK_COLS(4)
+        NMK_COLS(2)
+        NMK_COLS(1)
+#undef NMK_COLS
+    default: throw std::runtime_error("FFT columns: bad block shape");
     }
     NMK_LAUNCH_CHECK();
 }

@@ -77,7 +77,7 @@ void launch_view_filter(c64 *filt, int Nx, int Ny, double dx, double dy, const V
 // backward transform.  Work items run component-fastest and each XCD takes a
 // contiguous range, so the four components reading one filter tile share L2.
 // ---------------------------------------------------------------------------
-template <int C>
+template <int C, bool ODD>
 __global__ __launch_bounds__(256) void k_view_columns(c64 *__restrict__ data, const c64 *__restrict__ filt, int Nx,
                                                       int ngroups, int64_t nitems, FftPlan pl,
                                                       const c64 *__restrict__ tw) {
@@ -93,18 +93,20 @@ __global__ __launch_bounds__(256) void k_view_columns(c64 *__restrict__ data, co
     c64 *plane = data + (lq * 4 + comp) * N;
     const c64 *fp = filt + lq * N;
     const int x0 = g * C;
+    c64 *stw = fbuf + 2 * C * n;
     for (int t = threadIdx.x; t < n * C; t += blockDim.x) {
         const int i = t % C, y = t / C;
         fbuf[t] = (x0 + i < Nx) ? plane[(int64_t)y * Nx + x0 + i] : mk(0.0, 0.0);
     }
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
     __syncthreads();
-    c64 *a = fft_stages<C>(fbuf, fbuf + n * C, pl, tw, -1);
+    c64 *a = fft_stages<C, false, ODD>(fbuf, fbuf + n * C, pl, stw, -1);
     for (int t = threadIdx.x; t < n * C; t += blockDim.x) {
         const int i = t % C, y = t / C;
         a[t] = (x0 + i < Nx) ? a[t] * fp[(int64_t)y * Nx + x0 + i] : mk(0.0, 0.0);
     }
     __syncthreads();
-    const c64 *r = fft_stages<C>(a, a == fbuf ? fbuf + n * C : fbuf, pl, tw, +1);
+    const c64 *r = fft_stages<C, false, ODD>(a, a == fbuf ? fbuf + n * C : fbuf, pl, stw, +1);
     for (int t = threadIdx.x; t < n * C; t += blockDim.x) {
         const int i = t % C, y = t / C;
         if (x0 + i < Nx) plane[(int64_t)y * Nx + x0 + i] = r[t];
@@ -117,12 +119,26 @@ void launch_view_columns(c64 *data, const c64 *filt, int Nx, int Ny, int nlq, co
     const int ngroups = (Nx + C - 1) / C;
     const int64_t nitems = (int64_t)ngroups * nlq * 4;
     const unsigned grid = (unsigned)((nitems + 7) / 8 * 8);
-    const size_t lds = 2 * (size_t)Ny * C * sizeof(c64);
+    const size_t lds = fft_lds_bytes(Ny, C);
     switch (C) {
-    case 8: hipLaunchKernelGGL(k_view_columns<8>, dim3(grid), dim3(256), lds, st, data, filt, Nx, ngroups, nitems, py, tw_y); break;
-    case 4: hipLaunchKernelGGL(k_view_columns<4>, dim3(grid), dim3(256), lds, st, data, filt, Nx, ngroups, nitems, py, tw_y); break;
-    case 2: hipLaunchKernelGGL(k_view_columns<2>, dim3(grid), dim3(256), lds, st, data, filt, Nx, ngroups, nitems, py, tw_y); break;
-    default: hipLaunchKernelGGL(k_view_columns<1>, dim3(grid), dim3(256), lds, st, data, filt, Nx, ngroups, nitems, py, tw_y); break;
+#define NMK_VCOLS(CC)                                                                                         \
+    case CC:                                                                                                  \
+        if (fft_pow2(py)) {                                                                                   \
+            allow_kernel_lds(reinterpret_cast<const void *>(k_view_columns<CC, false>), lds);                \
+            hipLaunchKernelGGL((k_view_columns<CC, false>), dim3(grid), dim3(256), lds, st, data, filt, Nx,   \
+                               ngroups, nitems, py, tw_y);                                                    \
+        } else {                                                                                              \
+            allow_kernel_lds(reinterpret_cast<const void *>(k_view_columns<CC, true>), lds);                 \
+            hipLaunchKernelGGL((k_view_columns<CC, true>), dim3(grid), dim3(256), lds, st, data, filt, Nx,    \
+                               ngroups, nitems, py, tw_y);                                                    \
+        }                                                                                                     \
+        break;
+        NMK_VCOLS(8)
+        NMK_VCOLS(4)
+        NMK_VCOLS(2)
+        NMK_VCOLS(1)
+#undef NMK_VCOLS
+    default: throw std::runtime_error("refocus columns: bad block shape");
     }
     NMK_VIEW_LAUNCH_CHECK();
 }
