@@ -62,18 +62,19 @@ def algorithmic_bytes(fam, c, nprops, nslots, NP, Nfull, fd, nw):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_random512.txt")
+TRAFFIC_FILE_IMAGING = os.path.join(ROOT, "profiles", "hbm_traffic_imaging_droplet256.txt")
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, path=TRAFFIC_FILE):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain
     copy); (None, None) when there is none."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             for line in f:
                 parts = line.split()
                 if len(parts) == 3 and parts[0] == kernel and parts[1] == "hbm_bytes_per_launch":
-                    return float(parts[2]), os.path.relpath(TRAFFIC_FILE, ROOT)
+                    return float(parts[2]), os.path.relpath(path, ROOT)
     except OSError:
         pass
     return None, None
@@ -189,6 +190,9 @@ def imaging_leg(steps, warmup, with_cpu):
     dom = max(avg, key=avg.get)
     algo = imaging_bytes(dom, Nl, Nq, N, Nq)
     achieved = algo / (avg[dom] * 1e-3) / 1e9
+    kname = {"columns": "k_view_columns", "rows_fwd": "k_fft_rows", "rows_bwd": "k_fft_rows",
+             "spectrum": "k_view_spectrum", "filter": "k_view_filter", "colour": "k_view_colour"}[dom]
+    traffic, traffic_src = measured_traffic(kname, TRAFFIC_FILE_IMAGING)
     out = {"metric": "micrographs/s (refocus + 21-wavelength colour micrograph)", "value": steps / wall,
            "unit": "micrographs/s", "ms_per_step": 1e3 * wall / steps,
            "device_ms": {k: round(v, 4) for k, v in avg.items()},
@@ -196,9 +200,9 @@ def imaging_leg(steps, warmup, with_cpu):
                        "shape), z_focus %.1f, NA %.1f / condenser %.1f, tint plate, crossed polarisers"
                        % (Nl, Nq, w["Nx"], w["Ny"], w["z_focus"], w["NA_objective"], w["NA_condenser"]),
            "data": "synthetic (seeded normal fields)",
-           "roofline": {"bound": "hbm", "kernel": dom, "avg_launch_ms": avg[dom], "achieved": round(achieved, 2),
+           "roofline": {"bound": "hbm", "kernel": kname, "avg_launch_ms": avg[dom], "achieved": round(achieved, 2),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "algorithmic_bytes_per_launch": algo,
+                        "traffic": traffic, "traffic_source": traffic_src, "algorithmic_bytes_per_launch": algo,
                         "all": {k: round(imaging_bytes(k, Nl, Nq, N, Nq) / (avg[k] * 1e-3) / 1e9, 1)
                                 for k in avg if avg[k] > 0}}}
     of.close()
