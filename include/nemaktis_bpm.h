@@ -65,7 +65,11 @@ typedef struct {
                               y components, bit 2 polarisation pairs adjacent
                               in the XCD work ranges                            */
     int32_t lean;        /*   1: register-lean solver variants (large batches)   */
-    int32_t y_pair;      /*   1: y-line blocks hold both polarisations           */
+    int32_t y_pair;      /*   1: y-line blocks hold both polarisations,
+                              2: both field components of one polarisation     */
+    int32_t x_both;      /*   x-line blocks hold both field components: bit 0 for
+                              the first launch of a half-step, bit 1 for the
+                              Neumann launches                                 */
 } nmk_bpm_options;
 
 typedef struct {

@@ -20,7 +20,8 @@ class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("wl_batch", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("profile", ctypes.c_int32), ("x_chunks", ctypes.c_int32),
                 ("x_rows", ctypes.c_int32), ("y_chunks", ctypes.c_int32), ("y_cols", ctypes.c_int32),
-                ("item_order", ctypes.c_int32), ("lean", ctypes.c_int32), ("y_pair", ctypes.c_int32)]
+                ("item_order", ctypes.c_int32), ("lean", ctypes.c_int32), ("y_pair", ctypes.c_int32),
+                ("x_both", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -160,7 +161,8 @@ def dptr(a):
 
 
 def options(precision=64, wl_batch=1, use_graph=False, device=-1, profile=False, tuning=None):
-    """tuning: None (built-in shapes) or dict(x_chunks, x_rows, y_chunks, y_cols[, order 0..7][, lean][, y_pair])"""
+    """tuning: None (built-in shapes) or dict(x_chunks, x_rows, y_chunks, y_cols[, order 0..7][, lean][, y_pair]
+    [, x_both])"""
     o = Options()
     load().nmk_bpm_default_options(ctypes.byref(o))
     o.precision = int(precision)
@@ -174,7 +176,8 @@ def options(precision=64, wl_batch=1, use_graph=False, device=-1, profile=False,
         if tuning.get("order") is not None:
             o.item_order = 1 + int(tuning["order"])
         o.lean = 1 if tuning.get("lean") else 0
-        o.y_pair = 1 if tuning.get("y_pair") else 0
+        o.y_pair = int(tuning.get("y_pair") or 0)
+        o.x_both = int(tuning.get("x_both") or 0) & 3
     return o
 
 

@@ -6,7 +6,8 @@ columns for y lines).  The best shapes depend on the mesh, the precision and
 how many propagations march together, so they are measured on a shallow copy
 of the job (same transverse mesh, a few slabs), by coordinate descent: every
 x shape that fits the block limits (<= 512 threads, the device's LDS per
-block, chunks of at most 16 points), then every y shape (one and two
+block, chunks of at most 16 points; one or both field components per block),
+then every y shape (one and two
 polarisations per block), x again, the register-lean variants, and the eight
 XCD work orders.  The winner is cached per (device, precision, mesh,
 propagation count).
@@ -50,15 +51,17 @@ def _lds(M, chunks, lines, c, pols, smap=True):
     return pols * lines * (lmax * P + 7 * P) * c + (M * 4 if smap else 0)
 
 
-def x_candidates(Mx, precision):
+def x_candidates(Mx, precision, both=False):
+    """(chunks, rows) of x-line blocks; both: blocks hold both field components"""
     c = 8 if precision == 32 else 16
+    k = 4 if both else 2
     out = []
     for chunks in _pow2s(16, 512):
         P, lmax = _plan(Mx, chunks)
         if lmax == 0 or lmax > 16 or P != chunks:
             continue
         for rows in (1, 2, 4, 8):
-            if 2 * P * rows <= 512 and _lds(Mx, chunks, rows, c, 2, True) <= LDS_MAX:
+            if k * P * rows <= 512 and _lds(Mx, chunks, rows, c, k, True) <= LDS_MAX:
                 out.append((chunks, rows))
     return out
 
@@ -103,7 +106,7 @@ def _save_cache(c):
 
 
 def cache_key(precision, Nx, Ny, nprops):
-    return "gfx950|solver-v7|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
+    return "gfx950|solver-v8|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
 
 
 def _time(js, lc, n_E, precision, wl_batch, tuning, reps):
@@ -148,12 +151,15 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
     table = []
     t0 = time.time()
     best_ms = None
-    for (xc, xr) in xs:
-        cand = dict(best, x_chunks=xc, x_rows=xr)
-        ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
-        table.append((cand, ms))
-        if best_ms is None or ms < best_ms:
-            best, best_ms = cand, ms
+    # x_both bit 0: first launches hold both components, bit 1: Neumann launches
+    xs2 = x_candidates(Mx, precision, both=True)
+    for both, shapes in ((0, xs), (1, xs2), (3, xs2)):
+        for (xc, xr) in shapes:
+            cand = dict(best, x_chunks=xc, x_rows=xr, x_both=both)
+            ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+            table.append((cand, ms))
+            if best_ms is None or ms < best_ms:
+                best, best_ms = cand, ms
     for (yc, yw) in ys:
         cand = dict(best, y_chunks=yc, y_cols=yw)
         ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
@@ -170,31 +176,42 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
         if ms < best_ms:
             best, best_ms = cand, ms
     # second pass over the x shapes with the y shape found (coordinate descent)
-    for (xc, xr) in xs:
-        cand = dict(best, x_chunks=xc, x_rows=xr)
-        if cand == best:
-            continue
-        ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
-        table.append((cand, ms))
-        if ms < best_ms:
-            best, best_ms = cand, ms
+    for both, shapes in ((0, xs), (1, xs2), (2, xs2), (3, xs2)):
+        for (xc, xr) in shapes:
+            cand = dict(best, x_chunks=xc, x_rows=xr, x_both=both)
+            if cand == best:
+                continue
+            ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+            table.append((cand, ms))
+            if ms < best_ms:
+                best, best_ms = cand, ms
     # register-lean variants (more waves per SIMD) with the shapes found so far
     # and with shorter chunks
     for (xc, yc) in [(best["x_chunks"], best["y_chunks"]), (2 * best["x_chunks"], 2 * best["y_chunks"])]:
         cand = dict(best, x_chunks=xc, y_chunks=yc, lean=1)
-        if (xc, cand["x_rows"]) not in xs or (yc, cand["y_cols"]) not in y_candidates(My, precision,
-                                                                                     pair=bool(cand.get("y_pair"))):
+        xok = x_candidates(Mx, precision, both=bool(cand.get("x_both")))
+        if (xc, cand["x_rows"]) not in xok or (yc, cand["y_cols"]) not in y_candidates(My, precision,
+                                                                                      pair=bool(cand.get("y_pair"))):
             continue
         ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
         table.append((cand, ms))
         if ms < best_ms:
             best, best_ms = cand, ms
-    for order in range(8):  # which items share an XCD's L2 (components, neighbours, wavevectors)
-        cand = dict(best, order=order)
-        ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
-        table.append((cand, ms))
-        if ms < best_ms:
-            best, best_ms = cand, ms
+    # joint pass over the x block contents (components, rows) and the XCD work
+    # orders (which items share an XCD's L2: components, neighbours,
+    # wavevectors), which interact through L2 locality and load balance
+    for both, shapes in ((0, xs), (1, xs2), (2, xs2), (3, xs2)):
+        for (xc, xr) in shapes:
+            if xc != best["x_chunks"]:
+                continue
+            for order in range(8):
+                cand = dict(best, x_rows=xr, x_both=both, order=order)
+                if cand == best:
+                    continue
+                ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+                table.append((cand, ms))
+                if ms < best_ms:
+                    best, best_ms = cand, ms
     if log:
         log("autotune %s: %d candidates in %.1f s, best %s (%.3f ms for %d slabs)"
             % (key, len(table), time.time() - t0, best, best_ms, depth - 1))

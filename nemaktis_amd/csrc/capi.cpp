@@ -42,7 +42,8 @@ Options to_options(const nmk_bpm_options *o) {
             r.tuning.y_cols = o->y_cols;
             if (o->item_order > 0) r.tuning.order = (o->item_order - 1) & 7;
             r.tuning.lean = o->lean ? 1 : 0;
-            r.tuning.y_pair = o->y_pair ? 1 : 0;
+            r.tuning.y_pair = o->y_pair == 1 || o->y_pair == 2 ? o->y_pair : 0;
+            r.tuning.x_both = o->x_both & 3;
         }
     }
     return r;

@@ -488,12 +488,12 @@ private:
             // x half-steps: other = (I - mu A_x)^-1 (I + mu A_y) cur by its Neumann series
             if (g_.px) {
                 timed(KF_SOLVE_X, [&] {
-                    launch_xsolve<T>(0, cur, A, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, tune_.lean, st_);
+                    launch_xsolve<T>(0, cur, A, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, tune_.lean, tune_.x_both & 1, st_);
                 });
                 C2<T> *a = A, *b = B;
                 for (int it = 0; it < nw; it++) {
                     timed(KF_SOLVE_X, [&] {
-                        launch_xsolve<T>(1, a, b, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, tune_.lean, st_);
+                        launch_xsolve<T>(1, a, b, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, tune_.lean, (tune_.x_both >> 1) & 1, st_);
                     });
                     std::swap(a, b);
                 }
@@ -538,7 +538,8 @@ private:
         if (g_.px) lpx_ = make_line_plan(g_.Mx, t.x_chunks);
         if (g_.py) lpy_ = make_line_plan(g_.My, t.y_chunks);
         if (g_.px) {
-            if (2 * lpx_.P * t.x_rows > 512 || xsolve_lds_bytes<T>(lpx_, t.x_rows) > max_block_lds())
+            if ((t.x_both ? 4 : 2) * lpx_.P * t.x_rows > 512 ||
+                xsolve_lds_bytes<T>(lpx_, t.x_rows, t.x_both ? 2 : 1) > max_block_lds())
                 throw std::runtime_error("x line solver tuning exceeds the block limits");
             xt_.alloc(2 * (size_t)sz_.nslots_max * 2 * 3 * g_.My * lpx_.lmax * lpx_.P);
         }
@@ -572,9 +573,9 @@ private:
 Tuning default_tuning(int precision, int Mx, int My) {
     Tuning t;
     const size_t budget = max_block_lds();
-    auto xlds = [&](int chunks, int rows) {
+    auto xlds = [&](int chunks, int rows, int nc) {
         const LinePlan lp = make_line_plan(Mx, chunks);
-        return precision == 32 ? xsolve_lds_bytes<float>(lp, rows) : xsolve_lds_bytes<double>(lp, rows);
+        return precision == 32 ? xsolve_lds_bytes<float>(lp, rows, nc) : xsolve_lds_bytes<double>(lp, rows, nc);
     };
     auto ylds = [&](int chunks, int cols) {
         const LinePlan lp = make_line_plan(My, chunks);
@@ -582,9 +583,18 @@ Tuning default_tuning(int precision, int Mx, int My) {
     };
     // chunks of at most 8 points (16 for lines longer than 1024)
     t.x_chunks = std::min(128, std::max(16, pow2_at_least((Mx + 7) / 8)));
-    t.x_rows = 2;
-    if (Mx > 3)
-        while (t.x_rows > 1 && (2 * t.x_chunks * t.x_rows > 512 || xlds(t.x_chunks, t.x_rows) > budget)) t.x_rows /= 2;
+    // first half-step launches: both field components per block (their
+    // right-hand sides have 6 and 14 taps; blocks holding both carry equal
+    // work); Neumann launches (11 and 3 taps) one component per block
+    t.x_rows = 1;
+    t.x_both = 1;
+    if (4 * t.x_chunks > 512 || xlds(t.x_chunks, 1, 2) > budget) {
+        t.x_both = 0;
+        t.x_rows = 2;
+        if (Mx > 3)
+            while (t.x_rows > 1 && (2 * t.x_chunks * t.x_rows > 512 || xlds(t.x_chunks, t.x_rows, 1) > budget))
+                t.x_rows /= 2;
+    }
     t.y_chunks = std::min(128, std::max(16, pow2_at_least((My + 7) / 8)));
     t.y_cols = 8;
     if (My > 3)

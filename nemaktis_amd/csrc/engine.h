@@ -19,7 +19,8 @@ struct Tuning {
     int y_chunks = 0, y_cols = 0;  // y lines: chunks per column, columns per block
     int order = -1;                // XCD work order of the solvers (bits, see launch_xsolve); -1: by batch size
     int lean = 0;                  // 1: register-lean solver variants (see launch_xsolve)
-    int y_pair = 0;                // 1: y-line blocks hold both polarisations
+    int y_pair = 0;                // 1: y-line blocks hold both polarisations, 2: both components
+    int x_both = 0;                // x-line blocks hold both field components: bit 0 first launch, bit 1 Neumann
 };
 Tuning default_tuning(int precision, int Mx, int My);
 

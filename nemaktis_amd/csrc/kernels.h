@@ -42,7 +42,8 @@ void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, cons
 // in the XCD work ranges (else neighbouring row blocks), bit 1 the same for
 // y strips, bit 2 puts the polarisation pairs innermost.
 // lean: one gather point per round and no row prefetch (fewer registers,
-// more waves per SIMD for large batches).  pair (y lines): both polarisations
+// more waves per SIMD for large batches).  both (x lines): one block holds
+// both field components of its rows (equal work per block).  pair (y lines): both polarisations
 // in one block, gathered together (2*P*W threads).
 // cyclic block-tridiagonal line solves of the implicit half-steps, one
 // thread per chunk carrying both polarisations (nprops must be even, the
@@ -57,13 +58,13 @@ size_t max_block_lds();
 // raises the dynamic LDS limit of kernel f to lds bytes when above 64 KiB
 void allow_kernel_lds(const void *f, size_t lds);
 template <class T>
-size_t xsolve_lds_bytes(const LinePlan &lp, int R);
+size_t xsolve_lds_bytes(const LinePlan &lp, int R, int nc);
 template <class T>
 size_t ysolve_lds_bytes(const LinePlan &lp, int W, int pair);
 template <class T>
 void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
                    const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, int order,
-                   int lean, hipStream_t st);
+                   int lean, int both, hipStream_t st);
 template <class T>
 void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
                    const PropParams *pp, int nprops, const LinePlan &lp, int W, int order, int lean, int pair,

@@ -1048,10 +1048,13 @@ __device__ __forceinline__ void advance_rc(int &r, int &x, int step, int M) {
 }
 
 // ---- x lines: R rows per block for both polarisations (2 R P threads,
-// polarisation-major).  The right-hand sides are staged through LDS in the
-// chunk-interleaved order (element (p, i) of a row at i*P + p) so that global
-// loads and stores stay coalesced; the tridiagonal rows come pre-interleaved
-// from the coefficient kernel.
+// polarisation-major) of one component, or with nc = 2 of both components
+// (4 R P threads, component-major): the two components' right-hand sides have
+// different tap counts (6 and 14 for (I + mu A_y), 11 and 3 for mu O_x), so
+// blocks holding both carry equal work.  The right-hand sides are staged
+// through LDS in the chunk-interleaved order (element (p, i) of a row at
+// i*P + p) so that global loads and stores stay coalesced; the tridiagonal
+// rows come pre-interleaved from the coefficient kernel.
 //   neumann 0: dst = M_x (I + mu A_y) src, acc = dst
 //   neumann 1: dst = M_x (mu O_x src),     acc += dst
 template <class T, int LMAX, int NEU, int LEAN>
@@ -1059,29 +1062,32 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
                                                  C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf,
                                                  const C2<T> *__restrict__ xt, Geo g,
                                                  const PropParams *__restrict__ pp, LinePlan lp, int R,
-                                                 int nitems_pairs, int order) {
+                                                 int nitems_pairs, int order, int nc) {
     using C = C2<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const int P = lp.P, SL = LMAX * P, M = g.Mx, RP = R * P;
-    C *sd = reinterpret_cast<C *>(smem_raw);  // [pol][row][SL]
-    const RedLds<T> red(sd + 2 * R * SL, 2 * RP);
-    int *smap = reinterpret_cast<int *>(sd + 2 * R * SL + RedLds<T>::kArrays * 2 * RP);  // x -> i*P + p
+    const int P = lp.P, SL = LMAX * P, M = g.Mx, RP = R * P, NL = nc * 2 * R;  // NL lines per block
+    C *sd = reinterpret_cast<C *>(smem_raw);  // [component][pol][row][SL]
+    const RedLds<T> red(sd + NL * SL, NL * P);
+    int *smap = reinterpret_cast<int *>(sd + NL * SL + RedLds<T>::kArrays * NL * P);  // x -> i*P + p
     // work items (component and row block in the order `order & 1` selects,
     // polarisation pair outermost or, with `order & 4`, innermost); each XCD
     // takes a contiguous range of them, so both components of a row block, the
     // neighbouring row blocks or the wavevectors of one wavelength meet in L2
-    const int nrb = (g.My + R - 1) / R;
+    const int nrb = (g.My + R - 1) / R, nci = 3 - nc;  // component items per row block
     const int item = xcd_tile(blockIdx.x, gridDim.x);
-    if (item >= nrb * 2 * nitems_pairs) return;
+    if (item >= nrb * nci * nitems_pairs) return;
     int comp, row0, pair, rest;
     if (order & 4) {  // pair fastest: the wavevectors of one wavelength share the coefficients
         pair = item % nitems_pairs;
         rest = item / nitems_pairs;
     } else {
-        pair = item / (2 * nrb);
-        rest = item % (2 * nrb);
+        pair = item / (nci * nrb);
+        rest = item % (nci * nrb);
     }
-    if (order & 1) {  // component before row block: both components of a row block together
+    if (nc == 2) {
+        comp = 0;
+        row0 = rest * R;
+    } else if (order & 1) {  // component before row block: both components of a row block together
         comp = rest & 1;
         row0 = (rest >> 1) * R;
     } else {  // neighbouring row blocks together
@@ -1099,11 +1105,12 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     const C *cfs = cf + (size_t)slot * cf_slot_size(g.NP);
     const PairFields<T> fin(src, prop0, NP);
     const int nrow = min(R, g.My - row0), bs = blockDim.x, ntot = nrow * M;
-    C *sd0 = sd, *sd1 = sd + R * SL;
 
     // right-hand sides of the block's rows, both polarisations
-    {
+    for (int c = 0; c < nc; c++) {
         constexpr int KIND = NEU ? RHS_OX : RHS_YPLUS, NPR = LEAN ? 1 : rhs_points<T, LMAX>();
+        C *sd0 = sd + c * 2 * R * SL, *sd1 = sd0 + R * SL;
+        const int gc = nc == 2 ? c : comp;
         int r0 = (int)threadIdx.x / M, x0 = (int)threadIdx.x - r0 * M;
         for (int e0 = threadIdx.x; e0 < ntot; e0 += NPR * bs) {
             unsigned xs[NPR], ys[NPR];
@@ -1118,7 +1125,7 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
                 advance_rc(r0, x0, bs, M);
             }
             C d[NPR][2];
-            rhs_pair_n<T, KIND, NPR>(comp, xs, ys, v, cfs, fin, g, d);
+            rhs_pair_n<T, KIND, NPR>(gc, xs, ys, v, cfs, fin, g, d);
 #pragma unroll
             for (int k = 0; k < NPR; k++)
                 if (v[k]) {
@@ -1129,12 +1136,14 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     }
     __syncthreads();
 
-    // one chunk of one row of one polarisation per thread
-    const int pol = threadIdx.x / RP, t = threadIdx.x - pol * RP, r = t / P, p = t - r * P, y = row0 + r;
+    // one chunk of one row of one polarisation (of one component) per thread
+    const int ct = (int)threadIdx.x / (2 * RP), tc = (int)threadIdx.x - ct * 2 * RP;
+    const int pol = tc / RP, t = tc - pol * RP, r = t / P, p = t - r * P, y = row0 + r;
+    const int tcomp = nc == 2 ? ct : comp;
     const bool active = r < nrow;
-    const int n = chunk_len(lp, p) - 1, line = pol * R + r;
+    const int n = chunk_len(lp, p) - 1, line = (ct * 2 + pol) * R + r;
     C *sdl = sd + line * SL;
-    const int64_t xrow = ((((int64_t)slot * 2 + comp) * 3) * g.My + min(y, g.My - 1)) * SL;
+    const int64_t xrow = ((((int64_t)slot * 2 + tcomp) * 3) * g.My + min(y, g.My - 1)) * SL;
     const int64_t xstride = (int64_t)g.My * SL;
     const C *xa = xt + xrow, *xb = xa + xstride, *xc = xb + xstride;
     const int me = line * P + p, nx = line * P + ((p + 1) & (P - 1));
@@ -1196,40 +1205,52 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     __syncthreads();
 
     {
-        // a thread stores at most LMAX/2 points (nrow*M <= R*P*LMAX, 2*R*P threads);
-        // the accumulator values of all of them are fetched in one round trip
-        const size_t o0 = ((size_t)prop0 * 2 + comp) * NP, o1 = o0 + 2 * NP;
-        C *d0 = dst + o0, *d1 = dst + o1, *a0 = acc + o0, *a1 = acc + o1;
+        // a thread stores at most LMAX/2 points of the block's nc * nrow * M
+        // (nrow*M <= R*P*LMAX, 2*nc*R*P threads); element e is point e % ntot of
+        // component e / ntot; the accumulator values of all of them are fetched
+        // in one round trip
         constexpr int NO = LMAX / 2;
         C o0v[NO], o1v[NO];
-        const int rs = (int)threadIdx.x / M, xs = (int)threadIdx.x - rs * M;
+        int rr = (int)threadIdx.x / M, x = (int)threadIdx.x - rr * M, cc = 0;
+        while (rr >= nrow) {
+            rr -= nrow;
+            cc++;
+        }
+        auto step = [&]() {
+            advance_rc(rr, x, bs, M);
+            while (rr >= nrow) {
+                rr -= nrow;
+                cc++;
+            }
+        };
+        const size_t pbase = (size_t)prop0 * 2 * NP;
         if (NEU) {
-            int rr = rs, x = xs;
+            const int r1 = rr, x1 = x, c1 = cc;
 #pragma unroll
             for (int k = 0; k < NO; k++) {
-                const int e = threadIdx.x + k * bs;
-                if (e < ntot) {
-                    const unsigned q = x + M * (row0 + rr);
-                    o0v[k] = a0[q];
-                    o1v[k] = a1[q];
+                if (cc < nc) {
+                    const size_t o = pbase + (size_t)(nc == 2 ? cc : comp) * NP + x + M * (row0 + rr);
+                    o0v[k] = acc[o];
+                    o1v[k] = acc[o + 2 * NP];
                 }
-                advance_rc(rr, x, bs, M);
+                step();
             }
+            rr = r1;
+            x = x1;
+            cc = c1;
         }
-        int rr = rs, x = xs;
 #pragma unroll
         for (int k = 0; k < NO; k++) {
-            const int e = threadIdx.x + k * bs;
-            if (e < ntot) {
-                const int ia = rr * SL + smap[x];
-                const unsigned q = x + M * (row0 + rr);
-                const C v0 = sd0[ia], v1 = sd1[ia];
-                d0[q] = v0;
-                d1[q] = v1;
-                a0[q] = NEU ? o0v[k] + v0 : v0;
-                a1[q] = NEU ? o1v[k] + v1 : v1;
+            if (cc < nc) {
+                const int ia = (cc * 2 * R + rr) * SL + smap[x];
+                const size_t o = pbase + (size_t)(nc == 2 ? cc : comp) * NP + x + M * (row0 + rr);
+                const C v0 = sd[ia], v1 = sd[ia + R * SL];
+                dst[o] = v0;
+                dst[o + 2 * NP] = v1;
+                acc[o] = NEU ? o0v[k] + v0 : v0;
+                acc[o + 2 * NP] = NEU ? o1v[k] + v1 : v1;
             }
-            advance_rc(rr, x, bs, M);
+            step();
         }
     }
 }
@@ -1267,31 +1288,35 @@ template <class T, int LMAX, int NEU, int LEAN>
 __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
                                                  C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf, Geo g,
                                                  const PropParams *__restrict__ pp, LinePlan lp, int W,
-                                                 int npairs, int order) {
+                                                 int npairs, int order, int nc) {
     using C = C2<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int P = lp.P, M = g.My, TL = LMAX * P * W;
-    C *sd = reinterpret_cast<C *>(smem_raw);
-    const RedLds<T> red(sd + TL, W * P);
-    int *ymap = reinterpret_cast<int *>(sd + TL + RedLds<T>::kArrays * W * P);  // row j -> i*P + p
+    C *sd = reinterpret_cast<C *>(smem_raw);  // [component][TL]
+    const RedLds<T> red(sd + nc * TL, nc * W * P);
+    int *ymap = reinterpret_cast<int *>(sd + nc * TL + RedLds<T>::kArrays * nc * W * P);  // row j -> i*P + p
     // work items (polarisation fastest, then component and strip in the order
     // `order & 2` selects, pair outermost or, with `order & 4`, right after the
     // polarisation); each XCD takes a contiguous range, so both polarisations
     // of a strip and its components, neighbouring strips or the wavevectors of
-    // one wavelength share the 128-byte lines of fields and coefficients in L2
-    const int nstrip = (g.Mx + W - 1) / W;
+    // one wavelength share the 128-byte lines of fields and coefficients in L2.
+    // nc = 2: the block holds both components (nc*P*W threads, component-major)
+    const int nstrip = (g.Mx + W - 1) / W, nci = 3 - nc;
     const int item = xcd_tile(blockIdx.x, gridDim.x);
-    if (item >= nstrip * 4 * npairs) return;
+    if (item >= nstrip * 2 * nci * npairs) return;
     const int pol = item & 1;
     int comp, lb, pair, rest;
     if (order & 4) {
         pair = (item >> 1) % npairs;
         rest = (item >> 1) / npairs;
     } else {
-        pair = (item >> 1) / (2 * nstrip);
-        rest = (item >> 1) % (2 * nstrip);
+        pair = (item >> 1) / (nci * nstrip);
+        rest = (item >> 1) % (nci * nstrip);
     }
-    if (order & 2) {
+    if (nc == 2) {
+        comp = 0;
+        lb = rest;
+    } else if (order & 2) {
         comp = rest & 1;
         lb = rest >> 1;
     } else {
@@ -1309,27 +1334,38 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
         for (int i = 0; i < lc; i++) ymap[s0c + i] = i * P + pc;
     }
     __syncthreads();
-    // W is a power of two and the block has P*W threads: a thread keeps its
+    // W is a power of two and a component has P*W threads: a thread keeps its
     // column and steps P rows at a time
-    const int cc = threadIdx.x & (W - 1), j0 = (int)threadIdx.x >> (__ffs(W) - 1);
+    const int ct = (int)threadIdx.x / (P * W), tl = (int)threadIdx.x - ct * P * W;
+    const int cc = tl & (W - 1), j0 = tl >> (__ffs(W) - 1);
     const bool col_ok = cc < wv;
 
-    // right-hand sides of the tile
+    // right-hand sides of the tile: with nc = 2 every thread gathers rows of
+    // both components (rows ct*P + j0, + 2P, ...), so the two components'
+    // unequal tap counts are shared by all threads
     if (col_ok) {
         constexpr int KIND = NEU ? RHS_OY : RHS_XPLUS, NPT = LEAN ? 1 : rhs_points<T, LMAX>();
-        for (int j = j0; j < M; j += NPT * P) {
-            C d[NPT];
-            rhs_column<T, KIND, NPT>(comp, x0 + cc, j, P, M, cfs, f0, f1, g, d);
+        const int step = nc * P;
+        for (int c = 0; c < nc; c++) {
+            const int gc = nc == 2 ? c : comp;
+            C *sdc = sd + c * TL;
+            for (int j = ct * P + j0; j < M; j += NPT * step) {
+                C d[NPT];
+                rhs_column<T, KIND, NPT>(gc, x0 + cc, j, step, M, cfs, f0, f1, g, d);
 #pragma unroll
-            for (int k = 0; k < NPT; k++)
-                if (j + k * P < M) sd[ymap[j + k * P] * W + cc] = d[k];
+                for (int k = 0; k < NPT; k++)
+                    if (j + k * step < M) sdc[ymap[j + k * step] * W + cc] = d[k];
+            }
         }
     }
     __syncthreads();
 
     const int cofs = cc, p = j0, x = x0 + cofs;
     const bool active = col_ok;
-    const int blk = comp == 0 ? 0 : 3;
+    const int tcomp = nc == 2 ? ct : comp;
+    sd += ct * TL;
+    const int rb = ct * P * W;  // this component's reduced-system entries
+    const int blk = tcomp == 0 ? 0 : 3;
     const C *cdy = cfs + (CF_DY + blk * 3) * kCfTile;
     const int s = chunk_start(lp, p), n = chunk_len(lp, p) - 1;
     auto tri = [&](int j, C &a, C &b, C &c) {
@@ -1340,7 +1376,7 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
         b = C{(T)1 - vb.x, -vb.y};
         c = C{-vc.x, -vc.y};
     };
-    const int me = p * W + cofs, nx = ((p + 1) & (P - 1)) * W + cofs;
+    const int me = rb + p * W + cofs, nx = rb + ((p + 1) & (P - 1)) * W + cofs;
     Chunk<T, LMAX> ck;
     if (active) {
         chunk_eliminate<T, LMAX, sizeof(T) == 4 && !LEAN>(ck, n, [&](int i, C &a, C &b, C &c, C &d) {
@@ -1366,9 +1402,9 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
         red.D[me] = D;
     }
     __syncthreads();
-    reduced_pcr(red, P, p, cofs, W, active);
+    reduced_pcr(red, P, p, rb + cofs, W, active);
     if (active) {
-        const C xi = red.uf[me], xprev = red.uf[((p - 1) & (P - 1)) * W + cofs];
+        const C xi = red.uf[me], xprev = red.uf[rb + ((p - 1) & (P - 1)) * W + cofs];
         chunk_rebuild(ck, n, xprev, xi, [&](int i, C v) { sd[(i * P + p) * W + cofs] = v; });
         sd[(n * P + p) * W + cofs] = xi;
     }
@@ -1377,7 +1413,7 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     {
         // a thread stores at most LMAX points of its column (M <= P*LMAX);
         // the accumulator values of all of them are fetched in one round trip
-        C *dd = dst + ((size_t)prop * 2 + comp) * NP, *aa = acc + ((size_t)prop * 2 + comp) * NP;
+        C *dd = dst + ((size_t)prop * 2 + tcomp) * NP, *aa = acc + ((size_t)prop * 2 + tcomp) * NP;
         C ov[LMAX];
         if (NEU && col_ok) {
 #pragma unroll
@@ -1604,12 +1640,13 @@ LinePlan make_line_plan(int M, int chunks) {
 
 // staged right-hand sides of both polarisations + their reduced systems
 template <class T>
-size_t xsolve_lds_bytes(const LinePlan &lp, int R) {
-    return 2 * (size_t)R * ((size_t)lp.lmax * lp.P + (size_t)RedLds<T>::kArrays * lp.P) * sizeof(C2<T>) +
+size_t xsolve_lds_bytes(const LinePlan &lp, int R, int nc) {
+    return (size_t)nc * 2 * R * ((size_t)lp.lmax * lp.P + (size_t)RedLds<T>::kArrays * lp.P) * sizeof(C2<T>) +
            (size_t)lp.M * sizeof(int);
 }
 template <class T>
 size_t ysolve_lds_bytes(const LinePlan &lp, int W, int pair) {
+    // pair: 1 both polarisations, 2 both components (one polarisation)
     return (size_t)(pair ? 2 : 1) * W * ((size_t)lp.lmax * lp.P + (size_t)RedLds<T>::kArrays * lp.P) * sizeof(C2<T>) +
            (size_t)lp.M * sizeof(int);
 }
@@ -1617,21 +1654,21 @@ size_t ysolve_lds_bytes(const LinePlan &lp, int W, int pair) {
 template <class T, int NEU, int LEAN>
 static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
                             const C2<T> *cf, const C2<T> *xt, const Geo &g, const PropParams *pp, int npairs,
-                            int order, hipStream_t st) {
-    const dim3 block(2 * lp.P * R);
-    const size_t lds = xsolve_lds_bytes<T>(lp, R);
+                            int order, int nc, hipStream_t st) {
+    const dim3 block(nc * 2 * lp.P * R);
+    const size_t lds = xsolve_lds_bytes<T>(lp, R, nc);
     switch (lp.lmax) {
         case 4:
             allow_lds(k_xsolve<T, 4, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_xsolve<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order);
+            hipLaunchKernelGGL((k_xsolve<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order, nc);
             break;
         case 8:
             allow_lds(k_xsolve<T, 8, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_xsolve<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order);
+            hipLaunchKernelGGL((k_xsolve<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order, nc);
             break;
         case 16:
             allow_lds(k_xsolve<T, 16, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_xsolve<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order);
+            hipLaunchKernelGGL((k_xsolve<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order, nc);
             break;
         default: throw std::runtime_error("x line solver: unsupported chunk length");
     }
@@ -1667,22 +1704,22 @@ static void ysolve_pair_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<
 
 template <class T, int NEU, int LEAN>
 static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
-                            const C2<T> *cf, const Geo &g, const PropParams *pp, int npairs, int order,
+                            const C2<T> *cf, const Geo &g, const PropParams *pp, int npairs, int order, int nc,
                             hipStream_t st) {
-    const dim3 block(lp.P * W);
-    const size_t lds = ysolve_lds_bytes<T>(lp, W, 0);
+    const dim3 block(nc * lp.P * W);
+    const size_t lds = ysolve_lds_bytes<T>(lp, W, nc == 2 ? 2 : 0);
     switch (lp.lmax) {
         case 4:
             allow_lds(k_ysolve<T, 4, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_ysolve<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order);
+            hipLaunchKernelGGL((k_ysolve<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order, nc);
             break;
         case 8:
             allow_lds(k_ysolve<T, 8, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_ysolve<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order);
+            hipLaunchKernelGGL((k_ysolve<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order, nc);
             break;
         case 16:
             allow_lds(k_ysolve<T, 16, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_ysolve<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order);
+            hipLaunchKernelGGL((k_ysolve<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order, nc);
             break;
         default: throw std::runtime_error("y line solver: unsupported chunk length");
     }
@@ -1692,19 +1729,20 @@ static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *s
 template <class T>
 void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
                    const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, int order,
-                   int lean, hipStream_t st) {
-    if (2 * lp.P * R > 512 || lp.M != g.Mx || R < 1 || lp.P * lp.lmax < lp.M)
+                   int lean, int both, hipStream_t st) {
+    const int nc = both ? 2 : 1;
+    if (nc * 2 * lp.P * R > 512 || lp.M != g.Mx || R < 1 || lp.P * lp.lmax < lp.M)
         throw std::runtime_error("x line solver: bad launch shape");
     if (nprops % 2) throw std::runtime_error("x line solver: propagations come in polarisation pairs");
     const int npairs = nprops / 2;
-    const int64_t items = (int64_t)((g.My + R - 1) / R) * 2 * npairs;
+    const int64_t items = (int64_t)((g.My + R - 1) / R) * (3 - nc) * npairs;
     dim3 grid((unsigned)((items + 7) / 8 * 8));
     if (lean) {
-        if (neumann) xsolve_dispatch<T, 1, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, st);
-        else xsolve_dispatch<T, 0, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, st);
+        if (neumann) xsolve_dispatch<T, 1, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, nc, st);
+        else xsolve_dispatch<T, 0, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, nc, st);
     } else {
-        if (neumann) xsolve_dispatch<T, 1, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, st);
-        else xsolve_dispatch<T, 0, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, st);
+        if (neumann) xsolve_dispatch<T, 1, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, nc, st);
+        else xsolve_dispatch<T, 0, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, nc, st);
     }
 }
 
@@ -1712,11 +1750,12 @@ template <class T>
 void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
                    const PropParams *pp, int nprops, const LinePlan &lp, int W, int order, int lean, int pair,
                    hipStream_t st) {
-    if ((pair ? 2 : 1) * lp.P * W > 512 || lp.M != g.My || W < 1 || (W & (W - 1)) || lp.P * lp.lmax < lp.M)
+    if ((pair ? 2 : 1) * lp.P * W > 512 || lp.M != g.My || W < 1 || (W & (W - 1)) || lp.P * lp.lmax < lp.M ||
+        pair < 0 || pair > 2)
         throw std::runtime_error("y line solver: bad launch shape");
     if (nprops % 2) throw std::runtime_error("y line solver: propagations come in polarisation pairs");
     const int npairs = nprops / 2;
-    if (pair) {
+    if (pair == 1) {
         const int64_t items = (int64_t)((g.Mx + W - 1) / W) * 2 * npairs;
         dim3 grid((unsigned)((items + 7) / 8 * 8));
         if (lean) {
@@ -1728,14 +1767,15 @@ void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const 
         }
         return;
     }
-    const int64_t items = (int64_t)((g.Mx + W - 1) / W) * 4 * npairs;
+    const int nc = pair == 2 ? 2 : 1;
+    const int64_t items = (int64_t)((g.Mx + W - 1) / W) * 2 * (3 - nc) * npairs;
     dim3 grid((unsigned)((items + 7) / 8 * 8));
     if (lean) {
-        if (neumann) ysolve_dispatch<T, 1, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
-        else ysolve_dispatch<T, 0, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+        if (neumann) ysolve_dispatch<T, 1, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, nc, st);
+        else ysolve_dispatch<T, 0, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, nc, st);
     } else {
-        if (neumann) ysolve_dispatch<T, 1, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
-        else ysolve_dispatch<T, 0, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+        if (neumann) ysolve_dispatch<T, 1, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, nc, st);
+        else ysolve_dispatch<T, 0, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, nc, st);
     }
 }
 
@@ -2027,10 +2067,10 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
     template void launch_explicit<T>(int, const C2<T> *, C2<T> *, const C2<T> *, const Geo &, const PropParams *,    \
                                      int, hipStream_t);                                                               \
     template void launch_xsolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const C2<T> *, const Geo &, \
-                                   const PropParams *, int, const LinePlan &, int, int, int, hipStream_t);            \
+                                   const PropParams *, int, const LinePlan &, int, int, int, int, hipStream_t);       \
     template void launch_ysolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const Geo &,                  \
                                    const PropParams *, int, const LinePlan &, int, int, int, int, hipStream_t);       \
-    template size_t xsolve_lds_bytes<T>(const LinePlan &, int);                                                       \
+    template size_t xsolve_lds_bytes<T>(const LinePlan &, int, int);                                                  \
     template size_t ysolve_lds_bytes<T>(const LinePlan &, int, int);                                                  \
     template void launch_output<T>(const C2<T> *, const double *, const uint8_t *, const Geo &, const SlotParams *,  \
                                    const PropParams *, int, int, c64 *, hipStream_t);

@@ -94,6 +94,15 @@ __global__ __launch_bounds__(256) void k_view_columns(c64 *__restrict__ data, co
     const c64 *fp = filt + lq * N;
     const int x0 = g * C;
     c64 *stw = fbuf + 2 * C * n;
+    // the filter values this thread multiplies, loaded with the fields (n * C <= 8 * 256)
+    constexpr int kPer = 8;
+    c64 fr[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; e++) {
+        const int t = threadIdx.x + e * 256;
+        const int i = t % C, y = t / C;
+        fr[e] = (t < n * C && x0 + i < Nx) ? fp[(int64_t)y * Nx + x0 + i] : mk(0.0, 0.0);
+    }
     for (int t = threadIdx.x; t < n * C; t += blockDim.x) {
         const int i = t % C, y = t / C;
         fbuf[t] = (x0 + i < Nx) ? plane[(int64_t)y * Nx + x0 + i] : mk(0.0, 0.0);
@@ -101,9 +110,10 @@ __global__ __launch_bounds__(256) void k_view_columns(c64 *__restrict__ data, co
     for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = tw[t];
     __syncthreads();
     c64 *a = fft_stages<C, false, ODD>(fbuf, fbuf + n * C, pl, stw, -1);
-    for (int t = threadIdx.x; t < n * C; t += blockDim.x) {
-        const int i = t % C, y = t / C;
-        a[t] = (x0 + i < Nx) ? a[t] * fp[(int64_t)y * Nx + x0 + i] : mk(0.0, 0.0);
+#pragma unroll
+    for (int e = 0; e < kPer; e++) {
+        const int t = threadIdx.x + e * 256;
+        if (t < n * C) a[t] = a[t] * fr[e];
     }
     __syncthreads();
     const c64 *r = fft_stages<C, false, ODD>(a, a == fbuf ? fbuf + n * C : fbuf, pl, stw, +1);
@@ -116,6 +126,7 @@ __global__ __launch_bounds__(256) void k_view_columns(c64 *__restrict__ data, co
 void launch_view_columns(c64 *data, const c64 *filt, int Nx, int Ny, int nlq, const c64 *tw_y, const FftPlan &py,
                          hipStream_t st) {
     const int C = fft_cols_per_block(Ny);
+    if (Ny * C > 8 * 256) throw std::runtime_error("refocus columns: more than 2048 values per block");
     const int ngroups = (Nx + C - 1) / C;
     const int64_t nitems = (int64_t)ngroups * nlq * 4;
     const unsigned grid = (unsigned)((nitems + 7) / 8 * 8);
@@ -164,6 +175,7 @@ __global__ __launch_bounds__(256) void k_view_spectrum(const c64 *__restrict__ E
     const int l = blockIdx.y;
     const JonesSet J = jones[l];
     double acc = 0;
+#pragma unroll 4
     for (int q = q0; q < q1; q++) {
         const c64 *e = E + ((int64_t)l * Nq + q) * 4 * N + m;
         const c64 S[4] = {e[0], e[2 * N], e[N], e[3 * N]};

@@ -206,6 +206,9 @@ def test_random512_reduced_depth_fp32_vs_fp64():
     {"x_chunks": 32, "x_rows": 1, "y_chunks": 32, "y_cols": 1},   # chunks of 2-3 points
     {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8},   # chunks of 4-5 points
     {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8, "y_pair": 1, "lean": 1},  # paired y, lean
+    {"x_chunks": 16, "x_rows": 1, "y_chunks": 16, "y_cols": 4, "x_both": 3, "y_pair": 2},  # both components
+    {"x_chunks": 8, "x_rows": 2, "y_chunks": 8, "y_cols": 2, "x_both": 2, "y_pair": 2, "lean": 1},
+    {"x_chunks": 32, "x_rows": 4, "y_chunks": 16, "y_cols": 8, "x_both": 3},  # four rows of both components
 ])
 def test_line_solver_shapes_fp64(tuning):
     """Every autotuner candidate shape must give the oracle's answer."""
@@ -215,6 +218,15 @@ def test_line_solver_shapes_fp64(tuning):
     assert err < TOL64, err
 
 
+def test_both_component_blocks_with_lds_reduced_systems():
+    """x lines of 299 points in 128 chunks (reduced systems in LDS) with both
+    components per block, against 64 chunks (lane permutes), FP64."""
+    s, lc = _random_case((300, 40, 4), seed=23)
+    a = _gpu(s, lc, tuning={"x_chunks": 128, "x_rows": 1, "y_chunks": 16, "y_cols": 4, "x_both": 3})
+    b = _gpu(s, lc, tuning={"x_chunks": 64, "x_rows": 2, "y_chunks": 16, "y_cols": 4})
+    assert _rel(a, b) < 1e-12
+
+
 def test_xcd_work_orders_are_bitwise_equal():
     """The XCD work orders only move blocks between XCDs: identical output
     (two wavelengths x two wavevectors, so that every index varies)."""
@@ -222,6 +234,8 @@ def test_xcd_work_orders_are_bitwise_equal():
     base = {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8}
     outs = [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o)) for o in range(8)]
     outs += [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o, y_pair=1)) for o in (0, 6)]
+    outs += [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o, x_both=3, x_rows=1, y_pair=2,
+                                                                 y_cols=4)) for o in (1, 4)]
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
     want = oracle.propagate(s, lc)
