@@ -102,6 +102,10 @@ int nmk_bpm_upload_fields(nmk_bpm_ctx *ctx, const double *E_field_vals, int64_t 
                           size_t err_len);
 int nmk_bpm_propagate(nmk_bpm_ctx *ctx, nmk_bpm_stats *stats, char *err_msg, size_t err_len);
 int nmk_bpm_download(nmk_bpm_ctx *ctx, double *E_field_vals, int64_t n_E_vals, char *err_msg, size_t err_len);
+/* new line-solver launch shapes for the next propagations (the tuning fields
+ * of opts; x_chunks = 0: built-in shapes); on refusal the context keeps its
+ * shapes.  Used by the autotuner to time candidates on one resident job. */
+int nmk_bpm_set_tuning(nmk_bpm_ctx *ctx, const nmk_bpm_options *opts, char *err_msg, size_t err_len);
 void nmk_bpm_destroy(nmk_bpm_ctx *ctx);
 
 /* HBM bytes a run of these settings would hold (serving admission). */

@@ -109,16 +109,18 @@ def cache_key(precision, Nx, Ny, nprops):
     return "gfx950|solver-v8|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
 
 
-def _time(js, lc, n_E, precision, wl_batch, tuning, reps):
+def _time(ctx, tuning, reps):
+    """Median march time of the resident job with `tuning` (inf when the
+    device refuses the shape).  One context serves every candidate: buffers
+    that a captured graph has used are never freed between candidates."""
     try:
-        ctx = bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, use_graph=True, tuning=tuning)
+        ctx.set_tuning(tuning)
     except bpm_backend.BpmBackendError:
         return float("inf")  # shape beyond the device limits
-    with ctx:
-        ctx.propagate()
-        t = []
-        for _ in range(reps):
-            t.append(ctx.propagate()["march_ms"])
+    ctx.propagate()
+    t = []
+    for _ in range(reps):
+        t.append(ctx.propagate()["march_ms"])
     return float(np.median(t))
 
 
@@ -148,6 +150,24 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
     x0 = next((x for x in xs if _lds(Mx, x[0], x[1], c, 2, True) <= 64 * 1024), xs[0])
     y0 = next((y for y in ys if _lds(My, y[0], y[1], c, 1, True) <= 64 * 1024), ys[0])
     best = {"x_chunks": x0[0], "x_rows": x0[1], "y_chunks": y0[0], "y_cols": y0[1]}
+    ctx = bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, use_graph=True, tuning=best)
+    try:
+        best, best_ms, table, t0 = _search(ctx, best, xs, ys, Mx, My, precision, reps)
+    finally:
+        ctx.close()
+    if log:
+        log("autotune %s: %d candidates in %.1f s, best %s (%.3f ms for %d slabs)"
+            % (key, len(table), time.time() - t0, best, best_ms, depth - 1))
+        for c, ms in sorted(table, key=lambda cm: cm[1])[:8]:
+            log("   %.3f ms  %s" % (ms, c))
+    if use_cache:
+        cache[key] = {"tuning": best, "table": [[c, m] for c, m in table]}
+        _save_cache(cache)
+    return best, table
+
+
+def _search(ctx, best, xs, ys, Mx, My, precision, reps):
+    """Coordinate descent over the launch shapes on one resident job."""
     table = []
     t0 = time.time()
     best_ms = None
@@ -156,13 +176,13 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
     for both, shapes in ((0, xs), (1, xs2), (3, xs2)):
         for (xc, xr) in shapes:
             cand = dict(best, x_chunks=xc, x_rows=xr, x_both=both)
-            ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+            ms = _time(ctx, cand, reps)
             table.append((cand, ms))
             if best_ms is None or ms < best_ms:
                 best, best_ms = cand, ms
     for (yc, yw) in ys:
         cand = dict(best, y_chunks=yc, y_cols=yw)
-        ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+        ms = _time(ctx, cand, reps)
         table.append((cand, ms))
         if ms < best_ms:
             best, best_ms = cand, ms
@@ -171,7 +191,7 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
         if yw < 4:
             continue
         cand = dict(best, y_chunks=yc, y_cols=yw, y_pair=1)
-        ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+        ms = _time(ctx, cand, reps)
         table.append((cand, ms))
         if ms < best_ms:
             best, best_ms = cand, ms
@@ -181,7 +201,7 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
             cand = dict(best, x_chunks=xc, x_rows=xr, x_both=both)
             if cand == best:
                 continue
-            ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+            ms = _time(ctx, cand, reps)
             table.append((cand, ms))
             if ms < best_ms:
                 best, best_ms = cand, ms
@@ -193,7 +213,7 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
         if (xc, cand["x_rows"]) not in xok or (yc, cand["y_cols"]) not in y_candidates(My, precision,
                                                                                       pair=bool(cand.get("y_pair"))):
             continue
-        ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+        ms = _time(ctx, cand, reps)
         table.append((cand, ms))
         if ms < best_ms:
             best, best_ms = cand, ms
@@ -208,16 +228,8 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
                 cand = dict(best, x_rows=xr, x_both=both, order=order)
                 if cand == best:
                     continue
-                ms = _time(js, lc, n_E, precision, wl_batch, cand, reps)
+                ms = _time(ctx, cand, reps)
                 table.append((cand, ms))
                 if ms < best_ms:
                     best, best_ms = cand, ms
-    if log:
-        log("autotune %s: %d candidates in %.1f s, best %s (%.3f ms for %d slabs)"
-            % (key, len(table), time.time() - t0, best, best_ms, depth - 1))
-        for c, ms in sorted(table, key=lambda cm: cm[1])[:8]:
-            log("   %.3f ms  %s" % (ms, c))
-    if use_cache:
-        cache[key] = {"tuning": best, "table": [[c, m] for c, m in table]}
-        _save_cache(cache)
-    return best, table
+    return best, best_ms, table, t0

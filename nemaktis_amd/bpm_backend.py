@@ -152,6 +152,14 @@ class Context:
             raise BpmBackendError(err.value.decode())
         return st.as_dict()
 
+    def set_tuning(self, tuning):
+        """Switches the line-solver launch shapes (dict as for ``tuning=``; None: built-in shapes)."""
+        opts = _lib.options(tuning=tuning)
+        err = _lib.err_buf()
+        rc = self.lib.nmk_bpm_set_tuning(self.h, ctypes.byref(opts), err, len(err))
+        if rc:
+            raise BpmBackendError(err.value.decode())
+
     def download(self, E):
         _check_E(E)
         err = _lib.err_buf()

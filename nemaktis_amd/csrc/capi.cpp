@@ -236,6 +236,13 @@ int nmk_bpm_propagate(nmk_bpm_ctx *ctx, nmk_bpm_stats *stats, char *err, size_t 
     });
 }
 
+int nmk_bpm_set_tuning(nmk_bpm_ctx *ctx, const nmk_bpm_options *opts, char *err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!ctx) throw BpmError("null context");
+        ctx->runner->retune(to_options(opts).tuning);
+    });
+}
+
 int nmk_bpm_download(nmk_bpm_ctx *ctx, double *E, int64_t n_E, char *err, size_t errlen) {
     return guarded(err, errlen, [&] {
         if (!ctx) throw BpmError("null context");

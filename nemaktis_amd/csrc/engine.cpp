@@ -533,24 +533,35 @@ private:
         return cur;
     }
 
+    // validates t, then switches to it (the runner keeps its state when t is
+    // refused); the x-line row buffer only grows, so that re-tuning a runner
+    // does not free buffers a captured graph has used
     void set_tuning(const Tuning &t) {
-        tune_ = t;
-        if (g_.px) lpx_ = make_line_plan(g_.Mx, t.x_chunks);
-        if (g_.py) lpy_ = make_line_plan(g_.My, t.y_chunks);
-        if (g_.px) {
-            if ((t.x_both ? 4 : 2) * lpx_.P * t.x_rows > 512 ||
-                xsolve_lds_bytes<T>(lpx_, t.x_rows, t.x_both ? 2 : 1) > max_block_lds())
-                throw std::runtime_error("x line solver tuning exceeds the block limits");
-            xt_.alloc(2 * (size_t)sz_.nslots_max * 2 * 3 * g_.My * lpx_.lmax * lpx_.P);
-        }
-        if (g_.py && ((t.y_pair ? 2 : 1) * lpy_.P * t.y_cols > 512 ||
-                      ysolve_lds_bytes<T>(lpy_, t.y_cols, t.y_pair) > max_block_lds()))
+        const LinePlan lx = g_.px ? make_line_plan(g_.Mx, t.x_chunks) : lpx_;
+        const LinePlan ly = g_.py ? make_line_plan(g_.My, t.y_chunks) : lpy_;
+        if (g_.px && ((t.x_both ? 4 : 2) * lx.P * t.x_rows > 512 ||
+                      xsolve_lds_bytes<T>(lx, t.x_rows, t.x_both ? 2 : 1) > max_block_lds()))
+            throw std::runtime_error("x line solver tuning exceeds the block limits");
+        if (g_.py && ((t.y_pair ? 2 : 1) * ly.P * t.y_cols > 512 ||
+                      ysolve_lds_bytes<T>(ly, t.y_cols, t.y_pair) > max_block_lds()))
             throw std::runtime_error("y line solver tuning exceeds the block limits");
+        if (g_.px) {
+            const size_t need = 2 * (size_t)sz_.nslots_max * 2 * 3 * g_.My * lx.lmax * lx.P;
+            if (need > xt_.count) xt_.alloc(need);
+        }
+        tune_ = t;
+        lpx_ = lx;
+        lpy_ = ly;
         for (auto &kv : graphs_) {
             (void)hipGraphExecDestroy(kv.second.exec);
             (void)hipGraphDestroy(kv.second.graph);
         }
         graphs_.clear();
+    }
+
+    void retune(const Tuning &t) override {
+        HIPCHK(hipStreamSynchronize(st_));
+        set_tuning(t.x_chunks > 0 ? t : default_tuning(o_.precision, g_.Mx, g_.My));
     }
 
     void output_batch(int w0, int nslots, int nprops, const C2<T> *state) {

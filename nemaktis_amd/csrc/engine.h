@@ -70,6 +70,9 @@ public:
     // device copy of the output (caller layout), valid after propagate
     virtual const c64 *device_output() const = 0;
     virtual int64_t footprint() const = 0;
+    // new line-solver launch shapes (throws, keeping the old ones, when the
+    // device cannot run them)
+    virtual void retune(const Tuning &t) = 0;
     virtual hipStream_t stream() const = 0;
 };
 

@@ -227,6 +227,31 @@ def test_both_component_blocks_with_lds_reduced_systems():
     assert _rel(a, b) < 1e-12
 
 
+def test_retuned_context_matches_fresh_context():
+    """nmk_bpm_set_tuning on a resident job (graph replay) gives what a
+    context created with that tuning gives, and a refused shape leaves the
+    context usable."""
+    s, lc = _random_case((40, 36, 5), seed=29)
+    js = json.dumps(s)
+    shape = fields.output_shape(s)
+    n_E = int(np.prod(shape))
+    t1 = {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8}
+    t2 = {"x_chunks": 8, "x_rows": 1, "y_chunks": 8, "y_cols": 4, "x_both": 3, "y_pair": 2, "order": 5}
+    with bpm_backend.Context(js, lc, n_E, precision=32, wl_batch=0, use_graph=True, tuning=t1) as ctx:
+        ctx.propagate()
+        ctx.set_tuning(t2)
+        ctx.propagate()
+        got = ctx.download(np.empty(shape, np.complex128))
+        with pytest.raises(bpm_backend.BpmBackendError, match="block limits"):
+            ctx.set_tuning({"x_chunks": 16, "x_rows": 64, "y_chunks": 16, "y_cols": 8})
+        ctx.propagate()
+        again = ctx.download(np.empty(shape, np.complex128))
+    with bpm_backend.Context(js, lc, n_E, precision=32, wl_batch=0, use_graph=True, tuning=t2) as ctx:
+        ctx.propagate()
+        want = ctx.download(np.empty(shape, np.complex128))
+    assert np.array_equal(got, want) and np.array_equal(again, want)
+
+
 def test_xcd_work_orders_are_bitwise_equal():
     """The XCD work orders only move blocks between XCDs: identical output
     (two wavelengths x two wavevectors, so that every index varies)."""
