@@ -98,6 +98,8 @@ typedef struct nmk_bpm_ctx nmk_bpm_ctx;
 nmk_bpm_ctx *nmk_bpm_create(const char *json_str, const char *mask_formula, const double *lc_field_vals,
                             int64_t n_lc_vals, const double *mask_vals, int64_t n_mask_vals, int64_t n_E_vals,
                             const nmk_bpm_options *opts, char *err_msg, size_t err_len);
+/* input fields of "Beam profile": "None"; a propagation replaces them with
+ * its output (as the reference writes E in place), so upload before each */
 int nmk_bpm_upload_fields(nmk_bpm_ctx *ctx, const double *E_field_vals, int64_t n_E_vals, char *err_msg,
                           size_t err_len);
 int nmk_bpm_propagate(nmk_bpm_ctx *ctx, nmk_bpm_stats *stats, char *err_msg, size_t err_len);

@@ -23,7 +23,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 
 HIP_SOURCES = ["kernels.hip", "viewer_kernels.hip"]
-CXX_SOURCES = ["json.cpp", "expr.cpp", "settings.cpp", "engine.cpp", "server.cpp", "viewer.cpp", "capi.cpp"]
+CXX_SOURCES = ["json.cpp", "expr.cpp", "settings.cpp", "vti.cpp", "engine.cpp", "server.cpp", "viewer.cpp", "capi.cpp"]
 
 
 def _run(cmd):

@@ -268,6 +268,7 @@ int nmk_bpm_estimate_footprint(const char *json, int64_t n_lc, int has_mask, con
             p.qx.push_back(q.first);
             p.qy.push_back(q.second);
         }
+        p.bulk = s.bulk_activate;
         int64_t b = estimate_footprint(p, to_options(opts));
         if (has_mask) {
             // the mask and (upper bound) the interface weights of a mask formula

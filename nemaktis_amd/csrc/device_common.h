@@ -115,6 +115,7 @@ struct PropParams {
     double qx, qy;
     double ex, ey;        // input polarisation (cos, sin of pol*pi/2)
     double ph_re, ph_im;  // accumulated global phase of all shift half-steps
+    double ph_half;       // phase angle of one shift half-step, -dZ (qx^2 + qy^2) / (4 nref)
 };
 
 // chunking of one periodic line of M unknowns into P chunks (P a power of

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "vti.h"
 
 namespace nmk {
 
@@ -111,7 +112,7 @@ public:
             own_stream_ = true;
         }
         sz_ = sizes_of(p, o);
-        bind_source(o.allocator, lc_, weights_, q_, iso_t_, iso_n_, partials_, sm_, mask_, twx_, twy_, E_, sp_, pp_, cf_,
+        bind_source(o.allocator, bulk_, bt_, lc_, weights_, q_, iso_t_, iso_n_, partials_, sm_, mask_, twx_, twy_, E_, sp_, pp_, cf_,
                     xt_, fb_[0], fb_[1], fb_[2], fb_[3]);
         g_.Nx = p.Nx;
         g_.Ny = p.Ny;
@@ -175,6 +176,12 @@ public:
         set_tuning(o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, g_.Mx, g_.My));
         for (auto &b : fb_) b.alloc((size_t)sz_.nprops_max * 2 * sz_.NP);
         E_.alloc((size_t)sz_.nwl * sz_.nq * 4 * sz_.Nfull);
+        if (p.bulk) {
+            // every plane of every propagation stays on the device until the
+            // run writes its file (the reference holds them in host memory)
+            bulk_.alloc((size_t)sz_.nwl * sz_.nq * 2 * p.Nz * 2 * sz_.Nfull);
+            bt_.alloc((size_t)sz_.nprops_max * 2 * sz_.NP);
+        }
         HIPCHK(hipStreamSynchronize(st_));
     }
 
@@ -199,7 +206,8 @@ public:
     int64_t footprint() const override {
         size_t b = lc_.footprint() + weights_.footprint() + q_.footprint() + iso_t_.footprint() + iso_n_.footprint() +
                    partials_.footprint() + sm_.footprint() + mask_.footprint() + twx_.footprint() + twy_.footprint() +
-                   E_.footprint() + sp_.footprint() + pp_.footprint() + cf_.footprint() + xt_.footprint();
+                   E_.footprint() + sp_.footprint() + pp_.footprint() + cf_.footprint() + xt_.footprint() +
+                   bulk_.footprint() + bt_.footprint();
         for (auto &f : fb_) b += f.footprint();
         return (int64_t)b;
     }
@@ -243,6 +251,7 @@ public:
         }
         HIPCHK(hipEventRecord(e1, st_));
         HIPCHK(hipEventSynchronize(e1));
+        if (p_.bulk) write_bulk();
         if (stats) {
             float t;
             HIPCHK(hipEventElapsedTime(&t, e0, e1));
@@ -257,6 +266,25 @@ public:
     }
 
 private:
+    // the bulk planes to <Results folder name>/<Base name>.vti
+    // (PythonBackend/bpm_backend.cpp:44-47, PostprocessorLib/VolumeOutput.cpp:19-32)
+    void write_bulk() {
+        BulkGeometry bg;
+        bg.Nx = p_.Nx;
+        bg.Ny = p_.Ny;
+        bg.Nz = p_.Nz;
+        bg.dx = p_.dx;
+        bg.dy = p_.dy;
+        bg.dz = p_.dz;
+        bg.wl = p_.wl;
+        bg.qx = p_.qx;
+        bg.qy = p_.qy;
+        const size_t blk = (size_t)p_.Nz * 2 * sz_.Nfull;
+        write_bulk_vti(p_.bulk_path, p_.bulk_dir, bg, [&](size_t k, c64 *dst) {
+            HIPCHK(hipMemcpy(dst, bulk_.ptr + k * blk, blk * sizeof(c64), hipMemcpyDeviceToHost));
+        });
+    }
+
     Problem p_;
     Options o_;
     Sizes sz_;
@@ -273,6 +301,8 @@ private:
     DevBuf<PropParams> pp_;
     DevBuf<C2<T>> cf_, xt_;
     DevBuf<C2<T>> fb_[4];
+    DevBuf<c64> bulk_;   // bulk output planes (p.bulk)
+    DevBuf<C2<T>> bt_;   // one boundary-shifted plane of the bulk output
     LinePlan lpx_{}, lpy_{};
     Tuning tune_{};
     std::vector<SlotParams> hsp_;
@@ -395,6 +425,7 @@ private:
                     }
                     compose_taps(pp.wx, pp.ox, pp.w1x, pp.o1x, pp.w2x, pp.o2x);
                     compose_taps(pp.wy, pp.oy, pp.w1y, pp.o1y, pp.w2y, pp.o2y);
+                    pp.ph_half = -sp.dZ * (qx * qx + qy * qy) / (4 * nref);
                     const double phi = -sp.dZ * (qx * qx + qy * qy) / (4 * nref) * 2.0 * (p.Nz - 1);
                     pp.ph_re = std::cos(phi);
                     pp.ph_im = std::sin(phi);
@@ -483,6 +514,12 @@ private:
         // components, larger batches across neighbouring rows/strips (measured)
         const int ord = tune_.order >= 0 ? tune_.order : (nprops <= 2 ? 3 : 4);
         HIPCHK(hipStreamWaitEvent(st_, ready[0], 0));
+        // bulk plane 0: the entrance fields after the input interface
+        if (bulk_.ptr) {
+            const int beam = p_.beam == BeamKind::None ? 2 : p_.beam == BeamKind::Gaussian ? 1 : 0;
+            launch_bulk_entrance(lc_.ptr, mask_.ptr, g_, sp_.ptr, pp_.ptr, nprops, beam, p_.waist, p_.dx, p_.dy,
+                                 E_.ptr, sz_.nq, bulk_.ptr, st_);
+        }
         timed(KF_BOUNDARY, [&] { launch_boundary<T>(other, cur, nullptr, CF(0), g_, pp_.ptr, nprops, 1, st_); });
         for (int k = 0; k < nsl; k++) {
             // x half-steps: other = (I - mu A_x)^-1 (I + mu A_y) cur by its Neumann series
@@ -514,6 +551,11 @@ private:
                 }
             } else {
                 timed(KF_EXPLICIT, [&] { launch_explicit<T>(0, other, cur, CF(k), g_, pp_.ptr, nprops, st_); });
+            }
+            // bulk plane k+1: the step's closing P_k, Sh applied to a copy
+            if (bulk_.ptr) {
+                launch_boundary<T>(cur, bt_.ptr, CF(k), nullptr, g_, pp_.ptr, nprops, 1, st_);
+                launch_bulk_plane<T>(bt_.ptr, g_, sp_.ptr, pp_.ptr, nprops, k + 1, sz_.nq, bulk_.ptr, st_);
             }
             // P_k, Sh | Sh, P_{k+1}
             if (k + 1 < nsl) {
@@ -640,6 +682,10 @@ int64_t estimate_footprint(const Problem &p, const Options &o) {
     }
     for (int i = 0; i < 4; i++) add((int64_t)s.nprops_max * 2 * s.NP * fsz);
     add((int64_t)s.nwl * s.nq * 4 * s.Nfull * 16);
+    if (p.bulk) {
+        add((int64_t)s.nwl * s.nq * 2 * p.Nz * 2 * s.Nfull * 16);
+        add((int64_t)s.nprops_max * 2 * s.NP * fsz);
+    }
     return b;
 }
 

@@ -75,6 +75,18 @@ template <class T>
 void launch_output(const C2<T> *F, const double *lc, const uint8_t *mask, const Geo &g, const SlotParams *sp,
                    const PropParams *pp, int nprops, int nq, c64 *E, hipStream_t st);
 
+// plane 0 of the bulk output on the full grid: beam through the input
+// interface (beam 0 uniform, 1 Gaussian) or the caller's fields Ein (2)
+void launch_bulk_entrance(const double *lc, const uint8_t *mask, const Geo &g, const SlotParams *sp,
+                          const PropParams *pp, int nprops, int beam, double waist, double dx, double dy,
+                          const c64 *Ein, int nq, c64 *bulk, hipStream_t st);
+// one transverse plane of the bulk output: march state -> full grid with the
+// phase of 2*plane shift half-steps, into
+// bulk[(((iwl*nq + iq)*2 + pol)*Nz + plane)*2 + comp][m] (complex128)
+template <class T>
+void launch_bulk_plane(const C2<T> *F, const Geo &g, const SlotParams *sp, const PropParams *pp, int nprops,
+                       int plane, int nq, c64 *bulk, hipStream_t st);
+
 // screen filter: batched 2-D FFT of howmany planes (Ny x Nx) and the per-k
 // 2x2 transfer filter
 // radices of the stages, 4 bits each from the lowest (at most 16 stages)

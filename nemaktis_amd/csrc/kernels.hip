@@ -367,6 +367,26 @@ void launch_coefs(const double *sm, const Geo &g, const SlotParams *sp, int nslo
 // input beam (PhysicsLib/BeamProfile.cpp:17-39) + input Fresnel
 // (BeamPropagationLib/FresnelOperator.cpp:16-38), written to the march grid
 // ---------------------------------------------------------------------------
+// beam of polarisation (p.ex, p.ey) at full-grid point (ix, iy) through the
+// input interface
+__device__ __forceinline__ void entrance_field(const double *__restrict__ lc, const uint8_t *__restrict__ mask,
+                                               const Geo &g, const SlotParams &s, const PropParams &p, int ix,
+                                               int iy, int gaussian, double waist, double dx, double dy,
+                                               double &a, double &b) {
+    double e[12];
+    eps_voxel(lc, mask, g.field_dim, ix + (int64_t)g.Nx * iy, s, e);
+    double gx = 1.0;
+    if (gaussian) {
+        double x = (ix - (g.Nx - 1) / 2.) * dx, y = (iy - (g.Ny - 1) / 2.) * dy;
+        gx = exp(-(x * x + y * y) / (2 * waist * waist));
+    }
+    const double E0 = p.ex * gx, E1 = p.ey * gx;
+    const double n = s.nin, nxx = e[9], nyy = e[10], nxy = e[11];
+    const double det = (n + nxx) * (n + nyy) - nxy * nxy;
+    a = 2 * n * (n + nyy) / det * E0 - 2 * n * nxy / det * E1;
+    b = -2 * n * nxy / det * E0 + 2 * n * (n + nxx) / det * E1;
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_init_fields(const double *__restrict__ lc, const uint8_t *__restrict__ mask,
                                                      Geo g, const SlotParams *__restrict__ sp,
@@ -378,20 +398,47 @@ __global__ __launch_bounds__(256) void k_init_fields(const double *__restrict__ 
     const PropParams p = pp[prop];
     const SlotParams s = sp[p.slot];
     const int ix = (int)(ip % g.Mx), iy = (int)(ip / g.Mx);
-    double e[12];
-    eps_voxel(lc, mask, g.field_dim, ix + (int64_t)g.Nx * iy, s, e);
-    double gx = 1.0;
-    if (gaussian) {
-        double x = (ix - (g.Nx - 1) / 2.) * dx, y = (iy - (g.Ny - 1) / 2.) * dy;
-        gx = exp(-(x * x + y * y) / (2 * waist * waist));
-    }
-    const double E0 = p.ex * gx, E1 = p.ey * gx;
-    const double n = s.nin, nxx = e[9], nyy = e[10], nxy = e[11];
-    const double det = (n + nxx) * (n + nyy) - nxy * nxy;
-    const double a = 2 * n * (n + nyy) / det * E0 - 2 * n * nxy / det * E1;
-    const double b = -2 * n * nxy / det * E0 + 2 * n * (n + nxx) / det * E1;
+    double a, b;
+    entrance_field(lc, mask, g, s, p, ix, iy, gaussian, waist, dx, dy, a, b);
     F[((int64_t)prop * 2 + 0) * g.NP + ip] = C2<T>{(T)a, (T)0};
     F[((int64_t)prop * 2 + 1) * g.NP + ip] = C2<T>{(T)b, (T)0};
+}
+
+// plane 0 of the bulk output on the full grid, duplicate column/row included
+// (BPMIteration.cpp:88-120): the beam through the input interface at every
+// point (beam 0 uniform, 1 Gaussian), or the caller's fields as given (2)
+__global__ __launch_bounds__(256) void k_bulk_entrance(const double *__restrict__ lc,
+                                                       const uint8_t *__restrict__ mask, Geo g,
+                                                       const SlotParams *__restrict__ sp,
+                                                       const PropParams *__restrict__ pp, int beam, double waist,
+                                                       double dx, double dy, const c64 *__restrict__ Ein, int nq,
+                                                       c64 *__restrict__ bulk) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.Nfull) return;
+    const PropParams p = pp[blockIdx.y];
+    const SlotParams s = sp[p.slot];
+    c64 a, b;
+    if (beam == 2) {
+        const c64 *e = Ein + (((int64_t)s.iwl * nq + p.iq) * 4 + 2 * p.pol) * g.Nfull;
+        a = e[m];
+        b = e[g.Nfull + m];
+    } else {
+        double ar, br;
+        entrance_field(lc, mask, g, s, p, (int)(m % g.Nx), (int)(m / g.Nx), beam == 1, waist, dx, dy, ar, br);
+        a = mk(ar, 0.0);
+        b = mk(br, 0.0);
+    }
+    c64 *o = bulk + (((int64_t)s.iwl * nq + p.iq) * 2 + p.pol) * g.Nz * 2 * g.Nfull;
+    o[m] = a;
+    o[g.Nfull + m] = b;
+}
+
+void launch_bulk_entrance(const double *lc, const uint8_t *mask, const Geo &g, const SlotParams *sp,
+                          const PropParams *pp, int nprops, int beam, double waist, double dx, double dy,
+                          const c64 *Ein, int nq, c64 *bulk, hipStream_t st) {
+    hipLaunchKernelGGL(k_bulk_entrance, dim3(blocks_for(g.Nfull, 256), nprops), dim3(256), 0, st, lc, mask, g, sp,
+                       pp, beam, waist, dx, dy, Ein, nq, bulk);
+    NMK_LAUNCH_CHECK();
 }
 
 template <class T>
@@ -1818,6 +1865,43 @@ void launch_output(const C2<T> *F, const double *lc, const uint8_t *mask, const 
 }
 
 // ---------------------------------------------------------------------------
+// one transverse plane of the bulk output (BPMIteration.cpp:111-120, 153-168):
+// march state -> full grid (duplicate column/row copied) with the phase of the
+// 2*plane shift half-steps applied so far, no interface;
+// bulk[(((iwl*nq + iq)*2 + pol)*Nz + plane)*2 + comp][m]
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_bulk_plane(const C2<T> *__restrict__ F, Geo g,
+                                                    const SlotParams *__restrict__ sp,
+                                                    const PropParams *__restrict__ pp, int plane, int nq,
+                                                    c64 *__restrict__ bulk) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.Nfull) return;
+    const int prop = blockIdx.y;
+    const PropParams p = pp[prop];
+    const int iwl = sp[p.slot].iwl;
+    const int ix = (int)(m % g.Nx), iy = (int)(m / g.Nx);
+    const int jx = g.px ? ix % g.Mx : ix, jy = g.py ? iy % g.My : iy;
+    const int64_t ip = jx + (int64_t)g.Mx * jy;
+    double s, c;
+    sincos(p.ph_half * 2.0 * plane, &s, &c);
+    const c64 ph = mk(c, s);
+    const c64 a = ph * cast2<double>(F[((int64_t)prop * 2) * g.NP + ip]);
+    const c64 b = ph * cast2<double>(F[((int64_t)prop * 2 + 1) * g.NP + ip]);
+    c64 *o = bulk + ((((int64_t)iwl * nq + p.iq) * 2 + p.pol) * g.Nz + plane) * 2 * g.Nfull;
+    o[m] = a;
+    o[g.Nfull + m] = b;
+}
+
+template <class T>
+void launch_bulk_plane(const C2<T> *F, const Geo &g, const SlotParams *sp, const PropParams *pp, int nprops,
+                       int plane, int nq, c64 *bulk, hipStream_t st) {
+    hipLaunchKernelGGL(k_bulk_plane<T>, dim3(blocks_for(g.Nfull, 256), nprops), dim3(256), 0, st, F, g, sp, pp,
+                       plane, nq, bulk);
+    NMK_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
 // screen filter FFT: rows as one line per workgroup, columns as C adjacent
 // columns per workgroup (rows of C consecutive values in HBM), both with the
 // Stockham stages of fft_device.h
@@ -2066,6 +2150,8 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
                                      const PropParams *, int, int, hipStream_t);                                      \
     template void launch_explicit<T>(int, const C2<T> *, C2<T> *, const C2<T> *, const Geo &, const PropParams *,    \
                                      int, hipStream_t);                                                               \
+    template void launch_bulk_plane<T>(const C2<T> *, const Geo &, const SlotParams *, const PropParams *, int, \
+                                       int, int, c64 *, hipStream_t);                                           \
     template void launch_xsolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const C2<T> *, const Geo &, \
                                    const PropParams *, int, const LinePlan &, int, int, int, int, hipStream_t);       \
     template void launch_ysolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const Geo &,                  \

@@ -272,6 +272,14 @@ Problem build_problem(const Settings &s, const double *lc, int64_t n_lc, const d
     p.z_foc = s.z_shift;
     for (size_t i = 0; i < p.iso_thick.size(); i++) p.z_foc -= p.iso_thick[i] * p.nout_ref / p.iso_index[i];
     p.NA = s.NA;
+    // PythonBackend/bpm_backend.cpp:44-47: the bulk fields go to
+    // "<Results folder name>/<Base name>.vti" (the folder is used unless "" or ".")
+    p.bulk = s.bulk_activate;
+    if (p.bulk) {
+        const bool dir = !s.results_folder.empty() && s.results_folder != ".";
+        p.bulk_dir = dir ? s.results_folder : "";
+        p.bulk_path = (dir ? s.results_folder + "/" : std::string()) + s.bulk_name + ".vti";
+    }
     // PhysicsLib/PhysicsCoefficients.cpp:27-103, evaluated for every wavelength up front
     for (double l : p.wl) {
         p.no.push_back(eval_index(s.no_expr, l, "no"));

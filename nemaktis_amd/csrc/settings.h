@@ -58,6 +58,10 @@ struct Problem {
     int n_woodbury = 2;
     std::vector<double> iso_thick, iso_index;
     double z_foc = 0, NA = 0;  // z_foc already corrected for the layers
+    // bulk output: every transverse plane of the march written as a VTI file
+    // (PostprocessorLib/VolumeOutput.cpp:19-32, PostprocessorBase.cpp:24-35)
+    bool bulk = false;
+    std::string bulk_dir, bulk_path;
     // inputs (borrowed)
     const double *lc = nullptr;
     std::vector<uint8_t> mask;      // empty: every voxel is LC

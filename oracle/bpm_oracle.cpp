@@ -804,8 +804,17 @@ struct Oracle {
     }
 
     // BPMIteration::propagate_fields (BPL/BPMIteration.cpp:53-177) + screen filter
-    int propagate(cd *E) {
+    // bulk (optional): the field of every transverse plane, saved after the
+    // input interface (plane 0) and after each step (plane z+1), before the
+    // output interface (BPMIteration.cpp:111-120, 153-168); layout
+    // [w][q][pol][plane][comp][m]
+    int propagate(cd *E, cd *bulk = nullptr) {
         const size_t stride = 2 * (size_t)N;
+        auto save = [&](int w, int iq, int pol, int plane, const cd *f) {
+            if (!bulk) return;
+            cd *b = bulk + ((((size_t)w * p.n_q + iq) * 2 + pol) * Nz + plane) * stride;
+            std::memcpy(b, f, sizeof(cd) * stride);
+        };
         for (int w = 0; w < p.n_wl; w++) {
             set_wavelength(w);
             std::vector<Shift> shifts;
@@ -830,6 +839,9 @@ struct Oracle {
                     }
                     fresnel_in(f, p.nin[w]);
                 }
+            for (int pol = 0; pol < 2; pol++)
+                for (int iq = 0; iq < p.n_q; iq++)
+                    save(w, iq, pol, 0, E + (size_t)(pol + 2 * (iq + p.n_q * w)) * stride);
             for (int z = 0; z < Nz - 1; z++) {
                 update(z);
                 for (int pol = 0; pol < 2; pol++)
@@ -840,6 +852,7 @@ struct Oracle {
                         apply_primary(f);
                         apply_phase(f);
                         apply_shift(shifts[iq], f);
+                        save(w, iq, pol, z + 1, f);
                     }
             }
             for (int pol = 0; pol < 2; pol++)
@@ -877,6 +890,14 @@ int oracle_propagate(const OracleParams *p, const double *lc, double *E) {
     ThreadScope ts(p->nthreads);
     Oracle o(p, lc);
     return o.propagate(reinterpret_cast<cd *>(E));
+}
+
+int oracle_propagate_bulk(const OracleParams *p, const double *lc, double *E, double *bulk) {
+    int rc = validate(p);
+    if (rc) return rc;
+    ThreadScope ts(p->nthreads);
+    Oracle o(p, lc);
+    return o.propagate(reinterpret_cast<cd *>(E), reinterpret_cast<cd *>(bulk));
 }
 
 void *oracle_create(const OracleParams *p, const double *lc) {

@@ -68,6 +68,9 @@ typedef struct {
  * (re, im).  With beam == 2 it holds the caller's input fields on entry.
  * Returns 0 on success, a negative code on invalid input. */
 int oracle_propagate(const OracleParams *p, const double *lc, double *E);
+/* the same, also returning every transverse plane of the march (bulk output):
+ * bulk[w][q][pol][plane][comp][m] complex128, planes 0..Nz-1 */
+int oracle_propagate_bulk(const OracleParams *p, const double *lc, double *E, double *bulk);
 
 /* ---- operator-level access for unit tests (one wavelength at a time) ---- */
 void *oracle_create(const OracleParams *p, const double *lc);

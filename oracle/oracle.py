@@ -55,6 +55,7 @@ def lib():
         P = ctypes.POINTER(ctypes.c_double)
         vp = ctypes.c_void_p
         _lib.oracle_propagate.argtypes = [ctypes.POINTER(OracleParams), P, P]
+        _lib.oracle_propagate_bulk.argtypes = [ctypes.POINTER(OracleParams), P, P, P]
         _lib.oracle_create.argtypes = [ctypes.POINTER(OracleParams), P]
         _lib.oracle_create.restype = vp
         _lib.oracle_destroy.argtypes = [vp]
@@ -197,6 +198,23 @@ def propagate(settings, lc_vals, E=None, mask=None, weights=None, nthreads=0, ap
     if rc != 0:
         raise ValueError("oracle rejected the input (code %d)" % rc)
     return E
+
+
+def propagate_bulk(settings, lc_vals, E=None, mask=None, weights=None, nthreads=0):
+    """Runs the oracle and returns (E, bulk): bulk of shape (n_wl, n_q, 2, Nz, 2, Ny, Nx) complex128 holds
+    every transverse plane of the march (BPMIteration.cpp:111-120, 153-168)."""
+    p = Params(settings, mask, weights, nthreads, True)
+    Nx, Ny, Nz = p.shape
+    lc = np.ascontiguousarray(lc_vals, dtype=np.float64).reshape(-1)
+    if E is None:
+        E = np.zeros((p.nwl, p.nq, 4, Ny, Nx), dtype=np.complex128)
+    else:
+        E = np.ascontiguousarray(E, dtype=np.complex128).copy()
+    bulk = np.zeros((p.nwl, p.nq, 2, Nz, 2, Ny, Nx), dtype=np.complex128)
+    rc = lib().oracle_propagate_bulk(ctypes.byref(p.c), _dp(lc), _dp(E.view(np.float64)), _dp(bulk.view(np.float64)))
+    if rc != 0:
+        raise ValueError("oracle rejected the input (code %d)" % rc)
+    return E, bulk
 
 
 class Session:

@@ -1,0 +1,100 @@
+"""GPU parity of the bulk (per-plane) output: the VTI file the backend writes
+when "Bulk output" is active, read back and compared with the oracle's planes.
+
+Tolerances: relative L2 per (wavelength, wavevector, polarisation) over all
+planes, FP64 1e-10, FP32 1e-4 (as the output fields).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from nemaktis_amd import bpm_backend, fields, vti
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(dims=(20, 18, 6), seed=3, folder="", name="bulk"):
+    s = fields.make_settings(dims, (0.1, 0.12, 0.1), [0.55, 0.62], wavevectors=[(0.0, 0.0), (0.15, -0.1)],
+                             no="1.5", ne="1.7+0.01/lambda^2", ne_imag="0.01", nin="1.2", nout="1.1",
+                             na_objective=0.7)
+    s["Algorithm settings"]["General"]["Results folder name"] = folder
+    s["Postprocessor settings"]["Bulk output"] = {"Activate": True, "Base name": name}
+    lc = fields.random_smooth(dims[0], dims[1], dims[2], (0.1, 0.12, 0.1), seed=seed, corr_length=0.3)
+    return s, lc
+
+
+def _compare(path, bulk_want, tol):
+    got = vti.bulk_fields(path)
+    nwl, nq = bulk_want.shape[:2]
+    assert sorted(got) == [(w, q) for w in range(nwl) for q in range(nq)]
+    worst = 0.0
+    for (w, q), g in got.items():
+        for pol in range(2):
+            want = np.moveaxis(bulk_want[w, q, pol], 1, -1)          # (Nz, Ny, Nx, 2)
+            err = np.linalg.norm(g[pol] - want) / np.linalg.norm(want)
+            worst = max(worst, err)
+    assert worst < tol, worst
+    return worst
+
+
+def test_bulk_output_fp64_matches_oracle(tmp_path):
+    folder = str(tmp_path / "results")
+    s, lc = _case(folder=folder)
+    print("[gpu case] bulk fp64", flush=True)
+    E = np.zeros(fields.output_shape(s), np.complex128)
+    bpm_backend.run_backend_without_mask(json.dumps(s), lc, E, strict=True)
+    E_want, bulk_want = oracle.propagate_bulk(s, lc)
+    path = os.path.join(folder, "bulk.vti")
+    _compare(path, bulk_want, 1e-10)
+    geo, fd, pd = vti.read_vti(path)
+    assert geo["dims"] == (20, 18, 6)
+    np.testing.assert_allclose(geo["spacing"], (0.1, 0.12, 0.1))
+    np.testing.assert_allclose(geo["origin"], (-0.95, -1.02, -0.25))
+    np.testing.assert_array_equal(fd["lambda"], [0.55, 0.62])
+    np.testing.assert_array_equal(fd["qx"], [0.0, 0.15])
+    np.testing.assert_array_equal(fd["qy"], [0.0, -0.1])
+    assert len(pd) == 2 * 2 * 2 * 2 and np.all(pd["E_imag_inputY_1_1"][..., 2] == 0)
+    err = np.linalg.norm(E - E_want) / np.linalg.norm(E_want)
+    assert err < 1e-10
+
+
+def test_bulk_output_fp32_batched_graph_matches_oracle(tmp_path):
+    s, lc = _case(dims=(24, 22, 5), seed=5, folder=str(tmp_path), name="b32")
+    print("[gpu case] bulk fp32 graph", flush=True)
+    bpm_backend.propagate(json.dumps(s), lc, precision=32, wl_batch=0, use_graph=True)
+    _, bulk_want = oracle.propagate_bulk(s, lc)
+    _compare(str(tmp_path / "b32.vti"), bulk_want, 1e-4)
+
+
+def test_bulk_output_caller_fields_and_context_rewrites(tmp_path):
+    # "Beam profile": "None": plane 0 is the caller's field as given; a resident context writes the
+    # file on every propagation (the output replaces the uploaded fields, so they are uploaded again)
+    s, lc = _case(dims=(16, 14, 4), seed=9, folder=str(tmp_path), name="ctx")
+    s["Physics settings"]["Initial conditions"]["Beam profile"] = "None"
+    rng = np.random.default_rng(2)
+    shape = fields.output_shape(s)
+    E_in = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    _, bulk_want = oracle.propagate_bulk(s, lc, E=E_in)
+    path = str(tmp_path / "ctx.vti")
+    with bpm_backend.Context(json.dumps(s), lc, int(np.prod(shape))) as ctx:
+        ctx.upload_fields(E_in)
+        ctx.propagate()
+        _compare(path, bulk_want, 1e-10)
+        os.remove(path)
+        ctx.upload_fields(E_in)
+        ctx.propagate()
+        _compare(path, bulk_want, 1e-10)
+    g = vti.bulk_fields(path)
+    np.testing.assert_array_equal(g[(0, 0)][1, 0, ..., 0], E_in[0, 0, 2])
+
+
+def test_bulk_output_unwritable_folder_is_reported(tmp_path):
+    blocker = tmp_path / "file"
+    blocker.write_text("x")
+    s, lc = _case(dims=(12, 10, 3), folder=str(blocker / "sub"))
+    E = np.zeros(fields.output_shape(s), np.complex128)
+    with pytest.raises(bpm_backend.BpmBackendError, match="results folder|bulk output file"):
+        bpm_backend.run_backend_without_mask(json.dumps(s), lc, E, strict=True)
