@@ -228,6 +228,33 @@ def imaging_leg(steps, warmup, with_cpu):
     return out
 
 
+def bulk_leg(settings, lc, tuning, folder):
+    """Propagation of the benchmark job with "Bulk output" active (every plane kept in HBM, then written as
+    a VTI file) against the same job without it: device ms, file size and write time."""
+    from nemaktis_amd import bpm_backend, fields
+    s = json.loads(json.dumps(settings))
+    s["Algorithm settings"]["General"]["Results folder name"] = folder
+    s["Postprocessor settings"]["Bulk output"] = {"Activate": True, "Base name": "bench_bulk"}
+    n_E = int(np.prod(fields.output_shape(s)))
+    out = {}
+    for tag, ss in (("without", settings), ("with", s)):
+        with bpm_backend.Context(json.dumps(ss), lc, n_E, precision=32, wl_batch=0, use_graph=True,
+                                 tuning=tuning) as ctx:
+            ctx.propagate()
+            t0 = time.perf_counter()
+            st = ctx.propagate()
+            wall = time.perf_counter() - t0
+        out[tag] = {"device_ms": st["total_ms"], "wall_ms": 1e3 * wall, "footprint_gb": st["footprint_bytes"] / 1e9}
+    path = os.path.join(folder, "bench_bulk.vti")
+    size = os.path.getsize(path)
+    os.remove(path)
+    out["file_gb"] = size / 1e9
+    out["write_s"] = (out["with"]["wall_ms"] - out["with"]["device_ms"]) / 1e3
+    out["write_gbs"] = out["file_gb"] / max(1e-9, out["write_s"])
+    out["device_overhead"] = out["with"]["device_ms"] / out["without"]["device_ms"] - 1
+    return out
+
+
 def rel_errors(a, b):
     a = a.reshape(a.shape[0], a.shape[1], 2, -1)
     b = b.reshape(b.shape[0], b.shape[1], 2, -1)
@@ -293,6 +320,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=1.0, help="mean job arrivals per second")
     ap.add_argument("--serve-active", type=int, default=4, help="jobs on the device at once")
     ap.add_argument("--no-imaging", action="store_true", help="skip the refocus + micrograph leg")
+    ap.add_argument("--no-bulk", action="store_true", help="skip the bulk-output leg")
+    ap.add_argument("--bulk-dir", default="/tmp/nemaktis_amd_bench", help="folder of the bulk-output file")
     ap.add_argument("--probe", choices=["copy", "march", "imaging"], help="short run for a hardware-counter pass")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -376,6 +405,11 @@ def main():
         log("refocus + micrograph leg ...")
         imaging = imaging_leg(args.steps, args.warmup, not args.no_cpu)
 
+    bulk = None
+    if not args.no_bulk:
+        log("bulk-output leg ...")
+        bulk = bulk_leg(s, lc, tuning, args.bulk_dir)
+
     serve = None
     if not args.no_serve:
         log("serving loop trace ...")
@@ -395,6 +429,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": serve,
         "imaging": imaging,
+        "bulk_output": bulk,
     }
     if cpu:
         line["vs_cpu"] = value / cpu["value"]
