@@ -41,19 +41,22 @@ def algorithmic_bytes(fam, c, nprops, nslots, NP, Nfull, fd, nw):
     element it reads or writes counted once (c = bytes per complex value,
     NP march-grid points, Nfull full-grid points).
       line solves, first launch of a half-step: per propagation the source
-      field (2 components) in, dst and acc (2 components each) out; per
+      field (2 components) in, the solution (2 components) out; per
       wavelength slot 20 right-hand-side coefficient planes + 6 line-matrix
-      planes.  Neumann launches: source in, dst out, acc in and out (8
-      planes per propagation); 14 + 6 coefficient planes per slot.  The
-      family average weighs one first launch and nw Neumann launches."""
+      planes.  Neumann launch i of nw: source in, the sum so far in (i > 0;
+      launch 0 adds the source itself), the sum out, the solution out for the
+      next launch (i < nw - 1); 14 + 6 coefficient planes per slot.  The
+      family average weighs one first launch and the nw Neumann launches."""
     if fam == "boundary":
         return 4 * nprops * NP * c + 6 * nslots * NP * c
     if fam == "explicit":
         return 4 * nprops * NP * c + 20 * nslots * NP * c
     if fam in ("solve_x", "solve_y"):
-        init = (6 * nprops + 26 * nslots) * NP * c
-        neu = (8 * nprops + 20 * nslots) * NP * c
-        return (init + nw * neu) / (1 + nw)
+        total = (4 * nprops + 26 * nslots) * NP * c
+        for i in range(nw):
+            planes = 2 + (2 if i > 0 else 0) + 2 + (2 if i + 1 < nw else 0)
+            total += (planes * nprops + 20 * nslots) * NP * c
+        return total / (1 + nw)
     if fam == "coefs":
         return 15 * 8 * nslots * Nfull + 49 * nslots * NP * c
     if fam == "material":

@@ -523,14 +523,16 @@ private:
         timed(KF_BOUNDARY, [&] { launch_boundary<T>(other, cur, nullptr, CF(0), g_, pp_.ptr, nprops, 1, st_); });
         for (int k = 0; k < nsl; k++) {
             // x half-steps: other = (I - mu A_x)^-1 (I + mu A_y) cur by its Neumann series
+            // y0 + y1 + ... + y_nw: y0 goes to A, the first Neumann launch adds it to y1
+            // into other, the last one writes only the sum (in place)
             if (g_.px) {
                 timed(KF_SOLVE_X, [&] {
-                    launch_xsolve<T>(0, cur, A, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, tune_.lean, tune_.x_both & 1, st_);
+                    launch_xsolve<T>(0, cur, nw > 0 ? A : other, nullptr, nullptr, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, tune_.lean, tune_.x_both & 1, st_);
                 });
                 C2<T> *a = A, *b = B;
                 for (int it = 0; it < nw; it++) {
                     timed(KF_SOLVE_X, [&] {
-                        launch_xsolve<T>(1, a, b, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, tune_.lean, (tune_.x_both >> 1) & 1, st_);
+                        launch_xsolve<T>(1, a, it + 1 < nw ? b : nullptr, it == 0 ? a : other, other, CF(k), XT(k), g_, pp_.ptr, nprops, lpx_, tune_.x_rows, ord, tune_.lean, (tune_.x_both >> 1) & 1, st_);
                     });
                     std::swap(a, b);
                 }
@@ -540,12 +542,12 @@ private:
             // y half-steps: cur = (I - mu A_y)^-1 (I + mu A_x) other by its Neumann series
             if (g_.py) {
                 timed(KF_SOLVE_Y, [&] {
-                    launch_ysolve<T>(0, other, A, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, tune_.lean, tune_.y_pair, st_);
+                    launch_ysolve<T>(0, other, nw > 0 ? A : cur, nullptr, nullptr, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, tune_.lean, tune_.y_pair, st_);
                 });
                 C2<T> *a = A, *b = B;
                 for (int it = 0; it < nw; it++) {
                     timed(KF_SOLVE_Y, [&] {
-                        launch_ysolve<T>(1, a, b, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, tune_.lean, tune_.y_pair, st_);
+                        launch_ysolve<T>(1, a, it + 1 < nw ? b : nullptr, it == 0 ? a : cur, cur, CF(k), g_, pp_.ptr, nprops, lpy_, tune_.y_cols, ord, tune_.lean, tune_.y_pair, st_);
                     });
                     std::swap(a, b);
                 }

@@ -61,14 +61,17 @@ template <class T>
 size_t xsolve_lds_bytes(const LinePlan &lp, int R, int nc);
 template <class T>
 size_t ysolve_lds_bytes(const LinePlan &lp, int W, int pair);
+// Line solvers: y = M (RHS of src), then dst = y and acc = accin + y (neumann
+// launches) or acc = y (first launch); a null dst or acc skips that store, and
+// accin may alias src or acc
 template <class T>
-void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
-                   const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, int order,
-                   int lean, int both, hipStream_t st);
+void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, const C2<T> *accin, C2<T> *acc, const C2<T> *cf,
+                   const C2<T> *xt, const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R,
+                   int order, int lean, int both, hipStream_t st);
 template <class T>
-void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
-                   const PropParams *pp, int nprops, const LinePlan &lp, int W, int order, int lean, int pair,
-                   hipStream_t st);
+void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, const C2<T> *accin, C2<T> *acc, const C2<T> *cf,
+                   const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int W, int order, int lean,
+                   int pair, hipStream_t st);
 // interior march state -> full-grid complex128 fields with the global phase
 // and the output Fresnel interface
 template <class T>

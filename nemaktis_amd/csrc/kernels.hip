@@ -1102,11 +1102,14 @@ __device__ __forceinline__ void advance_rc(int &r, int &x, int step, int M) {
 // through LDS in the chunk-interleaved order (element (p, i) of a row at
 // i*P + p) so that global loads and stores stay coalesced; the tridiagonal
 // rows come pre-interleaved from the coefficient kernel.
-//   neumann 0: dst = M_x (I + mu A_y) src, acc = dst
-//   neumann 1: dst = M_x (mu O_x src),     acc += dst
+//   neumann 0: y = M_x (I + mu A_y) src
+//   neumann 1: y = M_x (mu O_x src)
+// then dst = y (skipped when dst is null) and acc = accin + y (neumann 1) or
+// acc = y (neumann 0; skipped when acc is null).  accin may be src or acc
+// itself: each element of accin is read and then written by the same thread.
 template <class T, int LMAX, int NEU, int LEAN>
 __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
-                                                 C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf,
+                                                 const C2<T> *accin, C2<T> *acc, const C2<T> *__restrict__ cf,
                                                  const C2<T> *__restrict__ xt, Geo g,
                                                  const PropParams *__restrict__ pp, LinePlan lp, int R,
                                                  int nitems_pairs, int order, int nc) {
@@ -1277,8 +1280,8 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
             for (int k = 0; k < NO; k++) {
                 if (cc < nc) {
                     const size_t o = pbase + (size_t)(nc == 2 ? cc : comp) * NP + x + M * (row0 + rr);
-                    o0v[k] = acc[o];
-                    o1v[k] = acc[o + 2 * NP];
+                    o0v[k] = accin[o];
+                    o1v[k] = accin[o + 2 * NP];
                 }
                 step();
             }
@@ -1292,10 +1295,14 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
                 const int ia = (cc * 2 * R + rr) * SL + smap[x];
                 const size_t o = pbase + (size_t)(nc == 2 ? cc : comp) * NP + x + M * (row0 + rr);
                 const C v0 = sd[ia], v1 = sd[ia + R * SL];
-                dst[o] = v0;
-                dst[o + 2 * NP] = v1;
-                acc[o] = NEU ? o0v[k] + v0 : v0;
-                acc[o + 2 * NP] = NEU ? o1v[k] + v1 : v1;
+                if (dst) {
+                    dst[o] = v0;
+                    dst[o + 2 * NP] = v1;
+                }
+                if (acc) {
+                    acc[o] = NEU ? o0v[k] + v0 : v0;
+                    acc[o + 2 * NP] = NEU ? o1v[k] + v1 : v1;
+                }
             }
             step();
         }
@@ -1333,7 +1340,7 @@ __device__ __forceinline__ void rhs_column(int comp, unsigned x, int j0, int ste
 
 template <class T, int LMAX, int NEU, int LEAN>
 __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
-                                                 C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf, Geo g,
+                                                 const C2<T> *accin, C2<T> *acc, const C2<T> *__restrict__ cf, Geo g,
                                                  const PropParams *__restrict__ pp, LinePlan lp, int W,
                                                  int npairs, int order, int nc) {
     using C = C2<T>;
@@ -1460,13 +1467,13 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     {
         // a thread stores at most LMAX points of its column (M <= P*LMAX);
         // the accumulator values of all of them are fetched in one round trip
-        C *dd = dst + ((size_t)prop * 2 + tcomp) * NP, *aa = acc + ((size_t)prop * 2 + tcomp) * NP;
+        const size_t ob = ((size_t)prop * 2 + tcomp) * NP;
         C ov[LMAX];
         if (NEU && col_ok) {
 #pragma unroll
             for (int k = 0; k < LMAX; k++) {
                 const int j = j0 + k * P;
-                if (j < M) ov[k] = aa[x0 + cc + g.Mx * j];
+                if (j < M) ov[k] = accin[ob + x0 + cc + g.Mx * j];
             }
         }
         if (col_ok) {
@@ -1475,9 +1482,9 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
                 const int j = j0 + k * P;
                 if (j < M) {
                     const C v = sd[ymap[j] * W + cc];
-                    const unsigned q = x0 + cc + g.Mx * j;
-                    dd[q] = v;
-                    aa[q] = NEU ? ov[k] + v : v;
+                    const size_t q = ob + x0 + cc + g.Mx * j;
+                    if (dst) dst[q] = v;
+                    if (acc) acc[q] = NEU ? ov[k] + v : v;
                 }
             }
         }
@@ -1490,7 +1497,8 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
 // thread then solves one polarisation as in k_ysolve.
 template <class T, int LMAX, int NEU, int LEAN>
 __global__ __launch_bounds__(512) void k_ysolve_pair(const C2<T> *__restrict__ src, C2<T> *__restrict__ dst,
-                                                      C2<T> *__restrict__ acc, const C2<T> *__restrict__ cf, Geo g,
+                                                      const C2<T> *accin, C2<T> *acc,
+                                                      const C2<T> *__restrict__ cf, Geo g,
                                                       const PropParams *__restrict__ pp, LinePlan lp, int W,
                                                       int npairs, int order) {
     using C = C2<T>;
@@ -1612,7 +1620,6 @@ __global__ __launch_bounds__(512) void k_ysolve_pair(const C2<T> *__restrict__ s
         // a thread stores at most LMAX/2 points of its column for each
         // polarisation (M <= P*LMAX, rows step 2P); accumulators in one round trip
         const size_t o0 = ((size_t)prop0 * 2 + comp) * NP, o1 = o0 + 2 * NP;
-        C *d0 = dst + o0, *d1 = dst + o1, *a0 = acc + o0, *a1 = acc + o1;
         constexpr int NO = (LMAX + 1) / 2;
         C o0v[NO], o1v[NO];
         if (NEU && col_ok) {
@@ -1621,8 +1628,8 @@ __global__ __launch_bounds__(512) void k_ysolve_pair(const C2<T> *__restrict__ s
                 const int j = jr + k * 2 * P;
                 if (j < M) {
                     const unsigned q = x0 + cc + g.Mx * j;
-                    o0v[k] = a0[q];
-                    o1v[k] = a1[q];
+                    o0v[k] = accin[o0 + q];
+                    o1v[k] = accin[o1 + q];
                 }
             }
         }
@@ -1634,10 +1641,14 @@ __global__ __launch_bounds__(512) void k_ysolve_pair(const C2<T> *__restrict__ s
                     const int ia = ymap[j] * W + cc;
                     const C v0 = sd0[ia], v1 = sd1[ia];
                     const unsigned q = x0 + cc + g.Mx * j;
-                    d0[q] = v0;
-                    d1[q] = v1;
-                    a0[q] = NEU ? o0v[k] + v0 : v0;
-                    a1[q] = NEU ? o1v[k] + v1 : v1;
+                    if (dst) {
+                        dst[o0 + q] = v0;
+                        dst[o1 + q] = v1;
+                    }
+                    if (acc) {
+                        acc[o0 + q] = NEU ? o0v[k] + v0 : v0;
+                        acc[o1 + q] = NEU ? o1v[k] + v1 : v1;
+                    }
                 }
             }
         }
@@ -1699,7 +1710,8 @@ size_t ysolve_lds_bytes(const LinePlan &lp, int W, int pair) {
 }
 
 template <class T, int NEU, int LEAN>
-static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
+static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *src, C2<T> *dst, const C2<T> *accin,
+                            C2<T> *acc,
                             const C2<T> *cf, const C2<T> *xt, const Geo &g, const PropParams *pp, int npairs,
                             int order, int nc, hipStream_t st) {
     const dim3 block(nc * 2 * lp.P * R);
@@ -1707,15 +1719,15 @@ static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *s
     switch (lp.lmax) {
         case 4:
             allow_lds(k_xsolve<T, 4, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_xsolve<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order, nc);
+            hipLaunchKernelGGL((k_xsolve<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, accin, acc, cf, xt, g, pp, lp, R, npairs, order, nc);
             break;
         case 8:
             allow_lds(k_xsolve<T, 8, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_xsolve<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order, nc);
+            hipLaunchKernelGGL((k_xsolve<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, accin, acc, cf, xt, g, pp, lp, R, npairs, order, nc);
             break;
         case 16:
             allow_lds(k_xsolve<T, 16, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_xsolve<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, xt, g, pp, lp, R, npairs, order, nc);
+            hipLaunchKernelGGL((k_xsolve<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, accin, acc, cf, xt, g, pp, lp, R, npairs, order, nc);
             break;
         default: throw std::runtime_error("x line solver: unsupported chunk length");
     }
@@ -1723,7 +1735,8 @@ static void xsolve_dispatch(const LinePlan &lp, int R, dim3 grid, const C2<T> *s
 }
 
 template <class T, int NEU, int LEAN>
-static void ysolve_pair_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
+static void ysolve_pair_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *src, C2<T> *dst, const C2<T> *accin,
+                            C2<T> *acc,
                                  const C2<T> *cf, const Geo &g, const PropParams *pp, int npairs, int order,
                                  hipStream_t st) {
     const dim3 block(2 * lp.P * W);
@@ -1731,17 +1744,17 @@ static void ysolve_pair_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<
     switch (lp.lmax) {
         case 4:
             allow_lds(k_ysolve_pair<T, 4, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_ysolve_pair<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W,
+            hipLaunchKernelGGL((k_ysolve_pair<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, accin, acc, cf, g, pp, lp, W,
                                npairs, order);
             break;
         case 8:
             allow_lds(k_ysolve_pair<T, 8, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_ysolve_pair<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W,
+            hipLaunchKernelGGL((k_ysolve_pair<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, accin, acc, cf, g, pp, lp, W,
                                npairs, order);
             break;
         case 16:
             allow_lds(k_ysolve_pair<T, 16, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_ysolve_pair<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp,
+            hipLaunchKernelGGL((k_ysolve_pair<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, accin, acc, cf, g, pp, lp,
                                W, npairs, order);
             break;
         default: throw std::runtime_error("y line solver: unsupported chunk length");
@@ -1750,7 +1763,8 @@ static void ysolve_pair_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<
 }
 
 template <class T, int NEU, int LEAN>
-static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *src, C2<T> *dst, C2<T> *acc,
+static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *src, C2<T> *dst, const C2<T> *accin,
+                            C2<T> *acc,
                             const C2<T> *cf, const Geo &g, const PropParams *pp, int npairs, int order, int nc,
                             hipStream_t st) {
     const dim3 block(nc * lp.P * W);
@@ -1758,15 +1772,15 @@ static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *s
     switch (lp.lmax) {
         case 4:
             allow_lds(k_ysolve<T, 4, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_ysolve<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order, nc);
+            hipLaunchKernelGGL((k_ysolve<T, 4, NEU, LEAN>), grid, block, lds, st, src, dst, accin, acc, cf, g, pp, lp, W, npairs, order, nc);
             break;
         case 8:
             allow_lds(k_ysolve<T, 8, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_ysolve<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order, nc);
+            hipLaunchKernelGGL((k_ysolve<T, 8, NEU, LEAN>), grid, block, lds, st, src, dst, accin, acc, cf, g, pp, lp, W, npairs, order, nc);
             break;
         case 16:
             allow_lds(k_ysolve<T, 16, NEU, LEAN>, lds);
-            hipLaunchKernelGGL((k_ysolve<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, acc, cf, g, pp, lp, W, npairs, order, nc);
+            hipLaunchKernelGGL((k_ysolve<T, 16, NEU, LEAN>), grid, block, lds, st, src, dst, accin, acc, cf, g, pp, lp, W, npairs, order, nc);
             break;
         default: throw std::runtime_error("y line solver: unsupported chunk length");
     }
@@ -1774,7 +1788,8 @@ static void ysolve_dispatch(const LinePlan &lp, int W, dim3 grid, const C2<T> *s
 }
 
 template <class T>
-void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const C2<T> *xt,
+void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, const C2<T> *accin, C2<T> *acc, const C2<T> *cf,
+                   const C2<T> *xt,
                    const Geo &g, const PropParams *pp, int nprops, const LinePlan &lp, int R, int order,
                    int lean, int both, hipStream_t st) {
     const int nc = both ? 2 : 1;
@@ -1785,16 +1800,17 @@ void launch_xsolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const 
     const int64_t items = (int64_t)((g.My + R - 1) / R) * (3 - nc) * npairs;
     dim3 grid((unsigned)((items + 7) / 8 * 8));
     if (lean) {
-        if (neumann) xsolve_dispatch<T, 1, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, nc, st);
-        else xsolve_dispatch<T, 0, 1>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, nc, st);
+        if (neumann) xsolve_dispatch<T, 1, 1>(lp, R, grid, src, dst, accin, acc, cf, xt, g, pp, npairs, order, nc, st);
+        else xsolve_dispatch<T, 0, 1>(lp, R, grid, src, dst, accin, acc, cf, xt, g, pp, npairs, order, nc, st);
     } else {
-        if (neumann) xsolve_dispatch<T, 1, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, nc, st);
-        else xsolve_dispatch<T, 0, 0>(lp, R, grid, src, dst, acc, cf, xt, g, pp, npairs, order, nc, st);
+        if (neumann) xsolve_dispatch<T, 1, 0>(lp, R, grid, src, dst, accin, acc, cf, xt, g, pp, npairs, order, nc, st);
+        else xsolve_dispatch<T, 0, 0>(lp, R, grid, src, dst, accin, acc, cf, xt, g, pp, npairs, order, nc, st);
     }
 }
 
 template <class T>
-void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const C2<T> *cf, const Geo &g,
+void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, const C2<T> *accin, C2<T> *acc, const C2<T> *cf,
+                   const Geo &g,
                    const PropParams *pp, int nprops, const LinePlan &lp, int W, int order, int lean, int pair,
                    hipStream_t st) {
     if ((pair ? 2 : 1) * lp.P * W > 512 || lp.M != g.My || W < 1 || (W & (W - 1)) || lp.P * lp.lmax < lp.M ||
@@ -1806,11 +1822,11 @@ void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const 
         const int64_t items = (int64_t)((g.Mx + W - 1) / W) * 2 * npairs;
         dim3 grid((unsigned)((items + 7) / 8 * 8));
         if (lean) {
-            if (neumann) ysolve_pair_dispatch<T, 1, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
-            else ysolve_pair_dispatch<T, 0, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+            if (neumann) ysolve_pair_dispatch<T, 1, 1>(lp, W, grid, src, dst, accin, acc, cf, g, pp, npairs, order, st);
+            else ysolve_pair_dispatch<T, 0, 1>(lp, W, grid, src, dst, accin, acc, cf, g, pp, npairs, order, st);
         } else {
-            if (neumann) ysolve_pair_dispatch<T, 1, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
-            else ysolve_pair_dispatch<T, 0, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, st);
+            if (neumann) ysolve_pair_dispatch<T, 1, 0>(lp, W, grid, src, dst, accin, acc, cf, g, pp, npairs, order, st);
+            else ysolve_pair_dispatch<T, 0, 0>(lp, W, grid, src, dst, accin, acc, cf, g, pp, npairs, order, st);
         }
         return;
     }
@@ -1818,11 +1834,11 @@ void launch_ysolve(int neumann, const C2<T> *src, C2<T> *dst, C2<T> *acc, const 
     const int64_t items = (int64_t)((g.Mx + W - 1) / W) * 2 * (3 - nc) * npairs;
     dim3 grid((unsigned)((items + 7) / 8 * 8));
     if (lean) {
-        if (neumann) ysolve_dispatch<T, 1, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, nc, st);
-        else ysolve_dispatch<T, 0, 1>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, nc, st);
+        if (neumann) ysolve_dispatch<T, 1, 1>(lp, W, grid, src, dst, accin, acc, cf, g, pp, npairs, order, nc, st);
+        else ysolve_dispatch<T, 0, 1>(lp, W, grid, src, dst, accin, acc, cf, g, pp, npairs, order, nc, st);
     } else {
-        if (neumann) ysolve_dispatch<T, 1, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, nc, st);
-        else ysolve_dispatch<T, 0, 0>(lp, W, grid, src, dst, acc, cf, g, pp, npairs, order, nc, st);
+        if (neumann) ysolve_dispatch<T, 1, 0>(lp, W, grid, src, dst, accin, acc, cf, g, pp, npairs, order, nc, st);
+        else ysolve_dispatch<T, 0, 0>(lp, W, grid, src, dst, accin, acc, cf, g, pp, npairs, order, nc, st);
     }
 }
 
@@ -2152,9 +2168,10 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
                                      int, hipStream_t);                                                               \
     template void launch_bulk_plane<T>(const C2<T> *, const Geo &, const SlotParams *, const PropParams *, int, \
                                        int, int, c64 *, hipStream_t);                                           \
-    template void launch_xsolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const C2<T> *, const Geo &, \
-                                   const PropParams *, int, const LinePlan &, int, int, int, int, hipStream_t);       \
-    template void launch_ysolve<T>(int, const C2<T> *, C2<T> *, C2<T> *, const C2<T> *, const Geo &,                  \
+    template void launch_xsolve<T>(int, const C2<T> *, C2<T> *, const C2<T> *, C2<T> *, const C2<T> *,              \
+                                   const C2<T> *, const Geo &, const PropParams *, int, const LinePlan &, int, int,  \
+                                   int, int, hipStream_t);                                                            \
+    template void launch_ysolve<T>(int, const C2<T> *, C2<T> *, const C2<T> *, C2<T> *, const C2<T> *, const Geo &, \
                                    const PropParams *, int, const LinePlan &, int, int, int, int, hipStream_t);       \
     template size_t xsolve_lds_bytes<T>(const LinePlan &, int, int);                                                  \
     template size_t ysolve_lds_bytes<T>(const LinePlan &, int, int);                                                  \
