@@ -32,8 +32,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-KERNELS = {"solve_x": "k_xsolve", "solve_y": "k_ysolve", "boundary": "k_boundary", "coefs": "k_coefs",
-           "material": "k_slab_material", "explicit": "k_explicit"}
+KERNELS = {"solve_x": "k_xsolve", "solve_y": "k_ysolve", "boundary": "k_boundary", "coefs": "k_slab_coefs",
+           "explicit": "k_explicit"}
 
 
 def algorithmic_bytes(fam, c, nprops, nslots, NP, Nfull, fd, nw):
@@ -58,9 +58,8 @@ def algorithmic_bytes(fam, c, nprops, nslots, NP, Nfull, fd, nw):
             total += (planes * nprops + 20 * nslots) * NP * c
         return total / (1 + nw)
     if fam == "coefs":
-        return 15 * 8 * nslots * Nfull + 49 * nslots * NP * c
-    if fam == "material":
-        return 2 * fd * 8 * Nfull + 15 * 8 * nslots * Nfull
+        # director planes iz, iz+1 in (FP64); 40 coefficient + 3 phase-operator + 6 line-matrix planes out per slot
+        return 2 * fd * 8 * Nfull + 49 * nslots * NP * c
     return None
 
 

@@ -161,10 +161,9 @@ __host__ __device__ __forceinline__ unsigned cf_point(unsigned p) {
     return (p / kCfTile) * (unsigned)(CF_COUNT * kCfTile) + (p % kCfTile);
 }
 
-// slab-material plane slots (per full-grid point, FP64)
+// slab-material fields the coefficients read (FP64, per point of a tile)
 enum : int {
     SM_EINV = 0, SM_KXX, SM_KYY, SM_KXY, SM_IXX, SM_IYY, SM_IXY, SM_VX, SM_VY,
-    SM_PXX_RE, SM_PXX_IM, SM_PYY_RE, SM_PYY_IM, SM_PXY_RE, SM_PXY_IM,
     SM_COUNT
 };
 

@@ -112,7 +112,7 @@ public:
             own_stream_ = true;
         }
         sz_ = sizes_of(p, o);
-        bind_source(o.allocator, bulk_, bt_, lc_, weights_, q_, iso_t_, iso_n_, partials_, sm_, mask_, twx_, twy_, E_, sp_, pp_, cf_,
+        bind_source(o.allocator, bulk_, bt_, lc_, weights_, q_, iso_t_, iso_n_, partials_, mask_, twx_, twy_, E_, sp_, pp_, cf_,
                     xt_, fb_[0], fb_[1], fb_[2], fb_[3]);
         g_.Nx = p.Nx;
         g_.Ny = p.Ny;
@@ -170,7 +170,6 @@ public:
         sp_.alloc(sz_.nslots_max);
         pp_.alloc(sz_.nprops_max);
         partials_.alloc((size_t)kNrefBlocks * sz_.nslots_max);
-        sm_.alloc((size_t)sz_.nslots_max * SM_COUNT * sz_.Nfull);
         cf_.alloc(2 * (size_t)sz_.nslots_max * cf_slot_size(sz_.NP));
         if (!o.profile) HIPCHK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
         set_tuning(o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, g_.Mx, g_.My));
@@ -205,7 +204,7 @@ public:
 
     int64_t footprint() const override {
         size_t b = lc_.footprint() + weights_.footprint() + q_.footprint() + iso_t_.footprint() + iso_n_.footprint() +
-                   partials_.footprint() + sm_.footprint() + mask_.footprint() + twx_.footprint() + twy_.footprint() +
+                   partials_.footprint() + mask_.footprint() + twx_.footprint() + twy_.footprint() +
                    E_.footprint() + sp_.footprint() + pp_.footprint() + cf_.footprint() + xt_.footprint() +
                    bulk_.footprint() + bt_.footprint();
         for (auto &f : fb_) b += f.footprint();
@@ -293,7 +292,7 @@ private:
     bool own_stream_ = false;
     std::vector<hipEvent_t> sync_evs_;
     FftPlan plan_x_, plan_y_;
-    DevBuf<double> lc_, weights_, q_, iso_t_, iso_n_, partials_, sm_;
+    DevBuf<double> lc_, weights_, q_, iso_t_, iso_n_, partials_;
     DevBuf<uint8_t> mask_;
     DevBuf<c64> twx_, twy_, E_;
     std::vector<c64> htwx_, htwy_;
@@ -498,10 +497,9 @@ private:
             return x;
         };
         auto coef = [&](int k) {
-            timed(KF_MATERIAL, [&] {
-                launch_slab_material(lc_.ptr, mask_.ptr, weights_.ptr, g_, sp_.ptr, nslots, k, sm_.ptr, cs);
+            timed(KF_COEFS, [&] {
+                launch_slab_coefs<T>(lc_.ptr, mask_.ptr, weights_.ptr, g_, sp_.ptr, nslots, k, CF(k), XT(k), lpx_, cs);
             });
-            timed(KF_COEFS, [&] { launch_coefs<T>(sm_.ptr, g_, sp_.ptr, nslots, CF(k), XT(k), lpx_, cs); });
             return record(cs);
         };
         if (overlap) HIPCHK(hipStreamWaitEvent(cs, record(st_), 0));
@@ -675,7 +673,6 @@ int64_t estimate_footprint(const Problem &p, const Options &o) {
     add((int64_t)s.nslots_max * (int64_t)sizeof(SlotParams));
     add((int64_t)s.nprops_max * (int64_t)sizeof(PropParams));
     add((int64_t)kNrefBlocks * s.nslots_max * 8);
-    add((int64_t)s.nslots_max * SM_COUNT * s.Nfull * 8);
     add(2 * (int64_t)s.nslots_max * cf_slot_size(s.NP) * fsz);
     if (p.Nx > 3) {
         const Tuning t = o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, Mx, My);

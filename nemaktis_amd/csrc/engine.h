@@ -44,7 +44,8 @@ struct Options {
     DeviceAllocator *allocator = nullptr;  // nullptr: hipMalloc / hipFree
 };
 
-// per-kernel-family device time of one propagate() (profile mode)
+// per-kernel-family device time of one propagate() (profile mode); the slab
+// material is evaluated inside the coefficient kernel, so KF_MATERIAL stays 0
 enum KernelFamily {
     KF_MATERIAL = 0, KF_COEFS, KF_BOUNDARY, KF_RESERVED, KF_EXPLICIT, KF_SOLVE_X, KF_SOLVE_Y, KF_OUTPUT, KF_FILTER,
     KF_SETUP, KF_COUNT

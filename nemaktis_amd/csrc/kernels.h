@@ -11,15 +11,13 @@ namespace nmk {
 void launch_nref(const double *lc, const uint8_t *mask, const Geo &g, SlotParams *sp, int nslots, double *partials,
                  int nblocks, hipStream_t st);
 
-// slab-averaged material data of slab iz for every slot (FP64, full grid)
-void launch_slab_material(const double *lc, const uint8_t *mask, const double *weights, const Geo &g,
-                          const SlotParams *sp, int nslots, int iz, double *sm, hipStream_t st);
-
-// mu*R stencil coefficients and the phase operator on the march grid
-// (xt: the x-line tridiagonal rows in the chunk-interleaved layout of lpx)
+// slab material of slab iz (FP64) and from it the mu*R stencil coefficients
+// and the phase operator on the march grid, for every slot (xt: the x-line
+// tridiagonal rows in the chunk-interleaved layout of lpx)
 template <class T>
-void launch_coefs(const double *sm, const Geo &g, const SlotParams *sp, int nslots, C2<T> *cf, C2<T> *xt,
-                  const LinePlan &lpx, hipStream_t st);
+void launch_slab_coefs(const double *lc, const uint8_t *mask, const double *weights, const Geo &g,
+                       const SlotParams *sp, int nslots, int iz, C2<T> *cf, C2<T> *xt, const LinePlan &lpx,
+                       hipStream_t st);
 
 // input beam + input Fresnel (beam != None) or packing of caller fields
 template <class T>

@@ -2,7 +2,6 @@
 //
 // Data layout in HBM
 //   director     lc[c + fd*(ix + Nx*(iy + Ny*iz))]   float64 (caller layout)
-//   slab material sm[(slot*SM_COUNT + c)*Nfull + m]   float64, full grid
 //   coefficients cf[slot*cf_slot_size(NP) + cf_point(p) + c*64]  C2<T>, march grid
 //   march state  F[(prop*2 + comp)*NP + p]            C2<T>, march grid
 //   output       E[(2*prop + comp)*Nfull + m]         complex128, full grid
@@ -129,41 +128,43 @@ void launch_nref(const double *lc, const uint8_t *mask, const Geo &g, SlotParams
 }
 
 // ---------------------------------------------------------------------------
-// slab material of slab iz (planes iz, iz+1), full grid, FP64:
-// averaged 1/eps_zz, K, sqrt(K)^-1, summed eps_az/eps_zz and the phase
-// operator exp(i alpha d) (PhaseEvolutionOperator.cpp:34-63)
+// Slab material and mu*R coefficients of slab iz (planes iz, iz+1), one
+// launch per slab for every slot.  A block takes a 32 x 8 tile of the march
+// grid: its threads first evaluate, in FP64, the slab-averaged material of
+// the tile and a one-point halo into LDS -- averaged 1/eps_zz, K,
+// sqrt(K)^-1, summed eps_az/eps_zz -- and, at the tile's own points, the
+// phase operator exp(i alpha d) (PhaseEvolutionOperator.cpp:34-63), stored
+// beside the coefficients; then each thread assembles the coefficients of
+// its point (ParaxialPrimaryEvolutionOperator.cpp:212-490) from the LDS
+// tile:
+//   D  = D0 + D1 K                      (D1 with the seam rule of :428-430)
+//   R  = i W + (D S^-1 + S^-1 D) / 4,    W = (v_a d_b + d_a v_b) / 2
+// for the three stencil families DX, DY, DXDY and their 2x2 blocks.  The
+// halo holds the periodic neighbours of the march grid; 1/eps_zz of the
+// duplicate column and row (full-grid index Nx-1, Ny-1), which the D1 seam
+// rule reads at ix+1 = Mx or iy+1 = My, is evaluated separately.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_slab_material(const double *__restrict__ lc, const uint8_t *__restrict__ mask,
-                                                       const double *__restrict__ weights, Geo g,
-                                                       const SlotParams *__restrict__ sp, int iz,
-                                                       double *__restrict__ sm) {
-    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= g.Nfull) return;
-    const int slot = blockIdx.y;
-    const SlotParams s = sp[slot];
-    double a[12], b[12];
-    eps_voxel(lc, mask, g.field_dim, m + g.Nfull * iz, s, a);
-    eps_voxel(lc, mask, g.field_dim, m + g.Nfull * (iz + 1), s, b);
-    double *o = sm + (int64_t)slot * SM_COUNT * g.Nfull + m;
-    const int64_t N = g.Nfull;
-    o[SM_EINV * N] = (1. / a[2] + 1. / b[2]) / 2;
-    o[SM_KXX * N] = (a[6] + b[6]) / 2;
-    o[SM_KYY * N] = (a[7] + b[7]) / 2;
-    o[SM_KXY * N] = (a[8] + b[8]) / 2;
-    double sxx = (a[9] + b[9]) / 2, syy = (a[10] + b[10]) / 2, sxy = (a[11] + b[11]) / 2;
-    double det = sxx * syy - sxy * sxy;
-    o[SM_IXX * N] = syy / det;
-    o[SM_IYY * N] = sxx / det;
-    o[SM_IXY * N] = -sxy / det;
-    o[SM_VX * N] = a[4] / a[2] + b[4] / b[2];
-    o[SM_VY * N] = a[5] / a[2] + b[5] / b[2];
+constexpr int kMatTx = 32, kMatTy = 8, kMatHx = kMatTx + 2, kMatHy = kMatTy + 2, kMatHalo = kMatHx * kMatHy;
 
-    double w = 0.5, wpz = 0.5;
-    if (weights) {
-        const int64_t v = m + N * iz;
-        w = weights[2 * v];
-        wpz = weights[2 * v + 1];
-    }
+// slab averages of the fields the coefficients read, from the permittivity
+// data a (plane iz) and b (plane iz+1) of one point
+__device__ __forceinline__ void slab_fields(const double *a, const double *b, double *o) {
+    o[SM_EINV] = (1. / a[2] + 1. / b[2]) / 2;
+    o[SM_KXX] = (a[6] + b[6]) / 2;
+    o[SM_KYY] = (a[7] + b[7]) / 2;
+    o[SM_KXY] = (a[8] + b[8]) / 2;
+    const double sxx = (a[9] + b[9]) / 2, syy = (a[10] + b[10]) / 2, sxy = (a[11] + b[11]) / 2;
+    const double det = sxx * syy - sxy * sxy;
+    o[SM_IXX] = syy / det;
+    o[SM_IYY] = sxx / det;
+    o[SM_IXY] = -sxy / det;
+    o[SM_VX] = a[4] / a[2] + b[4] / b[2];
+    o[SM_VY] = a[5] / a[2] + b[5] / b[2];
+}
+
+// phase operator exp(i alpha d) of one point, d = dZ/2 (w S(iz) + wpz S(iz+1) - no I)
+__device__ __forceinline__ void slab_phase(const double *a, const double *b, double w, double wpz,
+                                           const SlotParams &s, c64 &pxx, c64 &pyy, c64 &pxy) {
     const double dZ = s.dZ;
     double dxx = 0.5 * dZ * (w * a[9] + wpz * b[9] - s.no);
     double dyy = 0.5 * dZ * (w * a[10] + wpz * b[10] - s.no);
@@ -182,41 +183,94 @@ __global__ __launch_bounds__(256) void k_slab_material(const double *__restrict_
     c64 ph = mk(amp * cos(tr), amp * sin(tr));
     c64 ids = mk(-diff * sinc.y, diff * sinc.x);   // i*diff*sinc
     c64 ixs = mk(-dxy * sinc.y, dxy * sinc.x);     // i*dxy*sinc
-    c64 pxx = ph * (cs + ids), pyy = ph * (cs - ids), pxy = ph * ixs;
-    o[SM_PXX_RE * N] = pxx.x;
-    o[SM_PXX_IM * N] = pxx.y;
-    o[SM_PYY_RE * N] = pyy.x;
-    o[SM_PYY_IM * N] = pyy.y;
-    o[SM_PXY_RE * N] = pxy.x;
-    o[SM_PXY_IM * N] = pxy.y;
+    pxx = ph * (cs + ids);
+    pyy = ph * (cs - ids);
+    pxy = ph * ixs;
 }
 
-void launch_slab_material(const double *lc, const uint8_t *mask, const double *weights, const Geo &g,
-                          const SlotParams *sp, int nslots, int iz, double *sm, hipStream_t st) {
-    hipLaunchKernelGGL(k_slab_material, dim3(blocks_for(g.Nfull, 256), nslots), dim3(256), 0, st, lc, mask, weights,
-                       g, sp, iz, sm);
-    NMK_LAUNCH_CHECK();
+__device__ __forceinline__ int wrap_index(int v, int M) {
+    v %= M;
+    return v < 0 ? v + M : v;
 }
-
-// ---------------------------------------------------------------------------
-// mu*R coefficients (ParaxialPrimaryEvolutionOperator.cpp:212-490) assembled
-// per row point from the slab material of the point and its neighbours:
-//   D  = D0 + D1 K                      (D1 with the seam rule of :428-430)
-//   R  = i W + (D S^-1 + S^-1 D) / 4,    W = (v_a d_b + d_a v_b) / 2
-// for the three stencil families DX, DY, DXDY and their 2x2 blocks.
-// ---------------------------------------------------------------------------
 
 template <class T>
-__global__ __launch_bounds__(256) void k_coefs(const double *__restrict__ sm, Geo g, const SlotParams *__restrict__ sp,
-                                               C2<T> *__restrict__ cf, C2<T> *__restrict__ xt, LinePlan lpx) {
-    const int64_t ip = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (ip >= g.NP) return;
+__global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ lc, const uint8_t *__restrict__ mask,
+                                                    const double *__restrict__ weights, Geo g,
+                                                    const SlotParams *__restrict__ sp, int iz,
+                                                    C2<T> *__restrict__ cf, C2<T> *__restrict__ xt, LinePlan lpx) {
+    __shared__ double mat[SM_COUNT * kMatHalo];
+    __shared__ double seam_x[kMatTy], seam_y[kMatTx];
+    const int ntx = (g.Mx + kMatTx - 1) / kMatTx, nty = (g.My + kMatTy - 1) / kMatTy;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= ntx * nty) return;
+    const int ix0 = (tile % ntx) * kMatTx, iy0 = (tile / ntx) * kMatTy;
     const int slot = blockIdx.y;
     const SlotParams s = sp[slot];
-    const int ix = (int)(ip % g.Mx), iy = (int)(ip / g.Mx);
     const int64_t N = g.Nfull;
-    const double *S = sm + (int64_t)slot * SM_COUNT * N;
-    auto at = [&](int c, int jx, int jy) { return S[c * N + jx + (int64_t)g.Nx * jy]; };
+    C2<T> *cfs = cf + (size_t)slot * cf_slot_size(g.NP);
+
+    // material of the tile and its halo (periodic neighbours on the march grid)
+    for (int h = threadIdx.x; h < kMatHalo; h += blockDim.x) {
+        const int lx = h % kMatHx, ly = h / kMatHx;
+        const int jx = wrap_index(ix0 - 1 + lx, g.Mx), jy = wrap_index(iy0 - 1 + ly, g.My);
+        const int64_t m = jx + (int64_t)g.Nx * jy;
+        double a[12], b[12], f[SM_COUNT];
+        eps_voxel(lc, mask, g.field_dim, m + N * iz, s, a);
+        eps_voxel(lc, mask, g.field_dim, m + N * (iz + 1), s, b);
+        slab_fields(a, b, f);
+#pragma unroll
+        for (int c = 0; c < SM_COUNT; c++) mat[c * kMatHalo + h] = f[c];
+        const int ox = ix0 - 1 + lx, oy = iy0 - 1 + ly;
+        if (lx >= 1 && lx <= kMatTx && ly >= 1 && ly <= kMatTy && ox < g.Mx && oy < g.My) {
+            double w = 0.5, wpz = 0.5;
+            if (weights) {
+                const int64_t v = m + N * iz;
+                w = weights[2 * v];
+                wpz = weights[2 * v + 1];
+            }
+            c64 pxx, pyy, pxy;
+            slab_phase(a, b, w, wpz, s, pxx, pyy, pxy);
+            C2<T> *o = cfs + cf_point((unsigned)(ox + g.Mx * oy));
+            o[(CF_P + 0) * kCfTile] = C2<T>{(T)pxx.x, (T)pxx.y};
+            o[(CF_P + 1) * kCfTile] = C2<T>{(T)pyy.x, (T)pyy.y};
+            o[(CF_P + 2) * kCfTile] = C2<T>{(T)pxy.x, (T)pxy.y};
+        }
+    }
+    // 1/eps_zz of the duplicate column (right of a tile ending at Mx-1) and row
+    const int t = threadIdx.x;
+    if (g.px && t < kMatTy && ix0 + kMatTx >= g.Mx && iy0 + t < g.My) {
+        const int64_t m = g.Mx + (int64_t)g.Nx * (iy0 + t);
+        double a[12], b[12];
+        eps_voxel(lc, mask, g.field_dim, m + N * iz, s, a);
+        eps_voxel(lc, mask, g.field_dim, m + N * (iz + 1), s, b);
+        seam_x[t] = (1. / a[2] + 1. / b[2]) / 2;
+    }
+    if (g.py && t >= 64 && t < 64 + kMatTx && iy0 + kMatTy >= g.My && ix0 + t - 64 < g.Mx) {
+        const int64_t m = (ix0 + t - 64) + (int64_t)g.Nx * g.My;
+        double a[12], b[12];
+        eps_voxel(lc, mask, g.field_dim, m + N * iz, s, a);
+        eps_voxel(lc, mask, g.field_dim, m + N * (iz + 1), s, b);
+        seam_y[t - 64] = (1. / a[2] + 1. / b[2]) / 2;
+    }
+    __syncthreads();
+
+    const int ix = ix0 + t % kMatTx, iy = iy0 + t / kMatTx;
+    if (ix >= g.Mx || iy >= g.My) return;
+    const int64_t ip = ix + (int64_t)g.Mx * iy;
+    // material at march point (jx, jy) -- a periodic neighbour of the tile --
+    // or, for 1/eps_zz, at the duplicate column jx = Mx or row jy = My
+    auto at = [&](int c, int jx, int jy) {
+        if (c == SM_EINV) {
+            if (jx == g.Mx) return seam_x[jy - iy0];
+            if (jy == g.My) return seam_y[jx - ix0];
+        }
+        int hx = jx - ix0 + 1, hy = jy - iy0 + 1;
+        if (hx < 0) hx += g.Mx;
+        else if (hx > kMatTx + 1) hx -= g.Mx;
+        if (hy < 0) hy += g.My;
+        else if (hy > kMatTy + 1) hy -= g.My;
+        return mat[c * kMatHalo + hx + kMatHx * hy];
+    };
     const int xm = g.px ? (ix == 0 ? g.Mx - 1 : ix - 1) : ix;
     const int xp = g.px ? (ix == g.Mx - 1 ? 0 : ix + 1) : ix;
     const int ym = g.py ? (iy == 0 ? g.My - 1 : iy - 1) : iy;
@@ -333,12 +387,9 @@ __global__ __launch_bounds__(256) void k_coefs(const double *__restrict__ sm, Ge
     }
     // store mu*R with mu = i*dZ/2
     const double hz = 0.5 * s.dZ;
-    C2<T> *o = cf + (size_t)slot * cf_slot_size(g.NP) + cf_point((unsigned)ip);
+    C2<T> *o = cfs + cf_point((unsigned)ip);
 #pragma unroll
     for (int k = 0; k < 40; k++) o[k * kCfTile] = C2<T>{(T)(-hz * im[k]), (T)(hz * re[k])};
-    o[(CF_P + 0) * kCfTile] = C2<T>{(T)at(SM_PXX_RE, ix, iy), (T)at(SM_PXX_IM, ix, iy)};
-    o[(CF_P + 1) * kCfTile] = C2<T>{(T)at(SM_PYY_RE, ix, iy), (T)at(SM_PYY_IM, ix, iy)};
-    o[(CF_P + 2) * kCfTile] = C2<T>{(T)at(SM_PXY_RE, ix, iy), (T)at(SM_PXY_IM, ix, iy)};
     // (I - mu T) rows of the x-line systems (T = DX00, DX11) in the chunk-interleaved layout
     if (g.px && xt) {
         int pc, i;
@@ -356,10 +407,12 @@ __global__ __launch_bounds__(256) void k_coefs(const double *__restrict__ sm, Ge
 }
 
 template <class T>
-void launch_coefs(const double *sm, const Geo &g, const SlotParams *sp, int nslots, C2<T> *cf, C2<T> *xt,
-                  const LinePlan &lpx, hipStream_t st) {
-    hipLaunchKernelGGL(k_coefs<T>, dim3((blocks_for(g.NP, 256) + 7) / 8 * 8, nslots), dim3(256), 0, st, sm, g, sp, cf,
-                       xt, lpx);
+void launch_slab_coefs(const double *lc, const uint8_t *mask, const double *weights, const Geo &g,
+                       const SlotParams *sp, int nslots, int iz, C2<T> *cf, C2<T> *xt, const LinePlan &lpx,
+                       hipStream_t st) {
+    const int64_t tiles = (int64_t)((g.Mx + kMatTx - 1) / kMatTx) * ((g.My + kMatTy - 1) / kMatTy);
+    hipLaunchKernelGGL(k_slab_coefs<T>, dim3((unsigned)((tiles + 7) / 8 * 8), nslots), dim3(256), 0, st, lc, mask,
+                       weights, g, sp, iz, cf, xt, lpx);
     NMK_LAUNCH_CHECK();
 }
 
@@ -2157,8 +2210,9 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
 
 // explicit instantiations
 #define NMK_INST(T)                                                                                                   \
-    template void launch_coefs<T>(const double *, const Geo &, const SlotParams *, int, C2<T> *, C2<T> *,           \
-                                  const LinePlan &, hipStream_t);                                                     \
+    template void launch_slab_coefs<T>(const double *, const uint8_t *, const double *, const Geo &,                \
+                                       const SlotParams *, int, int, C2<T> *, C2<T> *, const LinePlan &,           \
+                                       hipStream_t);                                                                \
     template void launch_init_fields<T>(const double *, const uint8_t *, const Geo &, const SlotParams *,             \
                                         const PropParams *, int, int, double, double, double, C2<T> *, hipStream_t);  \
     template void launch_pack_fields<T>(const c64 *, const Geo &, const PropParams *, int, int, C2<T> *, hipStream_t); \
