@@ -41,25 +41,29 @@ def algorithmic_bytes(fam, c, nprops, nslots, NP, Nfull, fd, nw):
     element it reads or writes counted once (c = bytes per complex value,
     NP march-grid points, Nfull full-grid points).
       line solves, first launch of a half-step: per propagation the source
-      field (2 components) in, the solution (2 components) out; per
-      wavelength slot 20 right-hand-side coefficient planes + 6 line-matrix
-      planes.  Neumann launch i of nw: source in, the sum so far in (i > 0;
-      launch 0 adds the source itself), the sum out, the solution out for the
-      next launch (i < nw - 1); 14 + 6 coefficient planes per slot.  The
-      family average weighs one first launch and the nw Neumann launches."""
+      field (2 components) in, the solution (2 components) out.  Neumann
+      launch i of nw: source in, the sum so far in (i > 0; launch 0 adds the
+      source itself), the sum out, the solution out for the next launch
+      (i < nw - 1).  Per wavelength slot the coefficient planes of the taps
+      (complex planes count 1, imaginary-only planes 1/2) and the 6
+      line-matrix planes: x first 15 + 6, y first 13 + 6, Neumann 9 + 6.
+      The family average weighs one first launch and the nw Neumann
+      launches."""
     if fam == "boundary":
         return 4 * nprops * NP * c + 6 * nslots * NP * c
     if fam == "explicit":
-        return 4 * nprops * NP * c + 20 * nslots * NP * c
+        # (I + mu A_y) or (I + mu A_x) of both components: 15 or 13 coefficient planes, averaged
+        return 4 * nprops * NP * c + 14 * nslots * NP * c
     if fam in ("solve_x", "solve_y"):
-        total = (4 * nprops + 26 * nslots) * NP * c
+        total = (4 * nprops + (21 if fam == "solve_x" else 19) * nslots) * NP * c
         for i in range(nw):
             planes = 2 + (2 if i > 0 else 0) + 2 + (2 if i + 1 < nw else 0)
-            total += (planes * nprops + 20 * nslots) * NP * c
+            total += (planes * nprops + 15 * nslots) * NP * c
         return total / (1 + nw)
     if fam == "coefs":
-        # director planes iz, iz+1 in (FP64); 40 coefficient + 3 phase-operator + 6 line-matrix planes out per slot
-        return 2 * fd * 8 * Nfull + 49 * nslots * NP * c
+        # director planes iz, iz+1 in (FP64); per slot 16 complex + 24 imaginary-only coefficient planes,
+        # 3 phase-operator planes and 6 line-matrix planes out
+        return 2 * fd * 8 * Nfull + 37 * nslots * NP * c
     return None
 
 

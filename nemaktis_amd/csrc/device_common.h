@@ -145,20 +145,51 @@ enum : int {
     CF_DX = 0,     // 4 blocks x {minus, centre, plus}
     CF_DY = 12,    // 4 blocks x {minus, centre, plus}
     CF_XY = 24,    // 4 blocks x {mxmy, pxmy, mxpy, pxpy}
-    CF_P = 40,     // op_xx, op_yy, op_xy
-    CF_COUNT = 43
+    CF_NMU = 40    // mu*R planes
 };
+// mu*R = i (dZ/2) R.  R is real apart from the walk-off entries, which sit
+// in DX blocks 00, 01, 10 and DY blocks 01, 10, 11 at the minus and plus
+// slots (W_ab = (v_a d_b + d_a v_b) / 2 has no centre term and no DX part in
+// block 11 or DY part in block 00): those 12 planes are stored complex, and
+// so are DY00 and DY11 whole (the y-line rows, which a y-line block reads for
+// a component it knows only at run time); the other 24 planes hold the
+// imaginary part of mu*R only (their real part is zero).
+__host__ __device__ constexpr bool cf_complex(int k) {
+    const int fam = k < CF_DY ? 0 : 1, blk = (k - (fam ? CF_DY : CF_DX)) / 3, sl = k % 3;
+    return k < CF_XY && ((sl != 1 && blk != (fam ? 0 : 3)) || (fam == 1 && (blk == 0 || blk == 3)));
+}
+// index of plane k inside its region (complex or imaginary-only)
+__host__ __device__ constexpr int cf_index(int k) {
+    int n = 0;
+    for (int j = 0; j < k; j++) n += cf_complex(j) == cf_complex(k);
+    return n;
+}
+constexpr int CF_NCPLX = 16 + 3;  // complex planes: the 16 complex mu*R planes, then P
+constexpr int CF_P = 16;          // complex plane of op_xx (op_yy, op_xy follow)
+constexpr int CF_NIMAG = 24;      // imaginary-only planes
 // Coefficient planes are stored in tiles of 64 consecutive march-grid
-// points: element (point p, plane k) of a slot sits at
-// ((p / 64) * CF_COUNT + k) * 64 + p % 64, so a wave reads every tap of its
+// points: per tile the CF_NCPLX complex planes, then the CF_NIMAG
+// imaginary-only planes (64 scalars each), so a wave reads every tap of its
 // points as contiguous segments at offsets fixed at compile time.
 constexpr int kCfTile = 64;
+constexpr int kCfTileC2 = kCfTile * CF_NCPLX + kCfTile * CF_NIMAG / 2;  // one tile in complex elements
 __host__ __device__ __forceinline__ int64_t cf_slot_size(int64_t NP) {
-    return (NP + kCfTile - 1) / kCfTile * CF_COUNT * kCfTile;
+    return (NP + kCfTile - 1) / kCfTile * kCfTileC2;
 }
-// offset of plane 0 of point p inside its slot (p < 2^31 / CF_COUNT)
+// offset of complex plane 0 of point p inside its slot (p < 2^31 / kCfTileC2 * 64)
 __host__ __device__ __forceinline__ unsigned cf_point(unsigned p) {
-    return (p / kCfTile) * (unsigned)(CF_COUNT * kCfTile) + (p % kCfTile);
+    return (p / kCfTile) * (unsigned)kCfTileC2 + (p % kCfTile);
+}
+// mu*R plane K at the point whose complex plane 0 is at pc (lane = p % 64)
+template <int K, class T>
+__device__ __forceinline__ C2<T> cf_load(const C2<T> *pc, unsigned lane) {
+    constexpr int idx = cf_index(K);  // evaluated at compile time
+    if constexpr (cf_complex(K)) {
+        return pc[idx * kCfTile];
+    } else {
+        const T *pi = reinterpret_cast<const T *>(pc + CF_NCPLX * kCfTile) - lane;
+        return C2<T>{(T)0, pi[idx * kCfTile]};
+    }
 }
 
 // slab-material fields the coefficients read (FP64, per point of a tile)

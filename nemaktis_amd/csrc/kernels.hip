@@ -2,7 +2,8 @@
 //
 // Data layout in HBM
 //   director     lc[c + fd*(ix + Nx*(iy + Ny*iz))]   float64 (caller layout)
-//   coefficients cf[slot*cf_slot_size(NP) + cf_point(p) + c*64]  C2<T>, march grid
+//   coefficients cf[slot*cf_slot_size(NP) + cf_point(p) + c*64]  C2<T> (complex planes, then
+//                imaginary-only planes as T), march grid
 //   march state  F[(prop*2 + comp)*NP + p]            C2<T>, march grid
 //   output       E[(2*prop + comp)*Nfull + m]         complex128, full grid
 // with m = ix + Nx*iy on the full grid and p = jx + Mx*jy on the march grid
@@ -144,6 +145,18 @@ void launch_nref(const double *lc, const uint8_t *mask, const Geo &g, SlotParams
 // duplicate column and row (full-grid index Nx-1, Ny-1), which the D1 seam
 // rule reads at ix+1 = Mx or iy+1 = My, is evaluated separately.
 // ---------------------------------------------------------------------------
+// mu*R plane K and the following ones of one point, in the coefficient tile
+// layout (complex planes at o, imaginary-only planes at oi); the plane
+// indices are compile-time constants
+template <class T, int K>
+__device__ __forceinline__ void store_mu_r(C2<T> *o, T *oi, const double (&re)[CF_NMU], const double (&im)[CF_NMU],
+                                           double hz) {
+    constexpr int idx = cf_index(K);
+    if constexpr (cf_complex(K)) o[idx * kCfTile] = C2<T>{(T)(-hz * im[K]), (T)(hz * re[K])};
+    else oi[idx * kCfTile] = (T)(hz * re[K]);
+    if constexpr (K + 1 < CF_NMU) store_mu_r<T, K + 1>(o, oi, re, im, hz);
+}
+
 constexpr int kMatTx = 32, kMatTy = 8, kMatHx = kMatTx + 2, kMatHy = kMatTy + 2, kMatHalo = kMatHx * kMatHy;
 
 // slab averages of the fields the coefficients read, from the permittivity
@@ -276,7 +289,7 @@ __global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ l
     const int ym = g.py ? (iy == 0 ? g.My - 1 : iy - 1) : iy;
     const int yp = g.py ? (iy == g.My - 1 ? 0 : iy + 1) : iy;
 
-    double re[40], im[40];
+    double re[CF_NMU], im[CF_NMU];
 #pragma unroll
     for (int k = 0; k < 40; k++) re[k] = im[k] = 0.0;
 
@@ -388,8 +401,8 @@ __global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ l
     // store mu*R with mu = i*dZ/2
     const double hz = 0.5 * s.dZ;
     C2<T> *o = cfs + cf_point((unsigned)ip);
-#pragma unroll
-    for (int k = 0; k < 40; k++) o[k * kCfTile] = C2<T>{(T)(-hz * im[k]), (T)(hz * re[k])};
+    T *oi = reinterpret_cast<T *>(o + CF_NCPLX * kCfTile) - (unsigned)(ip % kCfTile);
+    store_mu_r<T, 0>(o, oi, re, im, hz);
     // (I - mu T) rows of the x-line systems (T = DX00, DX11) in the chunk-interleaved layout
     if (g.px && xt) {
         int pc, i;
@@ -757,6 +770,13 @@ __device__ __forceinline__ Nbr neighbours(const Geo &g, unsigned ix, unsigned iy
     return n;
 }
 
+// acc + c f for the coefficient of plane K (imaginary-only planes: c = i c.y)
+template <int K, class T>
+__device__ __forceinline__ C2<T> tap_mad(C2<T> acc, C2<T> c, C2<T> f) {
+    if constexpr (cf_complex(K)) return cmad(acc, c, f);
+    else return C2<T>{acc.x - c.y * f.y, acc.y + c.y * f.x};
+}
+
 // Gathered operands of one right-hand side: every coefficient and field
 // value of the point is loaded before any arithmetic, so that the loads of a
 // point overlap in one memory round trip.  Indices are 32-bit element
@@ -769,15 +789,21 @@ struct RhsGather {
 
     // tap I and the following ones; the tap table is evaluated at compile time
     template <int I>
-    __device__ __forceinline__ void load_taps(const C2<T> *__restrict__ pc, const C2<T> *__restrict__ f0,
-                                              const C2<T> *__restrict__ f1, const Nbr &n, unsigned jx, unsigned jy,
-                                              unsigned Mx) {
+    __device__ __forceinline__ void load_taps(const C2<T> *__restrict__ pc, unsigned lane,
+                                              const C2<T> *__restrict__ f0, const C2<T> *__restrict__ f1,
+                                              const Nbr &n, unsigned jx, unsigned jy, unsigned Mx) {
         constexpr Tap tp = rhs_taps(KIND, COMP).t[I];
         const unsigned col = tp.dx < 0 ? n.xm : tp.dx > 0 ? n.xp : jx;
         const unsigned row = tp.dy < 0 ? n.ym : tp.dy > 0 ? n.yp : jy;
-        c[I] = pc[tp.k * kCfTile];
+        c[I] = cf_load<tp.k>(pc, lane);
         f[I] = (tp.fc == 0 ? f0 : f1)[col + Mx * row];
-        if constexpr (I + 1 < N) load_taps<I + 1>(pc, f0, f1, n, jx, jy, Mx);
+        if constexpr (I + 1 < N) load_taps<I + 1>(pc, lane, f0, f1, n, jx, jy, Mx);
+    }
+    template <int I>
+    __device__ __forceinline__ C2<T> sum_taps(C2<T> a) const {
+        a = tap_mad<rhs_taps(KIND, COMP).t[I].k>(a, c[I], f[I]);
+        if constexpr (I + 1 < N) return sum_taps<I + 1>(a);
+        else return a;
     }
 
     __device__ __forceinline__ void load(unsigned jx, unsigned jy, const C2<T> *__restrict__ cfs,
@@ -786,15 +812,13 @@ struct RhsGather {
         (void)NP;
         const Nbr n = neighbours(g, jx, jy);
         const unsigned Mx = g.Mx, ip = jx + Mx * jy;
-        load_taps<0>(cfs + cf_point(ip), f0, f1, n, jx, jy, Mx);
+        load_taps<0>(cfs + cf_point(ip), ip % kCfTile, f0, f1, n, jx, jy, Mx);
         if constexpr (CTR >= 0) ctr = (CTR == 0 ? f0 : f1)[ip];
     }
     __device__ __forceinline__ C2<T> sum() const {
         C2<T> a = C2<T>{0, 0};
         if constexpr (CTR >= 0) a = ctr;
-#pragma unroll
-        for (int t = 0; t < N; t++) a = cmad(a, c[t], f[t]);
-        return a;
+        return sum_taps<0>(a);
     }
 };
 
@@ -808,25 +832,32 @@ struct RhsGather2 {
     C2<T> c[N], fa[N], fb[N], ca, cb;
 
     template <int I>
-    __device__ __forceinline__ void load_taps(const C2<T> *__restrict__ pc, const C2<T> *__restrict__ a0,
-                                              const C2<T> *__restrict__ a1, const C2<T> *__restrict__ b0,
-                                              const C2<T> *__restrict__ b1, const Nbr &n, unsigned jx, unsigned jy,
-                                              unsigned Mx) {
+    __device__ __forceinline__ void load_taps(const C2<T> *__restrict__ pc, unsigned lane,
+                                              const C2<T> *__restrict__ a0, const C2<T> *__restrict__ a1,
+                                              const C2<T> *__restrict__ b0, const C2<T> *__restrict__ b1,
+                                              const Nbr &n, unsigned jx, unsigned jy, unsigned Mx) {
         constexpr Tap tp = rhs_taps(KIND, COMP).t[I];
         const unsigned col = tp.dx < 0 ? n.xm : tp.dx > 0 ? n.xp : jx;
         const unsigned row = tp.dy < 0 ? n.ym : tp.dy > 0 ? n.yp : jy;
         const unsigned q = col + Mx * row;
-        c[I] = pc[tp.k * kCfTile];
+        c[I] = cf_load<tp.k>(pc, lane);
         fa[I] = (tp.fc == 0 ? a0 : a1)[q];
         fb[I] = (tp.fc == 0 ? b0 : b1)[q];
-        if constexpr (I + 1 < N) load_taps<I + 1>(pc, a0, a1, b0, b1, n, jx, jy, Mx);
+        if constexpr (I + 1 < N) load_taps<I + 1>(pc, lane, a0, a1, b0, b1, n, jx, jy, Mx);
+    }
+    template <int I>
+    __device__ __forceinline__ void sum_taps(C2<T> &x, C2<T> &y) const {
+        constexpr int K = rhs_taps(KIND, COMP).t[I].k;
+        x = tap_mad<K>(x, c[I], fa[I]);
+        y = tap_mad<K>(y, c[I], fb[I]);
+        if constexpr (I + 1 < N) sum_taps<I + 1>(x, y);
     }
     __device__ __forceinline__ void load(unsigned jx, unsigned jy, const C2<T> *__restrict__ cfs,
                                          const C2<T> *__restrict__ a0, const C2<T> *__restrict__ a1,
                                          const C2<T> *__restrict__ b0, const C2<T> *__restrict__ b1, const Geo &g) {
         const Nbr n = neighbours(g, jx, jy);
         const unsigned Mx = g.Mx, ip = jx + Mx * jy;
-        load_taps<0>(cfs + cf_point(ip), a0, a1, b0, b1, n, jx, jy, Mx);
+        load_taps<0>(cfs + cf_point(ip), ip % kCfTile, a0, a1, b0, b1, n, jx, jy, Mx);
         if constexpr (CTR >= 0) {
             ca = (CTR == 0 ? a0 : a1)[ip];
             cb = (CTR == 0 ? b0 : b1)[ip];
@@ -838,11 +869,7 @@ struct RhsGather2 {
             x = ca;
             y = cb;
         }
-#pragma unroll
-        for (int t = 0; t < N; t++) {
-            x = cmad(x, c[t], fa[t]);
-            y = cmad(y, c[t], fb[t]);
-        }
+        sum_taps<0>(x, y);
         ra = x;
         rb = y;
     }
@@ -1362,6 +1389,23 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     }
 }
 
+// the DY00 (comp 0) or DY11 (comp 1) planes of point q: the tridiagonal row
+// of the y-line system of that component (complex planes, selected without
+// a branch so that the rows of a chunk are fetched together)
+template <class T>
+__device__ __forceinline__ void dy_row(int comp, const C2<T> *cfs, unsigned q, C2<T> &a, C2<T> &b, C2<T> &c) {
+    static_assert(cf_complex(CF_DY + 0) && cf_complex(CF_DY + 1) && cf_complex(CF_DY + 2) &&
+                  cf_complex(CF_DY + 9) && cf_complex(CF_DY + 10) && cf_complex(CF_DY + 11) &&
+                  cf_index(CF_DY + 1) == cf_index(CF_DY + 0) + 1 && cf_index(CF_DY + 2) == cf_index(CF_DY + 0) + 2 &&
+                  cf_index(CF_DY + 10) == cf_index(CF_DY + 9) + 1 && cf_index(CF_DY + 11) == cf_index(CF_DY + 9) + 2,
+                  "y-line rows must be consecutive complex planes");
+    constexpr int i0 = cf_index(CF_DY + 0), i1 = cf_index(CF_DY + 9);
+    const C2<T> *pc = cfs + cf_point(q) + (comp == 0 ? i0 : i1) * kCfTile;
+    a = pc[0];
+    b = pc[kCfTile];
+    c = pc[2 * kCfTile];
+}
+
 // right-hand sides of component `comp` (uniform per block) of one
 // propagation at NPT points of one column x, rows j0, j0 + step, ... (rows
 // >= M skipped); every load of every point is issued before any sum
@@ -1472,13 +1516,11 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     const int tcomp = nc == 2 ? ct : comp;
     sd += ct * TL;
     const int rb = ct * P * W;  // this component's reduced-system entries
-    const int blk = tcomp == 0 ? 0 : 3;
-    const C *cdy = cfs + (CF_DY + blk * 3) * kCfTile;
     const int s = chunk_start(lp, p), n = chunk_len(lp, p) - 1;
     auto tri = [&](int j, C &a, C &b, C &c) {
         j = min(j, M - 1);  // rows past the line end are fetched but not used
-        const C *c3 = cdy + cf_point((unsigned)(x + g.Mx * j));
-        const C va = c3[0], vb = c3[kCfTile], vc = c3[2 * kCfTile];
+        C va, vb, vc;
+        dy_row<T>(tcomp, cfs, (unsigned)(x + g.Mx * j), va, vb, vc);
         a = C{-va.x, -va.y};
         b = C{(T)1 - vb.x, -vb.y};
         c = C{-vc.x, -vc.y};
@@ -1624,13 +1666,11 @@ __global__ __launch_bounds__(512) void k_ysolve_pair(const C2<T> *__restrict__ s
     const int pol = threadIdx.x / WP, t = threadIdx.x - pol * WP, cofs = t & (W - 1), p = t >> lw, x = x0 + cofs;
     const bool active = cofs < wv;
     C *sdl = sd + pol * TL;
-    const int blk = comp == 0 ? 0 : 3;
-    const C *cdy = cfs + (CF_DY + blk * 3) * kCfTile;
     const int s = chunk_start(lp, p), n = chunk_len(lp, p) - 1;
     auto tri = [&](int j, C &a, C &b, C &c) {
         j = min(j, M - 1);  // rows past the line end are fetched but not used
-        const C *c3 = cdy + cf_point((unsigned)(x + g.Mx * j));
-        const C va = c3[0], vb = c3[kCfTile], vc = c3[2 * kCfTile];
+        C va, vb, vc;
+        dy_row<T>(comp, cfs, (unsigned)(x + g.Mx * j), va, vb, vc);
         a = C{-va.x, -va.y};
         b = C{(T)1 - vb.x, -vb.y};
         c = C{-vc.x, -vc.y};
