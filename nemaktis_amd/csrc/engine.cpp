@@ -74,6 +74,13 @@ void compose_taps(const double *w, int o, double *w1, int &o1, double *w2, int &
     o2 = 2 * o;
 }
 
+// a 7-tap filter starting at offset o that returns its input unchanged
+bool identity_taps(const double *w, int o) {
+    for (int t = 0; t < 7; t++)
+        if (w[t] != (o + t == 0 ? 1.0 : 0.0)) return false;
+    return true;
+}
+
 struct Sizes {
     int nwl, nq, batch, nslots_max, nprops_max;
     int64_t Nfull, NP;
@@ -306,6 +313,7 @@ private:
     Tuning tune_{};
     std::vector<SlotParams> hsp_;
     std::vector<PropParams> hpp_;
+    bool unshifted_[3] = {false, false, false};  // by number of shifts of a boundary
     std::vector<hipEvent_t> evs_;
     struct Captured {
         hipGraphExec_t exec;
@@ -429,6 +437,15 @@ private:
                     pp.ph_re = std::cos(phi);
                     pp.ph_im = std::sin(phi);
                 }
+        // launches whose propagations are all unshifted (normal incidence) take the pointwise boundary
+        for (int ns = 1; ns <= 2; ns++) {
+            unshifted_[ns] = true;
+            for (const PropParams &pp : hpp_) {
+                const bool ix = !g_.px || identity_taps(ns == 2 ? pp.w2x : pp.w1x, ns == 2 ? pp.o2x : pp.o1x);
+                const bool iy = !g_.py || identity_taps(ns == 2 ? pp.w2y : pp.w1y, ns == 2 ? pp.o2y : pp.o1y);
+                unshifted_[ns] = unshifted_[ns] && ix && iy;
+            }
+        }
         timed(KF_SETUP, [&] {
             HIPCHK(hipMemcpyAsync(pp_.ptr, hpp_.data(), nprops * sizeof(PropParams), hipMemcpyHostToDevice, st_));
             C2<T> *F = fb_[0].ptr;
@@ -518,7 +535,7 @@ private:
             launch_bulk_entrance(lc_.ptr, mask_.ptr, g_, sp_.ptr, pp_.ptr, nprops, beam, p_.waist, p_.dx, p_.dy,
                                  E_.ptr, sz_.nq, bulk_.ptr, st_);
         }
-        timed(KF_BOUNDARY, [&] { launch_boundary<T>(other, cur, nullptr, CF(0), g_, pp_.ptr, nprops, 1, st_); });
+        timed(KF_BOUNDARY, [&] { launch_boundary<T>(other, cur, nullptr, CF(0), g_, pp_.ptr, nprops, 1, unshifted_[1], st_); });
         for (int k = 0; k < nsl; k++) {
             // x half-steps: other = (I - mu A_x)^-1 (I + mu A_y) cur by its Neumann series
             // y0 + y1 + ... + y_nw: y0 goes to A, the first Neumann launch adds it to y1
@@ -554,15 +571,15 @@ private:
             }
             // bulk plane k+1: the step's closing P_k, Sh applied to a copy
             if (bulk_.ptr) {
-                launch_boundary<T>(cur, bt_.ptr, CF(k), nullptr, g_, pp_.ptr, nprops, 1, st_);
+                launch_boundary<T>(cur, bt_.ptr, CF(k), nullptr, g_, pp_.ptr, nprops, 1, unshifted_[1], st_);
                 launch_bulk_plane<T>(bt_.ptr, g_, sp_.ptr, pp_.ptr, nprops, k + 1, sz_.nq, bulk_.ptr, st_);
             }
             // P_k, Sh | Sh, P_{k+1}
             if (k + 1 < nsl) {
                 HIPCHK(hipStreamWaitEvent(st_, ready[k + 1], 0));
-                timed(KF_BOUNDARY, [&] { launch_boundary<T>(cur, other, CF(k), CF(k + 1), g_, pp_.ptr, nprops, 2, st_); });
+                timed(KF_BOUNDARY, [&] { launch_boundary<T>(cur, other, CF(k), CF(k + 1), g_, pp_.ptr, nprops, 2, unshifted_[2], st_); });
             } else {
-                timed(KF_BOUNDARY, [&] { launch_boundary<T>(cur, other, CF(k), nullptr, g_, pp_.ptr, nprops, 1, st_); });
+                timed(KF_BOUNDARY, [&] { launch_boundary<T>(cur, other, CF(k), nullptr, g_, pp_.ptr, nprops, 1, unshifted_[1], st_); });
             }
             std::swap(cur, other);
             if (k + 2 < nsl) {

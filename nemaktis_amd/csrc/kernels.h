@@ -28,10 +28,11 @@ template <class T>
 void launch_pack_fields(const c64 *E, const Geo &g, const PropParams *pp, int nprops, int nq, C2<T> *F,
                         hipStream_t st);
 
-// slab boundary out = Pn Sh^nshift Pp in (cf_prev / cf_next may be null)
+// slab boundary out = Pn Sh^nshift Pp in (cf_prev / cf_next may be null);
+// unshifted: every propagation's shift filter is the identity (pointwise kernel)
 template <class T>
 void launch_boundary(const C2<T> *in, C2<T> *out, const C2<T> *cf_prev, const C2<T> *cf_next, const Geo &g,
-                     const PropParams *pp, int nprops, int nshift, hipStream_t st);
+                     const PropParams *pp, int nprops, int nshift, bool unshifted, hipStream_t st);
 // out = in + mu*A_y in (dir 1) or in + mu*A_x in (dir 0)
 template <class T>
 void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, const Geo &g, const PropParams *pp,

@@ -670,12 +670,51 @@ __global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, 
     }
 }
 
+// The same boundary when no propagation of the launch is shifted (normal
+// incidence: the Catmull-Rom filters are the identity), point by point
+template <class T>
+__global__ __launch_bounds__(256) void k_boundary_point(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
+                                                        const C2<T> *__restrict__ cf_prev,
+                                                        const C2<T> *__restrict__ cf_next, Geo g,
+                                                        const PropParams *__restrict__ pp) {
+    using C = C2<T>;
+    const unsigned q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (unsigned)g.NP) return;
+    const int prop0 = blockIdx.y * 2;
+    const int slot = pp[prop0].slot;
+    const size_t NP = (size_t)g.NP;
+    const C *f = in + (size_t)prop0 * 2 * NP;
+    C v[4] = {f[q], f[NP + q], f[2 * NP + q], f[3 * NP + q]};
+    auto apply = [&](const C *cfx) {
+        const C *c3 = cfx + (size_t)slot * cf_slot_size(g.NP) + CF_P * kCfTile + cf_point(q);
+        const C pxx = c3[0], pyy = c3[kCfTile], pxy = c3[2 * kCfTile];
+#pragma unroll
+        for (int pol = 0; pol < 2; pol++) {
+            const C a = pxx * v[2 * pol] + pxy * v[2 * pol + 1], b = pxy * v[2 * pol] + pyy * v[2 * pol + 1];
+            v[2 * pol] = a;
+            v[2 * pol + 1] = b;
+        }
+    };
+    if (cf_prev) apply(cf_prev);
+    if (cf_next) apply(cf_next);
+    C *o = out + (size_t)prop0 * 2 * NP;
+    o[q] = v[0];
+    o[NP + q] = v[1];
+    o[2 * NP + q] = v[2];
+    o[3 * NP + q] = v[3];
+}
+
 template <class T>
 void launch_boundary(const C2<T> *in, C2<T> *out, const C2<T> *cf_prev, const C2<T> *cf_next, const Geo &g,
-                     const PropParams *pp, int nprops, int nshift, hipStream_t st) {
+                     const PropParams *pp, int nprops, int nshift, bool unshifted, hipStream_t st) {
     if (nprops % 2) throw std::runtime_error("slab boundary: propagations come in polarisation pairs");
-    dim3 grid((g.Mx + kBTW - 1) / kBTW, (g.My + kBTH - 1) / kBTH, nprops / 2);
-    hipLaunchKernelGGL(k_boundary<T>, grid, dim3(256), 0, st, in, out, cf_prev, cf_next, g, pp, nshift);
+    if (unshifted) {
+        dim3 grid((unsigned)blocks_for(g.NP, 256), nprops / 2);
+        hipLaunchKernelGGL(k_boundary_point<T>, grid, dim3(256), 0, st, in, out, cf_prev, cf_next, g, pp);
+    } else {
+        dim3 grid((g.Mx + kBTW - 1) / kBTW, (g.My + kBTH - 1) / kBTH, nprops / 2);
+        hipLaunchKernelGGL(k_boundary<T>, grid, dim3(256), 0, st, in, out, cf_prev, cf_next, g, pp, nshift);
+    }
     NMK_LAUNCH_CHECK();
 }
 
@@ -2257,7 +2296,7 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
                                         const PropParams *, int, int, double, double, double, C2<T> *, hipStream_t);  \
     template void launch_pack_fields<T>(const c64 *, const Geo &, const PropParams *, int, int, C2<T> *, hipStream_t); \
     template void launch_boundary<T>(const C2<T> *, C2<T> *, const C2<T> *, const C2<T> *, const Geo &,             \
-                                     const PropParams *, int, int, hipStream_t);                                      \
+                                     const PropParams *, int, int, bool, hipStream_t);                                      \
     template void launch_explicit<T>(int, const C2<T> *, C2<T> *, const C2<T> *, const Geo &, const PropParams *,    \
                                      int, hipStream_t);                                                               \
     template void launch_bulk_plane<T>(const C2<T> *, const Geo &, const SlotParams *, const PropParams *, int, \
