@@ -106,7 +106,7 @@ def _save_cache(c):
 
 
 def cache_key(precision, Nx, Ny, nprops):
-    return "gfx950|solver-v8|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
+    return "gfx950|solver-v9|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
 
 
 def _time(ctx, tuning, reps):
