@@ -186,15 +186,17 @@ def _search(ctx, best, xs, ys, Mx, My, precision, reps):
         table.append((cand, ms))
         if ms < best_ms:
             best, best_ms = cand, ms
-    # y blocks holding both polarisations (one coefficient gather for the two)
-    for (yc, yw) in y_candidates(My, precision, pair=True):
-        if yw < 4:
-            continue
-        cand = dict(best, y_chunks=yc, y_cols=yw, y_pair=1)
-        ms = _time(ctx, cand, reps)
-        table.append((cand, ms))
-        if ms < best_ms:
-            best, best_ms = cand, ms
+    # y blocks holding both polarisations (one coefficient gather for the two),
+    # or both field components (whose right-hand sides have unequal tap counts)
+    for pair in (1, 2):
+        for (yc, yw) in y_candidates(My, precision, pair=True):
+            if yw < 4:
+                continue
+            cand = dict(best, y_chunks=yc, y_cols=yw, y_pair=pair)
+            ms = _time(ctx, cand, reps)
+            table.append((cand, ms))
+            if ms < best_ms:
+                best, best_ms = cand, ms
     # second pass over the x shapes with the y shape found (coordinate descent)
     for both, shapes in ((0, xs), (1, xs2), (2, xs2), (3, xs2)):
         for (xc, xr) in shapes:

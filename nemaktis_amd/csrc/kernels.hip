@@ -186,14 +186,23 @@ __device__ __forceinline__ void slab_phase(const double *a, const double *b, dou
     double dt = sqrt(dxy * dxy + diff * diff);
     // alpha = 1 + i*k: sin/cos of the complex argument alpha*dt = (dt, k*dt)
     const double kk = s.alpha_im;
-    double u = dt, vv = kk * dt;
-    double su = sin(u), cu = cos(u);
-    double chv = (vv == 0.0) ? 1.0 : cosh(vv), shv = (vv == 0.0) ? 0.0 : sinh(vv);
+    double su, cu, st, ct;
+    sincos(dt, &su, &cu);
+    sincos(tr, &st, &ct);
+    // the hyperbolic factors and the damping only in an absorbing medium (k is uniform per slot)
+    double chv = 1.0, shv = 0.0, amp = 1.0;
+    if (kk != 0.0) {
+        const double vv = kk * dt;
+        if (vv != 0.0) {
+            chv = cosh(vv);
+            shv = sinh(vv);
+        }
+        amp = exp(-kk * tr);
+    }
     c64 sinc = fabs(dt) < 1e-8 ? mk(1.0, 0.0) : mk(su * chv / dt, cu * shv / dt);
     c64 cs = mk(cu * chv, -su * shv);
     // exp(i*alpha*tr) = exp(-k*tr) * (cos tr, sin tr)
-    double amp = (kk == 0.0) ? 1.0 : exp(-kk * tr);
-    c64 ph = mk(amp * cos(tr), amp * sin(tr));
+    c64 ph = mk(amp * ct, amp * st);
     c64 ids = mk(-diff * sinc.y, diff * sinc.x);   // i*diff*sinc
     c64 ixs = mk(-dxy * sinc.y, dxy * sinc.x);     // i*dxy*sinc
     pxx = ph * (cs + ids);
