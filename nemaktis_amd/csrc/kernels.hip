@@ -148,13 +148,13 @@ void launch_nref(const double *lc, const uint8_t *mask, const Geo &g, SlotParams
 // mu*R plane K and the following ones of one point, in the coefficient tile
 // layout (complex planes at o, imaginary-only planes at oi); the plane
 // indices are compile-time constants
-template <class T, int K>
+template <class T, int K, int KEND>
 __device__ __forceinline__ void store_mu_r(C2<T> *o, T *oi, const double (&re)[CF_NMU], const double (&im)[CF_NMU],
                                            double hz) {
     constexpr int idx = cf_index(K);
     if constexpr (cf_complex(K)) o[idx * kCfTile] = C2<T>{(T)(-hz * im[K]), (T)(hz * re[K])};
     else oi[idx * kCfTile] = (T)(hz * re[K]);
-    if constexpr (K + 1 < CF_NMU) store_mu_r<T, K + 1>(o, oi, re, im, hz);
+    if constexpr (K + 1 < KEND) store_mu_r<T, K + 1, KEND>(o, oi, re, im, hz);
 }
 
 constexpr int kMatTx = 32, kMatTy = 8, kMatHx = kMatTx + 2, kMatHy = kMatTy + 2, kMatHalo = kMatHx * kMatHy;
@@ -310,6 +310,11 @@ __global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ l
     // For each stencil slot: D of the four blocks, the walk-off part i*W, then
     // D.Sinv (Sinv at the column point) and Sinv.D (Sinv at the row point),
     // accumulated in the reference's order.
+    // mu*R with mu = i*dZ/2 is stored family by family, so that only one
+    // family's accumulators are live at a time
+    const double hz = 0.5 * s.dZ;
+    C2<T> *o = cfs + cf_point((unsigned)ip);
+    T *oi = reinterpret_cast<T *>(o + CF_NCPLX * kCfTile) - (unsigned)(ip % kCfTile);
     if (g.px) {
         // ---- DX family: slots 0 minus, 1 centre, 2 plus ----
         const double e_mx = at(SM_EINV, xm, iy), e_px = at(SM_EINV, ix + 1, iy);  // ix+1: duplicate at the seam
@@ -341,6 +346,21 @@ __global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ l
                 re[CF_DX + (2 + cb) * 3 + sl] += 0.25 * ixy_r * D[0 + cb];
                 re[CF_DX + (2 + cb) * 3 + sl] += 0.25 * iyy_r * D[2 + cb];
             }
+        }
+    }
+    store_mu_r<T, CF_DX, CF_DY>(o, oi, re, im, hz);
+    // (I - mu T) rows of the x-line systems (T = DX00, DX11) in the chunk-interleaved layout
+    if (g.px && xt) {
+        int pc, i;
+        chunk_of(lpx, ix, pc, i);
+        const int64_t SL = (int64_t)lpx.lmax * lpx.P;
+        for (int c2 = 0; c2 < 2; c2++) {
+            const int k0 = CF_DX + (c2 == 0 ? 0 : 3) * 3;
+            const int64_t base = ((((int64_t)slot * 2 + c2) * 3) * g.My + iy) * SL + (int64_t)i * lpx.P + pc;
+            const int64_t kst = (int64_t)g.My * SL;
+            xt[base] = C2<T>{(T)(hz * im[k0 + 0]), (T)(-hz * re[k0 + 0])};
+            xt[base + kst] = C2<T>{(T)(1.0 + hz * im[k0 + 1]), (T)(-hz * re[k0 + 1])};
+            xt[base + 2 * kst] = C2<T>{(T)(hz * im[k0 + 2]), (T)(-hz * re[k0 + 2])};
         }
     }
     if (g.py) {
@@ -376,6 +396,7 @@ __global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ l
             }
         }
     }
+    store_mu_r<T, CF_DY, CF_XY>(o, oi, re, im, hz);
     if (g.px && g.py) {
         // ---- DXDY family: slots 0 mxmy, 1 pxmy, 2 mxpy, 3 pxpy ----
         const double dxy = dX * dY;
@@ -407,25 +428,7 @@ __global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ l
             }
         }
     }
-    // store mu*R with mu = i*dZ/2
-    const double hz = 0.5 * s.dZ;
-    C2<T> *o = cfs + cf_point((unsigned)ip);
-    T *oi = reinterpret_cast<T *>(o + CF_NCPLX * kCfTile) - (unsigned)(ip % kCfTile);
-    store_mu_r<T, 0>(o, oi, re, im, hz);
-    // (I - mu T) rows of the x-line systems (T = DX00, DX11) in the chunk-interleaved layout
-    if (g.px && xt) {
-        int pc, i;
-        chunk_of(lpx, ix, pc, i);
-        const int64_t SL = (int64_t)lpx.lmax * lpx.P;
-        for (int c2 = 0; c2 < 2; c2++) {
-            const int k0 = CF_DX + (c2 == 0 ? 0 : 3) * 3;
-            const int64_t base = ((((int64_t)slot * 2 + c2) * 3) * g.My + iy) * SL + (int64_t)i * lpx.P + pc;
-            const int64_t kst = (int64_t)g.My * SL;
-            xt[base] = C2<T>{(T)(hz * im[k0 + 0]), (T)(-hz * re[k0 + 0])};
-            xt[base + kst] = C2<T>{(T)(1.0 + hz * im[k0 + 1]), (T)(-hz * re[k0 + 1])};
-            xt[base + 2 * kst] = C2<T>{(T)(hz * im[k0 + 2]), (T)(-hz * re[k0 + 2])};
-        }
-    }
+    store_mu_r<T, CF_XY, CF_NMU>(o, oi, re, im, hz);
 }
 
 template <class T>
