@@ -1287,6 +1287,38 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     const PairFields<T> fin(src, prop0, NP);
     const int nrow = min(R, g.My - row0), bs = blockDim.x, ntot = nrow * M;
 
+    // a thread stores at most LMAX/2 points of the block's nc * nrow * M
+    // (nrow*M <= R*P*LMAX, 2*nc*R*P threads); element e is point e % ntot of
+    // component e / ntot.  The accumulator values of the Neumann launches are
+    // fetched here, ahead of the gathers, so their round trip overlaps the solve.
+    constexpr int NO = LMAX / 2;
+    C o0v[NO], o1v[NO];
+    int rr0 = (int)threadIdx.x / M, ex0 = (int)threadIdx.x - rr0 * M, cc0 = 0;
+    while (rr0 >= nrow) {
+        rr0 -= nrow;
+        cc0++;
+    }
+    auto step_rc = [&](int &rr, int &x, int &cc) {
+        advance_rc(rr, x, bs, M);
+        while (rr >= nrow) {
+            rr -= nrow;
+            cc++;
+        }
+    };
+    const size_t pbase = (size_t)prop0 * 2 * NP;
+    if (NEU) {
+        int rr = rr0, x = ex0, cc = cc0;
+#pragma unroll
+        for (int k = 0; k < NO; k++) {
+            if (cc < nc) {
+                const size_t o = pbase + (size_t)(nc == 2 ? cc : comp) * NP + x + M * (row0 + rr);
+                o0v[k] = accin[o];
+                o1v[k] = accin[o + 2 * NP];
+            }
+            step_rc(rr, x, cc);
+        }
+    }
+
     // right-hand sides of the block's rows, both polarisations
     for (int c = 0; c < nc; c++) {
         constexpr int KIND = NEU ? RHS_OX : RHS_YPLUS, NPR = LEAN ? 1 : rhs_points<T, LMAX>();
@@ -1386,40 +1418,7 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     __syncthreads();
 
     {
-        // a thread stores at most LMAX/2 points of the block's nc * nrow * M
-        // (nrow*M <= R*P*LMAX, 2*nc*R*P threads); element e is point e % ntot of
-        // component e / ntot; the accumulator values of all of them are fetched
-        // in one round trip
-        constexpr int NO = LMAX / 2;
-        C o0v[NO], o1v[NO];
-        int rr = (int)threadIdx.x / M, x = (int)threadIdx.x - rr * M, cc = 0;
-        while (rr >= nrow) {
-            rr -= nrow;
-            cc++;
-        }
-        auto step = [&]() {
-            advance_rc(rr, x, bs, M);
-            while (rr >= nrow) {
-                rr -= nrow;
-                cc++;
-            }
-        };
-        const size_t pbase = (size_t)prop0 * 2 * NP;
-        if (NEU) {
-            const int r1 = rr, x1 = x, c1 = cc;
-#pragma unroll
-            for (int k = 0; k < NO; k++) {
-                if (cc < nc) {
-                    const size_t o = pbase + (size_t)(nc == 2 ? cc : comp) * NP + x + M * (row0 + rr);
-                    o0v[k] = accin[o];
-                    o1v[k] = accin[o + 2 * NP];
-                }
-                step();
-            }
-            rr = r1;
-            x = x1;
-            cc = c1;
-        }
+        int rr = rr0, x = ex0, cc = cc0;
 #pragma unroll
         for (int k = 0; k < NO; k++) {
             if (cc < nc) {
@@ -1435,7 +1434,7 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
                     acc[o + 2 * NP] = NEU ? o1v[k] + v1 : v1;
                 }
             }
-            step();
+            step_rc(rr, x, cc);
         }
     }
 }
@@ -1542,6 +1541,19 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     const int cc = tl & (W - 1), j0 = tl >> (__ffs(W) - 1);
     const bool col_ok = cc < wv;
 
+    // accumulator values of the Neumann launches (a thread stores at most LMAX
+    // points of its column, M <= P*LMAX), fetched ahead of the gathers so that
+    // their round trip overlaps the solve
+    C ov[LMAX];
+    if (NEU && col_ok) {
+        const size_t ob = ((size_t)prop * 2 + (nc == 2 ? ct : comp)) * NP;
+#pragma unroll
+        for (int k = 0; k < LMAX; k++) {
+            const int j = j0 + k * P;
+            if (j < M) ov[k] = accin[ob + x0 + cc + g.Mx * j];
+        }
+    }
+
     // right-hand sides of the tile: with nc = 2 every thread gathers rows of
     // both components (rows ct*P + j0, + 2P, ...), so the two components'
     // unequal tap counts are shared by all threads
@@ -1611,17 +1623,8 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     __syncthreads();
 
     {
-        // a thread stores at most LMAX points of its column (M <= P*LMAX);
-        // the accumulator values of all of them are fetched in one round trip
+        // a thread stores at most LMAX points of its column (M <= P*LMAX)
         const size_t ob = ((size_t)prop * 2 + tcomp) * NP;
-        C ov[LMAX];
-        if (NEU && col_ok) {
-#pragma unroll
-            for (int k = 0; k < LMAX; k++) {
-                const int j = j0 + k * P;
-                if (j < M) ov[k] = accin[ob + x0 + cc + g.Mx * j];
-            }
-        }
         if (col_ok) {
 #pragma unroll
             for (int k = 0; k < LMAX; k++) {
@@ -1688,6 +1691,22 @@ __global__ __launch_bounds__(512) void k_ysolve_pair(const C2<T> *__restrict__ s
     // staging and output: a thread keeps column cc and steps 2P rows at a time
     const int cc = threadIdx.x & (W - 1), jr = (int)threadIdx.x >> lw;
     const bool col_ok = cc < wv;
+    // accumulator values of the Neumann launches (at most LMAX/2 rows of each
+    // polarisation per thread, rows step 2P), fetched ahead of the gathers
+    const size_t o0 = ((size_t)prop0 * 2 + comp) * NP, o1 = o0 + 2 * NP;
+    constexpr int NO = (LMAX + 1) / 2;
+    C o0v[NO], o1v[NO];
+    if (NEU && col_ok) {
+#pragma unroll
+        for (int k = 0; k < NO; k++) {
+            const int j = jr + k * 2 * P;
+            if (j < M) {
+                const unsigned q = x0 + cc + g.Mx * j;
+                o0v[k] = accin[o0 + q];
+                o1v[k] = accin[o1 + q];
+            }
+        }
+    }
 
     if (col_ok) {
         constexpr int KIND = NEU ? RHS_OY : RHS_XPLUS, NPR = LEAN ? 1 : rhs_points<T, LMAX>();
@@ -1763,20 +1782,6 @@ __global__ __launch_bounds__(512) void k_ysolve_pair(const C2<T> *__restrict__ s
     {
         // a thread stores at most LMAX/2 points of its column for each
         // polarisation (M <= P*LMAX, rows step 2P); accumulators in one round trip
-        const size_t o0 = ((size_t)prop0 * 2 + comp) * NP, o1 = o0 + 2 * NP;
-        constexpr int NO = (LMAX + 1) / 2;
-        C o0v[NO], o1v[NO];
-        if (NEU && col_ok) {
-#pragma unroll
-            for (int k = 0; k < NO; k++) {
-                const int j = jr + k * 2 * P;
-                if (j < M) {
-                    const unsigned q = x0 + cc + g.Mx * j;
-                    o0v[k] = accin[o0 + q];
-                    o1v[k] = accin[o1 + q];
-                }
-            }
-        }
         if (col_ok) {
 #pragma unroll
             for (int k = 0; k < NO; k++) {
