@@ -653,11 +653,11 @@ Tuning default_tuning(int precision, int Mx, int My) {
     };
     // chunks of at most 8 points (16 for lines longer than 1024)
     t.x_chunks = std::min(128, std::max(16, pow2_at_least((Mx + 7) / 8)));
-    // first half-step launches: both field components per block (their
-    // right-hand sides have 6 and 14 taps; blocks holding both carry equal
-    // work); Neumann launches (11 and 3 taps) one component per block
+    // both field components per block in every x launch (their right-hand
+    // sides have 6 and 14 taps in the first launch, 11 and 3 in the Neumann
+    // launches; blocks holding both carry equal work)
     t.x_rows = 1;
-    t.x_both = 1;
+    t.x_both = 3;
     if (4 * t.x_chunks > 512 || xlds(t.x_chunks, 1, 2) > budget) {
         t.x_both = 0;
         t.x_rows = 2;

@@ -179,7 +179,7 @@ def percentiles(values, qs=(50, 99)):
     return {q: float(np.percentile(v, q)) if v.size else float("nan") for q in qs}
 
 
-def run_trace(server, specs, cache=None, precision=32, wl_batch=0, keep_outputs=False, log=None):
+def run_trace(server, specs, cache=None, precision=32, wl_batch=0, keep_outputs=False, log=None, use_graph=False):
     """Replays the arrivals of ``specs`` against ``server`` (directors taken
     from / put into ``cache`` before the clock starts) and returns the
     latency percentiles and the aggregate throughput."""
@@ -196,7 +196,7 @@ def run_trace(server, specs, cache=None, precision=32, wl_batch=0, keep_outputs=
         if delay > 0:
             time.sleep(delay)
         jobs.append((s, server.submit(js, lc, mask_vals=mask, mask_formula=formula or "", precision=precision,
-                                      wl_batch=wl_batch)))
+                                      wl_batch=wl_batch, use_graph=use_graph)))
         if log:
             log("submitted job %d %s %s %d wl x %d q" % (s["job"], s["mesh"], s["generator"], len(s["wavelengths"]),
                                                         len(s["wavevectors"])))

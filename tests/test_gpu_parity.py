@@ -209,6 +209,7 @@ def test_random512_reduced_depth_fp32_vs_fp64():
     {"x_chunks": 16, "x_rows": 1, "y_chunks": 16, "y_cols": 4, "x_both": 3, "y_pair": 2},  # both components
     {"x_chunks": 8, "x_rows": 2, "y_chunks": 8, "y_cols": 2, "x_both": 2, "y_pair": 2, "lean": 1},
     {"x_chunks": 32, "x_rows": 4, "y_chunks": 16, "y_cols": 8, "x_both": 3},  # four rows of both components
+    {"x_chunks": 16, "x_rows": 1, "y_chunks": 16, "y_cols": 8, "x_both": 1},  # both components, first launches only
 ])
 def test_line_solver_shapes_fp64(tuning):
     """Every autotuner candidate shape must give the oracle's answer."""
