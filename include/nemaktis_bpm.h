@@ -64,7 +64,7 @@ typedef struct {
     int32_t item_order;  /*   0: by batch size; 1 + bits: bit 0 x components, bit 1
                               y components, bit 2 polarisation pairs adjacent
                               in the XCD work ranges                            */
-    int32_t lean;        /*   1: register-lean solver variants (large batches)   */
+    int32_t lean;        /*   1: register-lean solver variants                   */
     int32_t y_pair;      /*   1: y-line blocks hold both polarisations,
                               2: both field components of one polarisation     */
     int32_t x_both;      /*   x-line blocks hold both field components: bit 0 for
@@ -77,8 +77,9 @@ typedef struct {
     double updates;                                  /* voxel-slice updates of one run */
     double updates_per_s;
     int64_t footprint_bytes;                         /* HBM held by the run */
-    /* profile mode: device time per kernel family, indexed 0 material,
-       1 coefficients, 2 slab boundary, 3 reserved, 4 explicit half-step,
+    /* profile mode: device time per kernel family, indexed 0 material (now
+       evaluated inside the coefficient kernel: always 0), 1 material and
+       coefficients, 2 slab boundary, 3 reserved, 4 explicit half-step,
        5 x-line solves, 6 y-line solves, 7 output, 8 screen filter, 9 setup */
     double family_ms[10];
     int64_t family_launches[10];
