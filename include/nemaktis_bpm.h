@@ -70,6 +70,8 @@ typedef struct {
     int32_t x_both;      /*   x-line blocks hold both field components: bit 0 for
                               the first launch of a half-step, bit 1 for the
                               Neumann launches                                 */
+    int32_t coef_serial; /*   1: slab coefficients on the march stream instead
+                              of a second stream (no joins between streams)    */
 } nmk_bpm_options;
 
 typedef struct {

@@ -21,7 +21,7 @@ class Options(ctypes.Structure):
                 ("device", ctypes.c_int32), ("profile", ctypes.c_int32), ("x_chunks", ctypes.c_int32),
                 ("x_rows", ctypes.c_int32), ("y_chunks", ctypes.c_int32), ("y_cols", ctypes.c_int32),
                 ("item_order", ctypes.c_int32), ("lean", ctypes.c_int32), ("y_pair", ctypes.c_int32),
-                ("x_both", ctypes.c_int32)]
+                ("x_both", ctypes.c_int32), ("coef_serial", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -163,7 +163,7 @@ def dptr(a):
 
 def options(precision=64, wl_batch=1, use_graph=False, device=-1, profile=False, tuning=None):
     """tuning: None (built-in shapes) or dict(x_chunks, x_rows, y_chunks, y_cols[, order 0..7][, lean][, y_pair]
-    [, x_both])"""
+    [, x_both][, coef_serial])"""
     o = Options()
     load().nmk_bpm_default_options(ctypes.byref(o))
     o.precision = int(precision)
@@ -179,6 +179,7 @@ def options(precision=64, wl_batch=1, use_graph=False, device=-1, profile=False,
         o.lean = 1 if tuning.get("lean") else 0
         o.y_pair = int(tuning.get("y_pair") or 0)
         o.x_both = int(tuning.get("x_both") or 0) & 3
+        o.coef_serial = 1 if tuning.get("coef_serial") else 0
     return o
 
 

@@ -106,7 +106,7 @@ def _save_cache(c):
 
 
 def cache_key(precision, Nx, Ny, nprops):
-    return "gfx950|solver-v9|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
+    return "gfx950|solver-v10|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
 
 
 def _time(ctx, tuning, reps):
@@ -234,4 +234,10 @@ def _search(ctx, best, xs, ys, Mx, My, precision, reps):
                 table.append((cand, ms))
                 if ms < best_ms:
                     best, best_ms = cand, ms
+    # slab coefficients on the march stream (no stream joins at the slab boundaries)
+    cand = dict(best, coef_serial=1)
+    ms = _time(ctx, cand, reps)
+    table.append((cand, ms))
+    if ms < best_ms:
+        best, best_ms = cand, ms
     return best, best_ms, table, t0

@@ -44,6 +44,7 @@ Options to_options(const nmk_bpm_options *o) {
             r.tuning.lean = o->lean ? 1 : 0;
             r.tuning.y_pair = o->y_pair == 1 || o->y_pair == 2 ? o->y_pair : 0;
             r.tuning.x_both = o->x_both & 3;
+            r.tuning.coef_serial = o->coef_serial ? 1 : 0;
         }
     }
     return r;

@@ -498,14 +498,16 @@ private:
 
     // The z-march.  Coefficient planes of slab k+1 are computed on a second
     // stream (double-buffered) while slab k marches; the slab boundary
-    // kernel joins the two streams.
+    // kernel joins the two streams.  Each join costs the march ~10 us at the
+    // boundary, so small slabs may take the coefficients on the march stream
+    // instead (coef_serial, chosen by measurement).
     C2<T> *march_launches(int nslots, int nprops) {
         const int nsl = p_.Nz - 1, nw = p_.n_woodbury;
         const size_t cfsz = (size_t)sz_.nslots_max * cf_slot_size(sz_.NP);
         const size_t xtsz = xt_.count / 2;
         auto CF = [&](int k) { return cf_.ptr + (size_t)(k % 2) * cfsz; };
         auto XT = [&](int k) { return xt_.ptr ? xt_.ptr + (size_t)(k % 2) * xtsz : nullptr; };
-        const bool overlap = st2_ != nullptr;
+        const bool overlap = st2_ != nullptr && !tune_.coef_serial;
         hipStream_t cs = overlap ? st2_ : st_;
         size_t evn = 0;
         auto record = [&](hipStream_t s) {

@@ -21,6 +21,7 @@ struct Tuning {
     int lean = 0;                  // 1: register-lean solver variants (see launch_xsolve)
     int y_pair = 0;                // 1: y-line blocks hold both polarisations, 2: both components
     int x_both = 0;                // x-line blocks hold both field components: bit 0 first launch, bit 1 Neumann
+    int coef_serial = 0;           // 1: slab coefficients on the march stream (no second stream)
 };
 Tuning default_tuning(int precision, int Mx, int My);
 

@@ -210,6 +210,7 @@ def test_random512_reduced_depth_fp32_vs_fp64():
     {"x_chunks": 8, "x_rows": 2, "y_chunks": 8, "y_cols": 2, "x_both": 2, "y_pair": 2, "lean": 1},
     {"x_chunks": 32, "x_rows": 4, "y_chunks": 16, "y_cols": 8, "x_both": 3},  # four rows of both components
     {"x_chunks": 16, "x_rows": 1, "y_chunks": 16, "y_cols": 8, "x_both": 1},  # both components, first launches only
+    {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8, "coef_serial": 1},  # coefficients on the march stream
 ])
 def test_line_solver_shapes_fp64(tuning):
     """Every autotuner candidate shape must give the oracle's answer."""
@@ -237,7 +238,8 @@ def test_retuned_context_matches_fresh_context():
     shape = fields.output_shape(s)
     n_E = int(np.prod(shape))
     t1 = {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8}
-    t2 = {"x_chunks": 8, "x_rows": 1, "y_chunks": 8, "y_cols": 4, "x_both": 3, "y_pair": 2, "order": 5}
+    t2 = {"x_chunks": 8, "x_rows": 1, "y_chunks": 8, "y_cols": 4, "x_both": 3, "y_pair": 2, "order": 5,
+          "coef_serial": 1}
     with bpm_backend.Context(js, lc, n_E, precision=32, wl_batch=0, use_graph=True, tuning=t1) as ctx:
         ctx.propagate()
         ctx.set_tuning(t2)
