@@ -375,7 +375,7 @@ def main():
         tuning64 = None
         if not args.no_tune:
             from nemaktis_amd import autotune
-            tuning64, _ = autotune.tune(s, lc, precision=64, wl_batch=1, use_cache=True, log=log)
+            tuning64, _ = autotune.tune(s, lc, precision=64, wl_batch=1, use_cache=True, log=log, use_graph=False)
         wall64, dev64, st64, out64 = run_gpu_leg(js, lc, n_E, shape, 64, 1, False, k64, 1, tuning64)
         errs = rel_errors(out32, out64)
         fp64 = {"value": updates * k64 / wall64, "ms_per_step": 1e3 * wall64 / k64,

@@ -105,8 +105,8 @@ def _save_cache(c):
         pass
 
 
-def cache_key(precision, Nx, Ny, nprops):
-    return "gfx950|solver-v10|%d|%dx%d|%d" % (precision, Nx, Ny, nprops)
+def cache_key(precision, Nx, Ny, nprops, use_graph=True):
+    return "gfx950|solver-v10|%d|%dx%d|%d|%s" % (precision, Nx, Ny, nprops, "graph" if use_graph else "eager")
 
 
 def _time(ctx, tuning, reps):
@@ -124,15 +124,16 @@ def _time(ctx, tuning, reps):
     return float(np.median(t))
 
 
-def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache=True, log=None):
+def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache=True, log=None, use_graph=True):
     """Returns the best tuning dict for this job's transverse mesh, measured
-    on a `depth`-plane copy of it, plus the timing table."""
+    on a `depth`-plane copy of it (graph replay or eager launches, as the job
+    will run), plus the timing table."""
     ic = settings["Physics settings"]["Initial conditions"]
     Nx, Ny, Nz = ic["Mesh dimensions"]
     co = settings["Physics settings"]["Coefficients"]
     nwl = len(co["Wavelengths"])
     nprops = (nwl if wl_batch <= 0 else min(wl_batch, nwl)) * (len(co["Wavevectors"]) // 2) * 2
-    key = cache_key(precision, Nx, Ny, nprops)
+    key = cache_key(precision, Nx, Ny, nprops, use_graph)
     cache = _load_cache() if use_cache else {}
     if key in cache:
         return cache[key]["tuning"], cache[key].get("table", [])
@@ -150,7 +151,7 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
     x0 = next((x for x in xs if _lds(Mx, x[0], x[1], c, 2, True) <= 64 * 1024), xs[0])
     y0 = next((y for y in ys if _lds(My, y[0], y[1], c, 1, True) <= 64 * 1024), ys[0])
     best = {"x_chunks": x0[0], "x_rows": x0[1], "y_chunks": y0[0], "y_cols": y0[1]}
-    ctx = bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, use_graph=True, tuning=best)
+    ctx = bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, use_graph=use_graph, tuning=best)
     try:
         best, best_ms, table, t0 = _search(ctx, best, xs, ys, Mx, My, precision, reps)
     finally:
