@@ -77,8 +77,8 @@ void launch_view_filter(c64 *filt, int Nx, int Ny, double dx, double dy, const V
 // backward transform.  Work items run component-fastest and each XCD takes a
 // contiguous range, so the four components reading one filter tile share L2.
 // ---------------------------------------------------------------------------
-template <int C, bool ODD>
-__global__ __launch_bounds__(256) void k_view_columns(c64 *__restrict__ data, const c64 *__restrict__ filt, int Nx,
+template <int C, bool ODD, int NT = 256>
+__global__ __launch_bounds__(NT) void k_view_columns(c64 *__restrict__ data, const c64 *__restrict__ filt, int Nx,
                                                       int ngroups, int64_t nitems, FftPlan pl,
                                                       const c64 *__restrict__ tw) {
     extern __shared__ c64 fbuf[];
@@ -94,12 +94,12 @@ __global__ __launch_bounds__(256) void k_view_columns(c64 *__restrict__ data, co
     const c64 *fp = filt + lq * N;
     const int x0 = g * C;
     c64 *stw = fbuf + 2 * C * n;
-    // the filter values this thread multiplies, loaded with the fields (n * C <= 8 * 256)
-    constexpr int kPer = 8;
+    // the filter values this thread multiplies, loaded with the fields (n * C <= 2048)
+    constexpr int kPer = 2048 / NT;
     c64 fr[kPer];
 #pragma unroll
     for (int e = 0; e < kPer; e++) {
-        const int t = threadIdx.x + e * 256;
+        const int t = threadIdx.x + e * NT;
         const int i = t % C, y = t / C;
         fr[e] = (t < n * C && x0 + i < Nx) ? fp[(int64_t)y * Nx + x0 + i] : mk(0.0, 0.0);
     }
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void k_view_columns(c64 *__restrict__ data, co
     c64 *a = fft_stages<C, false, ODD>(fbuf, fbuf + n * C, pl, stw, -1);
 #pragma unroll
     for (int e = 0; e < kPer; e++) {
-        const int t = threadIdx.x + e * 256;
+        const int t = threadIdx.x + e * NT;
         if (t < n * C) a[t] = a[t] * fr[e];
     }
     __syncthreads();
@@ -125,12 +125,22 @@ __global__ __launch_bounds__(256) void k_view_columns(c64 *__restrict__ data, co
 
 void launch_view_columns(c64 *data, const c64 *filt, int Nx, int Ny, int nlq, const c64 *tw_y, const FftPlan &py,
                          hipStream_t st) {
-    const int C = fft_cols_per_block(Ny);
+    // power-of-two columns of at most 256 points: eight adjacent columns per
+    // block of 512 threads, so that a row of the block is a whole 128-byte line
+    const bool wide = fft_pow2(py) && 8 * Ny <= 2048 && Nx >= 8;
+    const int C = wide ? 8 : fft_cols_per_block(Ny);
     if (Ny * C > 8 * 256) throw std::runtime_error("refocus columns: more than 2048 values per block");
     const int ngroups = (Nx + C - 1) / C;
     const int64_t nitems = (int64_t)ngroups * nlq * 4;
     const unsigned grid = (unsigned)((nitems + 7) / 8 * 8);
     const size_t lds = fft_lds_bytes(Ny, C);
+    if (wide) {
+        allow_kernel_lds(reinterpret_cast<const void *>(k_view_columns<8, false, 512>), lds);
+        hipLaunchKernelGGL((k_view_columns<8, false, 512>), dim3(grid), dim3(512), lds, st, data, filt, Nx, ngroups,
+                           nitems, py, tw_y);
+        NMK_VIEW_LAUNCH_CHECK();
+        return;
+    }
     switch (C) {
 #define NMK_VCOLS(CC)                                                                                         \
     case CC:                                                                                                  \
