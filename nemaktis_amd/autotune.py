@@ -8,8 +8,8 @@ of the job (same transverse mesh, a few slabs), by coordinate descent: every
 x shape that fits the block limits (<= 512 threads, the device's LDS per
 block, chunks of at most 16 points; one or both field components per block),
 then every y shape (one and two
-polarisations per block), x again, the register-lean variants, and the eight
-XCD work orders.  The winner is cached per (device, precision, mesh,
+polarisations per block), x again, the register-lean variants, the eight
+XCD work orders, and the slab coefficients on their own stream.  The winner is cached per (device, precision, mesh,
 propagation count).
 """
 import json
@@ -106,7 +106,7 @@ def _save_cache(c):
 
 
 def cache_key(precision, Nx, Ny, nprops, use_graph=True):
-    return "gfx950|solver-v10|%d|%dx%d|%d|%s" % (precision, Nx, Ny, nprops, "graph" if use_graph else "eager")
+    return "gfx950|solver-v11|%d|%dx%d|%d|%s" % (precision, Nx, Ny, nprops, "graph" if use_graph else "eager")
 
 
 def _time(ctx, tuning, reps):
@@ -150,7 +150,7 @@ def tune(settings, lc_vals, precision=32, wl_batch=0, depth=9, reps=3, use_cache
     c = 8 if precision == 32 else 16
     x0 = next((x for x in xs if _lds(Mx, x[0], x[1], c, 2, True) <= 64 * 1024), xs[0])
     y0 = next((y for y in ys if _lds(My, y[0], y[1], c, 1, True) <= 64 * 1024), ys[0])
-    best = {"x_chunks": x0[0], "x_rows": x0[1], "y_chunks": y0[0], "y_cols": y0[1]}
+    best = {"x_chunks": x0[0], "x_rows": x0[1], "y_chunks": y0[0], "y_cols": y0[1], "coef_serial": 1}
     ctx = bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, use_graph=use_graph, tuning=best)
     try:
         best, best_ms, table, t0 = _search(ctx, best, xs, ys, Mx, My, precision, reps)
@@ -235,8 +235,9 @@ def _search(ctx, best, xs, ys, Mx, My, precision, reps):
                 table.append((cand, ms))
                 if ms < best_ms:
                     best, best_ms = cand, ms
-    # slab coefficients on the march stream (no stream joins at the slab boundaries)
-    cand = dict(best, coef_serial=1)
+    # slab coefficients on a second stream, overlapping the march (at the cost
+    # of two stream joins per slab)
+    cand = dict(best, coef_serial=0)
     ms = _time(ctx, cand, reps)
     table.append((cand, ms))
     if ms < best_ms:

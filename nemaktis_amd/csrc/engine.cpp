@@ -179,7 +179,7 @@ public:
         partials_.alloc((size_t)kNrefBlocks * sz_.nslots_max);
         cf_.alloc(2 * (size_t)sz_.nslots_max * cf_slot_size(sz_.NP));
         if (!o.profile) HIPCHK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
-        set_tuning(o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, g_.Mx, g_.My));
+        set_tuning(o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, g_.Mx, g_.My, sz_.nprops_max));
         for (auto &b : fb_) b.alloc((size_t)sz_.nprops_max * 2 * sz_.NP);
         E_.alloc((size_t)sz_.nwl * sz_.nq * 4 * sz_.Nfull);
         if (p.bulk) {
@@ -527,9 +527,13 @@ private:
         if (nsl > 1) ready[1] = coef(1);
 
         C2<T> *cur = fb_[1].ptr, *other = fb_[0].ptr, *A = fb_[2].ptr, *B = fb_[3].ptr;
-        // XCD work order: a lone polarisation pair shares L2 best across the two
-        // components, larger batches across neighbouring rows/strips (measured)
-        const int ord = tune_.order >= 0 ? tune_.order : (nprops <= 2 ? 3 : 4);
+        // XCD work order (measured): a lone polarisation pair shares L2 best
+        // across the two components; batches whose field planes total up to
+        // about 32 MiB do best with the pairs innermost (the wavevectors of a
+        // wavelength read the same coefficient rows on one XCD), larger ones
+        // with the pairs outermost (each XCD keeps to its own pairs' fields)
+        const size_t plane_bytes = (size_t)nprops * g_.Nx * g_.Ny * sizeof(C2<T>);
+        const int ord = tune_.order >= 0 ? tune_.order : nprops <= 2 ? 3 : plane_bytes <= (size_t(32) << 20) ? 6 : 0;
         HIPCHK(hipStreamWaitEvent(st_, ready[0], 0));
         // bulk plane 0: the entrance fields after the input interface
         if (bulk_.ptr) {
@@ -622,7 +626,7 @@ private:
 
     void retune(const Tuning &t) override {
         HIPCHK(hipStreamSynchronize(st_));
-        set_tuning(t.x_chunks > 0 ? t : default_tuning(o_.precision, g_.Mx, g_.My));
+        set_tuning(t.x_chunks > 0 ? t : default_tuning(o_.precision, g_.Mx, g_.My, sz_.nprops_max));
     }
 
     void output_batch(int w0, int nslots, int nprops, const C2<T> *state) {
@@ -642,8 +646,20 @@ private:
 
 }  // namespace
 
-Tuning default_tuning(int precision, int Mx, int My) {
+// Built-in line-solver shapes.  The chunks per line trade parallelism
+// (about 128 K chunk threads fill the GPU) against the length of the
+// per-chunk chains (at most 16 points, 8 in FP64, whose longer chains spill
+// registers): what the autotuner picks at 512^2 and 1024^2 with two
+// propagations (64 chunks) and on the 294-propagation 256^2 sweep (16).
+Tuning default_tuning(int precision, int Mx, int My, int nprops) {
     Tuning t;
+    const int lcap = precision == 32 ? 16 : 8;
+    auto chunks = [&](int M, int other) {
+        const int64_t lines = (int64_t)std::max(1, other) * 2 * std::max(2, nprops);
+        int par = 1;
+        while (par < 64 && (int64_t)par * lines < 131072) par *= 2;
+        return std::min(512, std::max(16, std::max(pow2_at_least((M + lcap - 1) / lcap), par)));
+    };
     const size_t budget = max_block_lds();
     auto xlds = [&](int chunks, int rows, int nc) {
         const LinePlan lp = make_line_plan(Mx, chunks);
@@ -653,8 +669,7 @@ Tuning default_tuning(int precision, int Mx, int My) {
         const LinePlan lp = make_line_plan(My, chunks);
         return precision == 32 ? ysolve_lds_bytes<float>(lp, cols, 0) : ysolve_lds_bytes<double>(lp, cols, 0);
     };
-    // chunks of at most 8 points (16 for lines longer than 1024)
-    t.x_chunks = std::min(128, std::max(16, pow2_at_least((Mx + 7) / 8)));
+    t.x_chunks = chunks(Mx, My);
     // both field components per block in every x launch (their right-hand
     // sides have 6 and 14 taps in the first launch, 11 and 3 in the Neumann
     // launches; blocks holding both carry equal work)
@@ -667,10 +682,13 @@ Tuning default_tuning(int precision, int Mx, int My) {
             while (t.x_rows > 1 && (2 * t.x_chunks * t.x_rows > 512 || xlds(t.x_chunks, t.x_rows, 1) > budget))
                 t.x_rows /= 2;
     }
-    t.y_chunks = std::min(128, std::max(16, pow2_at_least((My + 7) / 8)));
-    t.y_cols = 8;
+    t.y_chunks = chunks(My, Mx);
+    t.y_cols = std::max(1, std::min(32, 512 / t.y_chunks));
     if (My > 3)
         while (t.y_cols > 1 && (t.y_chunks * t.y_cols > 512 || ylds(t.y_chunks, t.y_cols) > budget)) t.y_cols /= 2;
+    // slab coefficients on the march stream: as fast or faster than the second
+    // stream on every measured batch (up to 12% on small ones, in graph replay)
+    t.coef_serial = 1;
     return t;
 }
 
@@ -694,7 +712,7 @@ int64_t estimate_footprint(const Problem &p, const Options &o) {
     add((int64_t)kNrefBlocks * s.nslots_max * 8);
     add(2 * (int64_t)s.nslots_max * cf_slot_size(s.NP) * fsz);
     if (p.Nx > 3) {
-        const Tuning t = o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, Mx, My);
+        const Tuning t = o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, Mx, My, s.nprops_max);
         const LinePlan lp = make_line_plan(Mx, t.x_chunks);
         add(2 * (int64_t)s.nslots_max * 2 * 3 * My * lp.lmax * lp.P * fsz);
     }

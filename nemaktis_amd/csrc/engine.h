@@ -23,7 +23,7 @@ struct Tuning {
     int x_both = 0;                // x-line blocks hold both field components: bit 0 first launch, bit 1 Neumann
     int coef_serial = 0;           // 1: slab coefficients on the march stream (no second stream)
 };
-Tuning default_tuning(int precision, int Mx, int My);
+Tuning default_tuning(int precision, int Mx, int My, int nprops);
 
 // source of a runner's device memory: hipMalloc when none is given, or a
 // caller-owned region (the serving loop hands every admitted job a slice of

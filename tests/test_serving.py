@@ -62,16 +62,16 @@ def test_isolation_alone_and_in_the_mix():
 def test_graph_jobs_match_eager_jobs():
     """Jobs captured as graphs (their buffers carved from the arena) give the
     eager jobs' results bit for bit, alone and beside each other, with the
-    coefficients on a second stream or on the march stream."""
+    coefficients on the march stream (built-in) or on a second stream."""
     a = _job((64, 64, 12), seed=4)
     b = _job((96, 80, 10), seed=5, gen="droplet")
-    t = {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8, "coef_serial": 1}
+    t = {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8, "coef_serial": 0}
     with serving.Server(max_active=3) as srv:
         print("[serving] eager A, B", flush=True)
         ea = srv.submit(a[0], a[1], precision=32)
         eb = srv.submit(b[0], b[1], mask_vals=b[2], mask_formula=b[3], precision=32)
         assert ea.wait()["state"] == "done" and eb.wait()["state"] == "done"
-        print("[serving] graph A, B, A (serial coefficients)", flush=True)
+        print("[serving] graph A, B, A (coefficients on a second stream)", flush=True)
         ga = srv.submit(a[0], a[1], precision=32, use_graph=True)
         gb = srv.submit(b[0], b[1], mask_vals=b[2], mask_formula=b[3], precision=32, use_graph=True)
         gs = srv.submit(a[0], a[1], precision=32, use_graph=True, tuning=t)
