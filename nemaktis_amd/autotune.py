@@ -106,7 +106,7 @@ def _save_cache(c):
 
 
 def cache_key(precision, Nx, Ny, nprops, use_graph=True):
-    return "gfx950|solver-v11|%d|%dx%d|%d|%s" % (precision, Nx, Ny, nprops, "graph" if use_graph else "eager")
+    return "gfx950|solver-v12|%d|%dx%d|%d|%s" % (precision, Nx, Ny, nprops, "graph" if use_graph else "eager")
 
 
 def _time(ctx, tuning, reps):

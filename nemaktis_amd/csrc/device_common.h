@@ -167,20 +167,32 @@ __host__ __device__ constexpr int cf_index(int k) {
 constexpr int CF_NCPLX = 16 + 3;  // complex planes: the 16 complex mu*R planes, then P
 constexpr int CF_P = 16;          // complex plane of op_xx (op_yy, op_xy follow)
 constexpr int CF_NIMAG = 24;      // imaginary-only planes
-// Coefficient planes are stored in tiles of 64 consecutive march-grid
-// points: per tile the CF_NCPLX complex planes, then the CF_NIMAG
-// imaginary-only planes (64 scalars each), so a wave reads every tap of its
-// points as contiguous segments at offsets fixed at compile time.
+// Coefficient planes are stored in tiles of kCfTw x kCfTh = 16 x 4 march-grid
+// points (row-major in the tile): per tile the CF_NCPLX complex planes, then
+// the CF_NIMAG imaginary-only planes (64 scalars each), so a wave reads every
+// tap of its points as contiguous segments at offsets fixed at compile time.
+// Tiles are row-major over the grid, ceil(Mx / kCfTw) per tile row.  A y-line
+// wave (8 columns x 8 rows) then touches 2 tiles instead of 8 (64 x 1
+// tiles), while x-line waves (64 columns of one row) still read whole
+// 128-byte segments of every complex plane.
 constexpr int kCfTile = 64;
+constexpr int kCfTw = 16, kCfTh = kCfTile / kCfTw;
+static_assert(kCfTw * kCfTh == kCfTile && (kCfTw & (kCfTw - 1)) == 0, "tile shape");
 constexpr int kCfTileC2 = kCfTile * CF_NCPLX + kCfTile * CF_NIMAG / 2;  // one tile in complex elements
-__host__ __device__ __forceinline__ int64_t cf_slot_size(int64_t NP) {
-    return (NP + kCfTile - 1) / kCfTile * kCfTileC2;
+__host__ __device__ __forceinline__ int64_t cf_slot_size(int64_t Mx, int64_t My) {
+    return ((Mx + kCfTw - 1) / kCfTw) * ((My + kCfTh - 1) / kCfTh) * kCfTileC2;
 }
-// offset of complex plane 0 of point p inside its slot (p < 2^31 / kCfTileC2 * 64)
-__host__ __device__ __forceinline__ unsigned cf_point(unsigned p) {
-    return (p / kCfTile) * (unsigned)kCfTileC2 + (p % kCfTile);
+// position of point (x, y) inside its tile
+__host__ __device__ __forceinline__ unsigned cf_lane(unsigned x, unsigned y) {
+    return (y % kCfTh) * kCfTw + x % kCfTw;
 }
-// mu*R plane K at the point whose complex plane 0 is at pc (lane = p % 64)
+// offset of complex plane 0 of point (x, y) inside its slot (slots stay below
+// 2^32 elements: at most 8.6e6 tiles)
+__host__ __device__ __forceinline__ unsigned cf_point(unsigned x, unsigned y, unsigned Mx) {
+    const unsigned tpr = (Mx + kCfTw - 1) / kCfTw;
+    return ((y / kCfTh) * tpr + x / kCfTw) * (unsigned)kCfTileC2 + cf_lane(x, y);
+}
+// mu*R plane K at the point whose complex plane 0 is at pc (lane = cf_lane of the point)
 template <int K, class T>
 __device__ __forceinline__ C2<T> cf_load(const C2<T> *pc, unsigned lane) {
     constexpr int idx = cf_index(K);  // evaluated at compile time

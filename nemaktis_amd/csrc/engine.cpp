@@ -177,7 +177,7 @@ public:
         sp_.alloc(sz_.nslots_max);
         pp_.alloc(sz_.nprops_max);
         partials_.alloc((size_t)kNrefBlocks * sz_.nslots_max);
-        cf_.alloc(2 * (size_t)sz_.nslots_max * cf_slot_size(sz_.NP));
+        cf_.alloc(2 * (size_t)sz_.nslots_max * cf_slot_size(g_.Mx, g_.My));
         if (!o.profile) HIPCHK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
         set_tuning(o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, g_.Mx, g_.My, sz_.nprops_max));
         for (auto &b : fb_) b.alloc((size_t)sz_.nprops_max * 2 * sz_.NP);
@@ -503,7 +503,7 @@ private:
     // instead (coef_serial, chosen by measurement).
     C2<T> *march_launches(int nslots, int nprops) {
         const int nsl = p_.Nz - 1, nw = p_.n_woodbury;
-        const size_t cfsz = (size_t)sz_.nslots_max * cf_slot_size(sz_.NP);
+        const size_t cfsz = (size_t)sz_.nslots_max * cf_slot_size(g_.Mx, g_.My);
         const size_t xtsz = xt_.count / 2;
         auto CF = [&](int k) { return cf_.ptr + (size_t)(k % 2) * cfsz; };
         auto XT = [&](int k) { return xt_.ptr ? xt_.ptr + (size_t)(k % 2) * xtsz : nullptr; };
@@ -710,7 +710,7 @@ int64_t estimate_footprint(const Problem &p, const Options &o) {
     add((int64_t)s.nslots_max * (int64_t)sizeof(SlotParams));
     add((int64_t)s.nprops_max * (int64_t)sizeof(PropParams));
     add((int64_t)kNrefBlocks * s.nslots_max * 8);
-    add(2 * (int64_t)s.nslots_max * cf_slot_size(s.NP) * fsz);
+    add(2 * (int64_t)s.nslots_max * cf_slot_size(Mx, My) * fsz);
     if (p.Nx > 3) {
         const Tuning t = o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, Mx, My, s.nprops_max);
         const LinePlan lp = make_line_plan(Mx, t.x_chunks);

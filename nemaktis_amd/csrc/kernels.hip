@@ -2,8 +2,8 @@
 //
 // Data layout in HBM
 //   director     lc[c + fd*(ix + Nx*(iy + Ny*iz))]   float64 (caller layout)
-//   coefficients cf[slot*cf_slot_size(NP) + cf_point(p) + c*64]  C2<T> (complex planes, then
-//                imaginary-only planes as T), march grid
+//   coefficients cf[slot*cf_slot_size(Mx, My) + cf_point(jx, jy, Mx) + c*64]  C2<T> (64-point
+//                tiles: complex planes, then imaginary-only planes as T), march grid
 //   march state  F[(prop*2 + comp)*NP + p]            C2<T>, march grid
 //   output       E[(2*prop + comp)*Nfull + m]         complex128, full grid
 // with m = ix + Nx*iy on the full grid and p = jx + Mx*jy on the march grid
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ l
     const int slot = blockIdx.y;
     const SlotParams s = sp[slot];
     const int64_t N = g.Nfull;
-    C2<T> *cfs = cf + (size_t)slot * cf_slot_size(g.NP);
+    C2<T> *cfs = cf + (size_t)slot * cf_slot_size(g.Mx, g.My);
 
     // material of the tile and its halo (periodic neighbours on the march grid)
     for (int h = threadIdx.x; h < kMatHalo; h += blockDim.x) {
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ l
             }
             c64 pxx, pyy, pxy;
             slab_phase(a, b, w, wpz, s, pxx, pyy, pxy);
-            C2<T> *o = cfs + cf_point((unsigned)(ox + g.Mx * oy));
+            C2<T> *o = cfs + cf_point(ox, oy, g.Mx);
             o[(CF_P + 0) * kCfTile] = C2<T>{(T)pxx.x, (T)pxx.y};
             o[(CF_P + 1) * kCfTile] = C2<T>{(T)pyy.x, (T)pyy.y};
             o[(CF_P + 2) * kCfTile] = C2<T>{(T)pxy.x, (T)pxy.y};
@@ -313,8 +313,8 @@ __global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ l
     // mu*R with mu = i*dZ/2 is stored family by family, so that only one
     // family's accumulators are live at a time
     const double hz = 0.5 * s.dZ;
-    C2<T> *o = cfs + cf_point((unsigned)ip);
-    T *oi = reinterpret_cast<T *>(o + CF_NCPLX * kCfTile) - (unsigned)(ip % kCfTile);
+    C2<T> *o = cfs + cf_point(ix, iy, g.Mx);
+    T *oi = reinterpret_cast<T *>(o + CF_NCPLX * kCfTile) - cf_lane(ix, iy);
     if (g.px) {
         // ---- DX family: slots 0 minus, 1 centre, 2 plus ----
         const double e_mx = at(SM_EINV, xm, iy), e_px = at(SM_EINV, ix + 1, iy);  // ix+1: duplicate at the seam
@@ -592,14 +592,14 @@ __global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, 
     }
     __syncthreads();
     const C *f = in + (size_t)prop0 * 2 * NP;  // planes: pol 0 (x, y), pol 1 (x, y)
-    const C *cp = cf_prev ? cf_prev + (size_t)p.slot * cf_slot_size(g.NP) + CF_P * kCfTile : nullptr;
+    const C *cp = cf_prev ? cf_prev + (size_t)p.slot * cf_slot_size(g.Mx, g.My) + CF_P * kCfTile : nullptr;
     // input window with the previous slab's phase operator applied
     for (int e = threadIdx.x; e < WH * WW; e += blockDim.x) {
         const int r = e / WW, c = e - r * WW;
         const unsigned q = colw[c] + (unsigned)g.Mx * roww[r];
         C a0 = f[q], b0 = f[NP + q], a1 = f[2 * NP + q], b1 = f[3 * NP + q];
         if (cp) {
-            const C *c3 = cp + cf_point(q);
+            const C *c3 = cp + cf_point(colw[c], roww[r], g.Mx);
             const C pxx = c3[0], pyy = c3[kCfTile], pxy = c3[2 * kCfTile];
             const C na0 = pxx * a0 + pxy * b0, nb0 = pxy * a0 + pyy * b0;
             const C na1 = pxx * a1 + pxy * b1, nb1 = pxy * a1 + pyy * b1;
@@ -650,7 +650,7 @@ __global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, 
     }
     __syncthreads();
     // y pass and the next slab's phase operator
-    const C *cn = cf_next ? cf_next + (size_t)p.slot * cf_slot_size(g.NP) + CF_P * kCfTile : nullptr;
+    const C *cn = cf_next ? cf_next + (size_t)p.slot * cf_slot_size(g.Mx, g.My) + CF_P * kCfTile : nullptr;
     C *o = out + (size_t)prop0 * 2 * NP;
     for (int e = threadIdx.x; e < kBTH * kBTW; e += blockDim.x) {
         const int r = e / kBTW, c = e - r * kBTW;
@@ -666,7 +666,7 @@ __global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, 
         }
         const unsigned q = x + (unsigned)g.Mx * y;
         if (cn) {
-            const C *c3 = cn + cf_point(q);
+            const C *c3 = cn + cf_point(x, y, g.Mx);
             const C pxx = c3[0], pyy = c3[kCfTile], pxy = c3[2 * kCfTile];
 #pragma unroll
             for (int pol = 0; pol < 2; pol++) {
@@ -697,8 +697,9 @@ __global__ __launch_bounds__(256) void k_boundary_point(const C2<T> *__restrict_
     const size_t NP = (size_t)g.NP;
     const C *f = in + (size_t)prop0 * 2 * NP;
     C v[4] = {f[q], f[NP + q], f[2 * NP + q], f[3 * NP + q]};
+    const unsigned qy = q / (unsigned)g.Mx, qx = q - qy * (unsigned)g.Mx;
     auto apply = [&](const C *cfx) {
-        const C *c3 = cfx + (size_t)slot * cf_slot_size(g.NP) + CF_P * kCfTile + cf_point(q);
+        const C *c3 = cfx + (size_t)slot * cf_slot_size(g.Mx, g.My) + CF_P * kCfTile + cf_point(qx, qy, g.Mx);
         const C pxx = c3[0], pyy = c3[kCfTile], pxy = c3[2 * kCfTile];
 #pragma unroll
         for (int pol = 0; pol < 2; pol++) {
@@ -863,7 +864,7 @@ struct RhsGather {
         (void)NP;
         const Nbr n = neighbours(g, jx, jy);
         const unsigned Mx = g.Mx, ip = jx + Mx * jy;
-        load_taps<0>(cfs + cf_point(ip), ip % kCfTile, f0, f1, n, jx, jy, Mx);
+        load_taps<0>(cfs + cf_point(jx, jy, Mx), cf_lane(jx, jy), f0, f1, n, jx, jy, Mx);
         if constexpr (CTR >= 0) ctr = (CTR == 0 ? f0 : f1)[ip];
     }
     __device__ __forceinline__ C2<T> sum() const {
@@ -908,7 +909,7 @@ struct RhsGather2 {
                                          const C2<T> *__restrict__ b0, const C2<T> *__restrict__ b1, const Geo &g) {
         const Nbr n = neighbours(g, jx, jy);
         const unsigned Mx = g.Mx, ip = jx + Mx * jy;
-        load_taps<0>(cfs + cf_point(ip), ip % kCfTile, a0, a1, b0, b1, n, jx, jy, Mx);
+        load_taps<0>(cfs + cf_point(jx, jy, Mx), cf_lane(jx, jy), a0, a1, b0, b1, n, jx, jy, Mx);
         if constexpr (CTR >= 0) {
             ca = (CTR == 0 ? a0 : a1)[ip];
             cb = (CTR == 0 ? b0 : b1)[ip];
@@ -986,7 +987,7 @@ __global__ __launch_bounds__(256) void k_explicit(const C2<T> *__restrict__ in, 
     const int prop = blockIdx.y;
     const int slot = pp[prop].slot;
     const unsigned NP = (unsigned)g.NP, ix = ip % (unsigned)g.Mx, iy = ip / (unsigned)g.Mx;
-    const C2<T> *c = cf + (size_t)slot * cf_slot_size(NP);
+    const C2<T> *c = cf + (size_t)slot * cf_slot_size(g.Mx, g.My);
     const C2<T> *f0 = in + ((size_t)prop * 2) * NP, *f1 = f0 + NP;
     constexpr int KIND = DIR == 1 ? RHS_YPLUS : RHS_XPLUS;
     RhsGather<T, KIND, 0> a;
@@ -1283,7 +1284,7 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     __syncthreads();
     const int slot = pp[prop0].slot;
     const size_t NP = (size_t)g.NP;
-    const C *cfs = cf + (size_t)slot * cf_slot_size(g.NP);
+    const C *cfs = cf + (size_t)slot * cf_slot_size(g.Mx, g.My);
     const PairFields<T> fin(src, prop0, NP);
     const int nrow = min(R, g.My - row0), bs = blockDim.x, ntot = nrow * M;
 
@@ -1443,14 +1444,15 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
 // of the y-line system of that component (complex planes, selected without
 // a branch so that the rows of a chunk are fetched together)
 template <class T>
-__device__ __forceinline__ void dy_row(int comp, const C2<T> *cfs, unsigned q, C2<T> &a, C2<T> &b, C2<T> &c) {
+__device__ __forceinline__ void dy_row(int comp, const C2<T> *cfs, unsigned x, unsigned y, unsigned Mx, C2<T> &a,
+                                       C2<T> &b, C2<T> &c) {
     static_assert(cf_complex(CF_DY + 0) && cf_complex(CF_DY + 1) && cf_complex(CF_DY + 2) &&
                   cf_complex(CF_DY + 9) && cf_complex(CF_DY + 10) && cf_complex(CF_DY + 11) &&
                   cf_index(CF_DY + 1) == cf_index(CF_DY + 0) + 1 && cf_index(CF_DY + 2) == cf_index(CF_DY + 0) + 2 &&
                   cf_index(CF_DY + 10) == cf_index(CF_DY + 9) + 1 && cf_index(CF_DY + 11) == cf_index(CF_DY + 9) + 2,
                   "y-line rows must be consecutive complex planes");
     constexpr int i0 = cf_index(CF_DY + 0), i1 = cf_index(CF_DY + 9);
-    const C2<T> *pc = cfs + cf_point(q) + (comp == 0 ? i0 : i1) * kCfTile;
+    const C2<T> *pc = cfs + cf_point(x, y, Mx) + (comp == 0 ? i0 : i1) * kCfTile;
     a = pc[0];
     b = pc[kCfTile];
     c = pc[2 * kCfTile];
@@ -1527,7 +1529,7 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     const int x0 = lb * W, prop = pair * 2 + pol;
     const int slot = pp[prop].slot;
     const size_t NP = (size_t)g.NP;
-    const C *cfs = cf + (size_t)slot * cf_slot_size(g.NP);
+    const C *cfs = cf + (size_t)slot * cf_slot_size(g.Mx, g.My);
     const C *f0 = src + (size_t)prop * 2 * NP, *f1 = f0 + NP;
     const int wv = min(W, g.Mx - x0);
     if ((int)threadIdx.x < P) {
@@ -1583,7 +1585,7 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     auto tri = [&](int j, C &a, C &b, C &c) {
         j = min(j, M - 1);  // rows past the line end are fetched but not used
         C va, vb, vc;
-        dy_row<T>(tcomp, cfs, (unsigned)(x + g.Mx * j), va, vb, vc);
+        dy_row<T>(tcomp, cfs, x, j, g.Mx, va, vb, vc);
         a = C{-va.x, -va.y};
         b = C{(T)1 - vb.x, -vb.y};
         c = C{-vc.x, -vc.y};
@@ -1679,7 +1681,7 @@ __global__ __launch_bounds__(512) void k_ysolve_pair(const C2<T> *__restrict__ s
     const int x0 = lb * W, prop0 = pair * 2;
     const int slot = pp[prop0].slot;
     const size_t NP = (size_t)g.NP;
-    const C *cfs = cf + (size_t)slot * cf_slot_size(g.NP);
+    const C *cfs = cf + (size_t)slot * cf_slot_size(g.Mx, g.My);
     const PairFields<T> fin(src, prop0, NP);
     const int wv = min(W, g.Mx - x0), lw = __ffs(W) - 1;
     if ((int)threadIdx.x < P) {
@@ -1740,7 +1742,7 @@ __global__ __launch_bounds__(512) void k_ysolve_pair(const C2<T> *__restrict__ s
     auto tri = [&](int j, C &a, C &b, C &c) {
         j = min(j, M - 1);  // rows past the line end are fetched but not used
         C va, vb, vc;
-        dy_row<T>(comp, cfs, (unsigned)(x + g.Mx * j), va, vb, vc);
+        dy_row<T>(comp, cfs, x, j, g.Mx, va, vb, vc);
         a = C{-va.x, -va.y};
         b = C{(T)1 - vb.x, -vb.y};
         c = C{-vc.x, -vc.y};
