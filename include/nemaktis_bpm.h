@@ -71,7 +71,8 @@ typedef struct {
                               the first launch of a half-step, bit 1 for the
                               Neumann launches                                 */
     int32_t coef_serial; /*   1: slab coefficients on the march stream instead
-                              of a second stream (no joins between streams)    */
+                              of a second stream (no joins between streams);
+                              the built-in shapes (x_chunks 0) always use 1    */
 } nmk_bpm_options;
 
 typedef struct {
