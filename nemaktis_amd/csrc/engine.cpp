@@ -651,6 +651,7 @@ private:
 // per-chunk chains (at most 16 points, 8 in FP64, whose longer chains spill
 // registers): what the autotuner picks at 512^2 and 1024^2 with two
 // propagations (64 chunks) and on the 294-propagation 256^2 sweep (16).
+// y strips are as wide as the block allows up to one coefficient tile.
 Tuning default_tuning(int precision, int Mx, int My, int nprops) {
     Tuning t;
     const int lcap = precision == 32 ? 16 : 8;
@@ -683,7 +684,8 @@ Tuning default_tuning(int precision, int Mx, int My, int nprops) {
                 t.x_rows /= 2;
     }
     t.y_chunks = chunks(My, Mx);
-    t.y_cols = std::max(1, std::min(32, 512 / t.y_chunks));
+    // strips at most one coefficient tile wide: a wave then reads whole tiles
+    t.y_cols = std::max(1, std::min(kCfTw, 512 / t.y_chunks));
     if (My > 3)
         while (t.y_cols > 1 && (t.y_chunks * t.y_cols > 512 || ylds(t.y_chunks, t.y_cols) > budget)) t.y_cols /= 2;
     // slab coefficients on the march stream: as fast or faster than the second
