@@ -659,7 +659,7 @@ Tuning default_tuning(int precision, int Mx, int My, int nprops) {
         const int64_t lines = (int64_t)std::max(1, other) * 2 * std::max(2, nprops);
         int par = 1;
         while (par < 64 && (int64_t)par * lines < 131072) par *= 2;
-        return std::min(512, std::max(16, std::max(pow2_at_least((M + lcap - 1) / lcap), par)));
+        return std::min(128, std::max(16, std::max(pow2_at_least((M + lcap - 1) / lcap), par)));
     };
     const size_t budget = max_block_lds();
     auto xlds = [&](int chunks, int rows, int nc) {
