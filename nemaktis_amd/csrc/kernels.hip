@@ -810,15 +810,19 @@ __host__ __device__ constexpr TapList rhs_taps(int kind, int comp) {
     return L;
 }
 
+// periodic neighbour columns of a point and the element offsets Mx*row of
+// its own row and the rows above and below (one multiply per point)
 struct Nbr {
-    unsigned xm, xp, ym, yp;
+    unsigned xm, xp, rm, r0, rp;
 };
 __device__ __forceinline__ Nbr neighbours(const Geo &g, unsigned ix, unsigned iy) {
     Nbr n;
-    n.xm = g.px ? (ix == 0 ? g.Mx - 1 : ix - 1) : ix;
-    n.xp = g.px ? (ix == (unsigned)g.Mx - 1 ? 0 : ix + 1) : ix;
-    n.ym = g.py ? (iy == 0 ? g.My - 1 : iy - 1) : iy;
-    n.yp = g.py ? (iy == (unsigned)g.My - 1 ? 0 : iy + 1) : iy;
+    const unsigned Mx = g.Mx;
+    n.xm = g.px ? (ix == 0 ? Mx - 1 : ix - 1) : ix;
+    n.xp = g.px ? (ix == Mx - 1 ? 0 : ix + 1) : ix;
+    n.r0 = Mx * iy;
+    n.rm = g.py ? (iy == 0 ? (unsigned)g.NP - Mx : n.r0 - Mx) : n.r0;
+    n.rp = g.py ? (iy == (unsigned)g.My - 1 ? 0 : n.r0 + Mx) : n.r0;
     return n;
 }
 
@@ -843,13 +847,13 @@ struct RhsGather {
     template <int I>
     __device__ __forceinline__ void load_taps(const C2<T> *__restrict__ pc, unsigned lane,
                                               const C2<T> *__restrict__ f0, const C2<T> *__restrict__ f1,
-                                              const Nbr &n, unsigned jx, unsigned jy, unsigned Mx) {
+                                              const Nbr &n, unsigned jx) {
         constexpr Tap tp = rhs_taps(KIND, COMP).t[I];
         const unsigned col = tp.dx < 0 ? n.xm : tp.dx > 0 ? n.xp : jx;
-        const unsigned row = tp.dy < 0 ? n.ym : tp.dy > 0 ? n.yp : jy;
+        const unsigned row = tp.dy < 0 ? n.rm : tp.dy > 0 ? n.rp : n.r0;
         c[I] = cf_load<tp.k>(pc, lane);
-        f[I] = (tp.fc == 0 ? f0 : f1)[col + Mx * row];
-        if constexpr (I + 1 < N) load_taps<I + 1>(pc, lane, f0, f1, n, jx, jy, Mx);
+        f[I] = (tp.fc == 0 ? f0 : f1)[col + row];
+        if constexpr (I + 1 < N) load_taps<I + 1>(pc, lane, f0, f1, n, jx);
     }
     template <int I>
     __device__ __forceinline__ C2<T> sum_taps(C2<T> a) const {
@@ -863,8 +867,8 @@ struct RhsGather {
                                          unsigned NP) {
         (void)NP;
         const Nbr n = neighbours(g, jx, jy);
-        const unsigned Mx = g.Mx, ip = jx + Mx * jy;
-        load_taps<0>(cfs + cf_point(jx, jy, Mx), cf_lane(jx, jy), f0, f1, n, jx, jy, Mx);
+        const unsigned ip = jx + n.r0;
+        load_taps<0>(cfs + cf_point(jx, jy, g.Mx), cf_lane(jx, jy), f0, f1, n, jx);
         if constexpr (CTR >= 0) ctr = (CTR == 0 ? f0 : f1)[ip];
     }
     __device__ __forceinline__ C2<T> sum() const {
@@ -887,15 +891,14 @@ struct RhsGather2 {
     __device__ __forceinline__ void load_taps(const C2<T> *__restrict__ pc, unsigned lane,
                                               const C2<T> *__restrict__ a0, const C2<T> *__restrict__ a1,
                                               const C2<T> *__restrict__ b0, const C2<T> *__restrict__ b1,
-                                              const Nbr &n, unsigned jx, unsigned jy, unsigned Mx) {
+                                              const Nbr &n, unsigned jx) {
         constexpr Tap tp = rhs_taps(KIND, COMP).t[I];
         const unsigned col = tp.dx < 0 ? n.xm : tp.dx > 0 ? n.xp : jx;
-        const unsigned row = tp.dy < 0 ? n.ym : tp.dy > 0 ? n.yp : jy;
-        const unsigned q = col + Mx * row;
+        const unsigned q = col + (tp.dy < 0 ? n.rm : tp.dy > 0 ? n.rp : n.r0);
         c[I] = cf_load<tp.k>(pc, lane);
         fa[I] = (tp.fc == 0 ? a0 : a1)[q];
         fb[I] = (tp.fc == 0 ? b0 : b1)[q];
-        if constexpr (I + 1 < N) load_taps<I + 1>(pc, lane, a0, a1, b0, b1, n, jx, jy, Mx);
+        if constexpr (I + 1 < N) load_taps<I + 1>(pc, lane, a0, a1, b0, b1, n, jx);
     }
     template <int I>
     __device__ __forceinline__ void sum_taps(C2<T> &x, C2<T> &y) const {
@@ -908,8 +911,8 @@ struct RhsGather2 {
                                          const C2<T> *__restrict__ a0, const C2<T> *__restrict__ a1,
                                          const C2<T> *__restrict__ b0, const C2<T> *__restrict__ b1, const Geo &g) {
         const Nbr n = neighbours(g, jx, jy);
-        const unsigned Mx = g.Mx, ip = jx + Mx * jy;
-        load_taps<0>(cfs + cf_point(jx, jy, Mx), cf_lane(jx, jy), a0, a1, b0, b1, n, jx, jy, Mx);
+        const unsigned ip = jx + n.r0;
+        load_taps<0>(cfs + cf_point(jx, jy, g.Mx), cf_lane(jx, jy), a0, a1, b0, b1, n, jx);
         if constexpr (CTR >= 0) {
             ca = (CTR == 0 ? a0 : a1)[ip];
             cb = (CTR == 0 ? b0 : b1)[ip];
