@@ -89,7 +89,9 @@ def measured_traffic(kernel, path=TRAFFIC_FILE):
 def probe(kind, steps):
     """Short runs for hardware-counter passes (rocprofv3 --pmc ... -- python
     bench.py --probe KIND): 'copy' copies 256 MiB device to device `steps`
-    times (counter calibration), 'march' runs the FP32 workload."""
+    times (counter calibration), 'march' runs the FP32 workload, 'march-droplet'
+    the 256x256x80 droplet sweep (294 propagations, FP32, built-in shapes),
+    'imaging' the refocus + micrograph step, all with eager launches."""
     if kind == "copy":
         import torch
         n = 64 << 20
@@ -112,11 +114,16 @@ def probe(kind, steps):
         of.close()
         return
     from nemaktis_amd import fields, autotune, bpm_backend
-    s, lc, _, _ = fields.workload("random512")
-    tuning, _ = autotune.tune(s, lc, precision=32, wl_batch=0, use_cache=True, log=log)
+    if kind == "march-droplet":
+        s, lc, mask, formula = fields.workload("droplet256")
+        tuning = None
+    else:
+        s, lc, mask, formula = fields.workload("random512")
+        tuning, _ = autotune.tune(s, lc, precision=32, wl_batch=0, use_cache=True, log=log)
     js = json.dumps(s)
     n_E = int(np.prod(fields.output_shape(s)))
-    with bpm_backend.Context(js, lc, n_E, precision=32, wl_batch=0, use_graph=False, tuning=tuning) as ctx:
+    with bpm_backend.Context(js, lc, n_E, mask_vals=mask, mask_formula=formula or "", precision=32, wl_batch=0,
+                             use_graph=False, tuning=tuning) as ctx:
         for _ in range(steps):
             st = ctx.propagate()
     log("march probe: %d propagations, last %.2f ms, tuning %s" % (steps, st["total_ms"], tuning))
@@ -328,7 +335,8 @@ def main():
     ap.add_argument("--no-imaging", action="store_true", help="skip the refocus + micrograph leg")
     ap.add_argument("--no-bulk", action="store_true", help="skip the bulk-output leg")
     ap.add_argument("--bulk-dir", default="/tmp/nemaktis_amd_bench", help="folder of the bulk-output file")
-    ap.add_argument("--probe", choices=["copy", "march", "imaging"], help="short run for a hardware-counter pass")
+    ap.add_argument("--probe", choices=["copy", "march", "march-droplet", "imaging"],
+                    help="short run for a hardware-counter pass")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
