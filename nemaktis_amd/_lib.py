@@ -162,7 +162,7 @@ def dptr(a):
 
 
 def options(precision=64, wl_batch=1, use_graph=False, device=-1, profile=False, tuning=None):
-    """tuning: None (built-in shapes) or dict(x_chunks, x_rows, y_chunks, y_cols[, order 0..7][, lean][, y_pair]
+    """tuning: None (built-in shapes) or dict(x_chunks, x_rows, y_chunks, y_cols[, order 0..15][, lean][, y_pair]
     [, x_both][, coef_serial])"""
     o = Options()
     load().nmk_bpm_default_options(ctypes.byref(o))

@@ -9,7 +9,8 @@ x shape that fits the block limits (<= 512 threads, the device's LDS per
 block, chunks of at most 16 points; one or both field components per block),
 then every y shape (one and two
 polarisations per block), x again, the register-lean variants, the eight
-XCD work orders, and the slab coefficients on their own stream.  The winner is cached per (device, precision, mesh,
+XCD work orders, the eight orders with each wavelength's wavevector pairs
+grouped, and the slab coefficients on their own stream.  The winner is cached per (device, precision, mesh,
 propagation count).
 """
 import json
@@ -106,7 +107,7 @@ def _save_cache(c):
 
 
 def cache_key(precision, Nx, Ny, nprops, use_graph=True):
-    return "gfx950|solver-v12|%d|%dx%d|%d|%s" % (precision, Nx, Ny, nprops, "graph" if use_graph else "eager")
+    return "gfx950|solver-v13|%d|%dx%d|%d|%s" % (precision, Nx, Ny, nprops, "graph" if use_graph else "eager")
 
 
 def _time(ctx, tuning, reps):
@@ -235,6 +236,14 @@ def _search(ctx, best, xs, ys, Mx, My, precision, reps):
                 table.append((cand, ms))
                 if ms < best_ms:
                     best, best_ms = cand, ms
+    # the wavevector pairs of each wavelength grouped (orders 8-15) with the
+    # shapes found
+    for order in range(8, 16):
+        cand = dict(best, order=order)
+        ms = _time(ctx, cand, reps)
+        table.append((cand, ms))
+        if ms < best_ms:
+            best, best_ms = cand, ms
     # slab coefficients on a second stream, overlapping the march (at the cost
     # of two stream joins per slab)
     cand = dict(best, coef_serial=0)

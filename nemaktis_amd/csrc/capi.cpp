@@ -40,7 +40,7 @@ Options to_options(const nmk_bpm_options *o) {
             r.tuning.x_rows = o->x_rows;
             r.tuning.y_chunks = o->y_chunks;
             r.tuning.y_cols = o->y_cols;
-            if (o->item_order > 0) r.tuning.order = (o->item_order - 1) & 7;
+            if (o->item_order > 0) r.tuning.order = (o->item_order - 1) & 15;
             r.tuning.lean = o->lean ? 1 : 0;
             r.tuning.y_pair = o->y_pair == 1 || o->y_pair == 2 ? o->y_pair : 0;
             r.tuning.x_both = o->x_both & 3;

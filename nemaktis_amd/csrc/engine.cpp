@@ -532,8 +532,15 @@ private:
         // about 32 MiB do best with the pairs innermost (the wavevectors of a
         // wavelength read the same coefficient rows on one XCD), larger ones
         // with the pairs outermost (each XCD keeps to its own pairs' fields)
+        // in FP32, and in FP64 with each wavelength's pairs grouped (bit 3:
+        // a third of the HBM fetches, which only the FP64 solvers turn into
+        // time)
         const size_t plane_bytes = (size_t)nprops * g_.Nx * g_.Ny * sizeof(C2<T>);
-        const int ord = tune_.order >= 0 ? tune_.order : nprops <= 2 ? 3 : plane_bytes <= (size_t(32) << 20) ? 6 : 0;
+        const int ord0 = tune_.order >= 0 ? tune_.order
+                         : nprops <= 2 ? 3
+                         : plane_bytes <= (size_t(32) << 20) ? 6
+                         : sizeof(T) == 8 ? 10 : 0;
+        const int ord = ord0 | (sz_.nq << 4);  // group size of order bit 3: the pairs of one slot
         HIPCHK(hipStreamWaitEvent(st_, ready[0], 0));
         // bulk plane 0: the entrance fields after the input interface
         if (bulk_.ptr) {

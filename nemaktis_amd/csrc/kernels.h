@@ -39,7 +39,9 @@ void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, cons
                      int nprops, hipStream_t st);
 // order: bit 0 puts the two components of an x row block next to each other
 // in the XCD work ranges (else neighbouring row blocks), bit 1 the same for
-// y strips, bit 2 puts the polarisation pairs innermost.
+// y strips, bit 2 puts the polarisation pairs innermost, bit 3 (before bit 2)
+// puts the pairs of one group innermost and the groups outermost, the group
+// size (pairs per wavelength slot) in bits 4 and up.
 // lean: one gather point per round and no row prefetch (fewer registers,
 // more waves per SIMD for large batches).  both (x lines): one block holds
 // both field components of its rows (equal work per block).  pair (y lines): both polarisations

@@ -1025,6 +1025,18 @@ void launch_explicit(int dir, const C2<T> *in, C2<T> *out, const C2<T> *cf, cons
 // chunk-boundary unknowns form a cyclic tridiagonal system solved by
 // parallel cyclic reduction in LDS, and the thread rebuilds its chunk.
 // ---------------------------------------------------------------------------
+// Work item `it` of a solver launch as (pair, rest) when the pairs come in
+// groups of G (the wavevector pairs of one wavelength) and each group sweeps
+// the nrest row blocks / strips with its G pairs innermost, so that one
+// wavelength's coefficient rows serve its pairs while they are in L2
+__device__ __forceinline__ void split_grouped(int it, int nrest, int npairs, int G, int &pair, int &rest) {
+    G = max(1, min(G, npairs));
+    const int grp = it / (G * nrest), r = it - grp * G * nrest;
+    const int gsz = min(G, npairs - grp * G);
+    pair = grp * G + r % gsz;
+    rest = r / gsz;
+}
+
 template <class T, int LMAX>
 struct Chunk {
     C2<T> gam[LMAX], yu[LMAX], yg[LMAX];
@@ -1262,7 +1274,9 @@ __global__ __launch_bounds__(512) void k_xsolve(const C2<T> *__restrict__ src, C
     const int item = xcd_tile(blockIdx.x, gridDim.x);
     if (item >= nrb * nci * nitems_pairs) return;
     int comp, row0, pair, rest;
-    if (order & 4) {  // pair fastest: the wavevectors of one wavelength share the coefficients
+    if (order & 8) {  // pairs of one wavelength innermost, wavelength groups outermost
+        split_grouped(item, nci * nrb, nitems_pairs, order >> 4, pair, rest);
+    } else if (order & 4) {  // pair fastest: the wavevectors of one wavelength share the coefficients
         pair = item % nitems_pairs;
         rest = item / nitems_pairs;
     } else {
@@ -1512,7 +1526,9 @@ __global__ __launch_bounds__(512) void k_ysolve(const C2<T> *__restrict__ src, C
     if (item >= nstrip * 2 * nci * npairs) return;
     const int pol = item & 1;
     int comp, lb, pair, rest;
-    if (order & 4) {
+    if (order & 8) {
+        split_grouped(item >> 1, nci * nstrip, npairs, order >> 4, pair, rest);
+    } else if (order & 4) {
         pair = (item >> 1) % npairs;
         rest = (item >> 1) / npairs;
     } else {
@@ -1667,7 +1683,9 @@ __global__ __launch_bounds__(512) void k_ysolve_pair(const C2<T> *__restrict__ s
     const int item = xcd_tile(blockIdx.x, gridDim.x);
     if (item >= nstrip * 2 * npairs) return;
     int comp, lb, pair, rest;
-    if (order & 4) {
+    if (order & 8) {
+        split_grouped(item, 2 * nstrip, npairs, order >> 4, pair, rest);
+    } else if (order & 4) {
         pair = item % npairs;
         rest = item / npairs;
     } else {

@@ -257,11 +257,12 @@ def test_retuned_context_matches_fresh_context():
 
 def test_xcd_work_orders_are_bitwise_equal():
     """The XCD work orders only move blocks between XCDs: identical output
-    (two wavelengths x two wavevectors, so that every index varies)."""
+    (two wavelengths x two wavevectors, so that every index varies; orders 8-15
+    group the two pairs of each wavelength)."""
     s, lc = _random_case((70, 66, 5), seed=19)
     base = {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8}
-    outs = [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o)) for o in range(8)]
-    outs += [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o, y_pair=1)) for o in (0, 6)]
+    outs = [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o)) for o in range(16)]
+    outs += [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o, y_pair=1)) for o in (0, 6, 8, 14)]
     outs += [_gpu(s, lc, precision=32, wl_batch=0, tuning=dict(base, order=o, x_both=3, x_rows=1, y_pair=2,
                                                                  y_cols=4)) for o in (1, 4)]
     for o in outs[1:]:
