@@ -63,7 +63,8 @@ typedef struct {
     int32_t y_cols;
     int32_t item_order;  /*   0: by batch size; 1 + bits: bit 0 x components, bit 1
                               y components, bit 2 polarisation pairs adjacent
-                              in the XCD work ranges                            */
+                              in the XCD work ranges, bit 3 the pairs of each
+                              wavelength adjacent (bits 0-3 only)              */
     int32_t lean;        /*   1: register-lean solver variants                   */
     int32_t y_pair;      /*   1: y-line blocks hold both polarisations,
                               2: both field components of one polarisation     */
