@@ -220,6 +220,20 @@ def test_line_solver_shapes_fp64(tuning):
     assert err < TOL64, err
 
 
+def test_builtin_shapes_mid_mesh_batched():
+    """Built-in launch shapes on a mesh with 299- and 259-point lines and 12
+    propagations (32 chunks per line, 16-column y strips, pairs-innermost
+    work order), FP32 graph replay and FP64, against the oracle."""
+    s = _base((300, 260, 4), wl=[0.5, 0.6, 0.7], wavevectors=[(0.0, 0.0), (0.1, 0.05)], no="1.5", ne="1.7",
+              nhost="1.5", na_objective=0.4)
+    lc = fields.random_smooth(300, 260, 4, (0.1, 0.1, 0.1), seed=31, corr_length=0.3)
+    want = oracle.propagate(s, lc)
+    err32 = _rel(_gpu(s, lc, precision=32, wl_batch=0, use_graph=True), want)
+    assert err32 < TOL32, err32
+    err64 = _rel(_gpu(s, lc, precision=64, wl_batch=0), want)
+    assert err64 < TOL64, err64
+
+
 def test_both_component_blocks_with_lds_reduced_systems():
     """x lines of 299 points in 128 chunks (reduced systems in LDS) with both
     components per block, against 64 chunks (lane permutes), FP64."""
