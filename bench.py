@@ -275,6 +275,28 @@ def rel_errors(a, b):
     return e.reshape(-1)
 
 
+def droplet_leg(steps, warmup):
+    """The 256x256x80 droplet sweep (21 wavelengths x 7 wavevectors x 2
+    polarisations = 294 propagations marched together, raw mask), FP32 with
+    graph replay and the built-in launch shapes: wall time per step with the
+    director resident in HBM."""
+    from nemaktis_amd import bpm_backend, fields
+    s, lc, mask, formula = fields.workload("droplet256")
+    n_E = int(np.prod(fields.output_shape(s)))
+    with bpm_backend.Context(json.dumps(s), lc, n_E, mask_vals=mask, mask_formula=formula or "", precision=32,
+                             wl_batch=0, use_graph=True) as ctx:
+        for _ in range(warmup):
+            ctx.propagate()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.propagate()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+    updates = fields.updates_per_run(s)
+    return {"workload": "droplet 256x256x80, 21 wavelengths x 7 wavevectors x 2 polarisations, FP32, graph, "
+                        "built-in shapes", "ms_per_step": ms, "value": updates / ms * 1e3,
+            "unit": "voxel-slice updates/s", "steps": steps, "warmup": warmup}
+
+
 def run_gpu_leg(js, lc, n_E, shape, precision, wl_batch, use_graph, steps, warmup, tuning=None):
     from nemaktis_amd import bpm_backend
     with bpm_backend.Context(js, lc, n_E, precision=precision, wl_batch=wl_batch, use_graph=use_graph,
@@ -334,6 +356,7 @@ def main():
     ap.add_argument("--serve-active", type=int, default=4, help="jobs on the device at once")
     ap.add_argument("--no-imaging", action="store_true", help="skip the refocus + micrograph leg")
     ap.add_argument("--no-bulk", action="store_true", help="skip the bulk-output leg")
+    ap.add_argument("--no-droplet", action="store_true", help="skip the 294-propagation droplet sweep leg")
     ap.add_argument("--bulk-dir", default="/tmp/nemaktis_amd_bench", help="folder of the bulk-output file")
     ap.add_argument("--probe", choices=["copy", "march", "march-droplet", "imaging"],
                     help="short run for a hardware-counter pass")
@@ -424,6 +447,11 @@ def main():
         log("bulk-output leg ...")
         bulk = bulk_leg(s, lc, tuning, args.bulk_dir)
 
+    droplet = None
+    if not args.no_droplet:
+        log("droplet sweep leg ...")
+        droplet = droplet_leg(3, 1)
+
     serve = None
     if not args.no_serve:
         log("serving loop trace ...")
@@ -444,6 +472,7 @@ def main():
         "serving": serve,
         "imaging": imaging,
         "bulk_output": bulk,
+        "droplet_sweep": droplet,
     }
     if cpu:
         line["vs_cpu"] = value / cpu["value"]
