@@ -17,7 +17,10 @@
  *                  overwritten in place with the micrograph-plane fields.
  * Sizes are element counts (n_E_vals counts complex values).  Every entry
  * point returns 0 on success and a non-zero code on failure, with the
- * message copied into err_msg (if non-NULL); nothing is printed.
+ * message copied into err_msg (if non-NULL); nothing is printed.  On success
+ * err_msg holds the notice the reference prints to stdout, if any (a mask
+ * formula that disagrees with the mask, UtilitiesLib/VectorField.cpp:107-111),
+ * else the empty string.
  */
 #ifndef NEMAKTIS_BPM_H
 #define NEMAKTIS_BPM_H
@@ -54,7 +57,8 @@ typedef struct {
     int32_t precision;   /* 64: complex128 march (default); 32: complex64 march
                             with FP64 permittivity / coefficient precompute */
     int32_t wl_batch;    /* wavelengths marched together; <= 0: all at once */
-    int32_t use_graph;   /* 1: capture the z-march as a hipGraph and replay it */
+    int32_t use_graph;   /* 1: capture the z-march as a hipGraph and replay it
+                            (contexts; one-shot nmk_bpm_run launches eagerly) */
     int32_t device;      /* HIP device ordinal, -1: current device */
     int32_t profile;     /* 1: time every kernel family (disables the graph) */
     int32_t x_chunks;    /* line-solver launch shapes; 0: built-in defaults      */
@@ -118,6 +122,17 @@ void nmk_bpm_destroy(nmk_bpm_ctx *ctx);
 /* HBM bytes a run of these settings would hold (serving admission). */
 int nmk_bpm_estimate_footprint(const char *json_str, int64_t n_lc_vals, int has_mask, const nmk_bpm_options *opts,
                                int64_t *bytes, char *err_msg, size_t err_len);
+
+/* Interface interpolation weights a masked run computes from its mask formula
+ * (UtilitiesLib/VectorField.cpp:64-113, which keeps them in
+ * VectorField::interp_weigths): weights[2*i] and weights[2*i+1] for voxel
+ * i = ix + Nx*(iy + Ny*iz), iz < Nz-1, n_weights = 2*Nx*Ny*(Nz-1).  When the
+ * formula's zero crossings disagree with the mask every weight is 0.5 and
+ * err_msg holds the reference's notice (status 0).  Host-only: no device is
+ * needed. */
+int nmk_bpm_interface_weights(const char *json_str, const char *mask_formula, const double *mask_vals,
+                              int64_t n_mask_vals, double *weights, int64_t n_weights, char *err_msg,
+                              size_t err_len);
 
 /* Evaluates an index expression (muParser syntax, variable `lambda` in um) as
  * the backend does for "no", "ne", ... (PhysicsLib/PhysicsCoefficients.cpp:27-103). */

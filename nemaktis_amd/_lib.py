@@ -94,7 +94,8 @@ WAVEPLATES = {"No": 0, "Quarter-wave": 1, "Half-wave": 2, "Tint-sensitive": 3}
 EXPORTED = [
     "nmk_bpm_run_backend_without_mask", "nmk_bpm_run_backend_with_mask", "nmk_bpm_default_options",
     "nmk_bpm_run", "nmk_bpm_create", "nmk_bpm_upload_fields", "nmk_bpm_propagate", "nmk_bpm_download",
-    "nmk_bpm_destroy", "nmk_bpm_set_tuning", "nmk_bpm_estimate_footprint", "nmk_bpm_eval_expression", "nmk_bpm_version",
+    "nmk_bpm_destroy", "nmk_bpm_set_tuning", "nmk_bpm_estimate_footprint", "nmk_bpm_interface_weights",
+    "nmk_bpm_eval_expression", "nmk_bpm_version",
     "nmk_srv_default_config", "nmk_srv_create", "nmk_srv_submit", "nmk_srv_wait", "nmk_srv_cancel",
     "nmk_srv_stats", "nmk_srv_destroy",
     "nmk_view_default_optics", "nmk_view_create", "nmk_view_from_ctx", "nmk_view_set_fields", "nmk_view_focus",
@@ -126,6 +127,7 @@ def load():
     lib.nmk_bpm_destroy.argtypes = [ctypes.c_void_p]
     lib.nmk_bpm_estimate_footprint.argtypes = [cp, i64, ctypes.c_int, ctypes.POINTER(Options),
                                                ctypes.POINTER(i64), cp, sz]
+    lib.nmk_bpm_interface_weights.argtypes = [cp, cp, c_double_p, i64, c_double_p, i64, cp, sz]
     lib.nmk_bpm_eval_expression.argtypes = [cp, ctypes.c_double, c_double_p, cp, sz]
     lib.nmk_bpm_version.restype = ctypes.c_char_p
     vp = ctypes.c_void_p

@@ -16,9 +16,12 @@ stderr in the reference's frame and the call returns normally
 pass ``strict=True``) to raise ``BpmBackendError`` instead.  Array-type
 violations raise ``TypeError`` as the SWIG typemaps do.
 
-Precision and batching options (``precision=32``, ``wl_batch=0``,
-``use_graph=True``) are extensions; the defaults run the FP64 path one
-wavelength at a time, as the reference does.
+Precision and batching options (``precision=32``, ``wl_batch=0``) are
+extensions; the defaults run the FP64 path one wavelength at a time, as the
+reference does.  One-shot calls launch their kernels eagerly: a graph captured
+for one march would be replayed once.  Graph replay belongs to ``Context``
+(``use_graph=True``), which keeps its buffers and graphs between
+propagations.
 """
 import ctypes
 import os
@@ -54,6 +57,11 @@ def _check_E(E):
 
 def _report(rc, err, strict):
     if rc == 0:
+        # a notice the reference prints to stdout on success (mask formula
+        # that disagrees with the mask, UtilitiesLib/VectorField.cpp:107-111)
+        if err.value:
+            sys.stdout.write(err.value.decode("utf-8", "replace") + "\n")
+            sys.stdout.flush()
         return
     msg = err.value.decode("utf-8", "replace")
     if _strict(strict):
@@ -87,8 +95,11 @@ def run_backend_with_mask(json_str, mask_formula, lc_field_vals, mask_vals, E_fi
 
 
 def _run(json_str, formula, lc, mask, E, precision, wl_batch, use_graph, strict, stats, tuning=None):
+    if use_graph:
+        raise ValueError("one-shot backend calls launch eagerly; replay graphs through "
+                         "Context(..., use_graph=True), which keeps its buffers between propagations")
     lib = _lib.load()
-    opts = _lib.options(precision=precision, wl_batch=wl_batch, use_graph=use_graph, tuning=tuning)
+    opts = _lib.options(precision=precision, wl_batch=wl_batch, use_graph=False, tuning=tuning)
     st = _lib.Stats()
     err = _lib.err_buf()
     Ef = E.reshape(-1).view(np.float64)
@@ -136,6 +147,10 @@ class Context:
                                          self.n_E, ctypes.byref(opts), err, len(err))
         if not self.h:
             raise BpmBackendError(err.value.decode())
+        self.notice = err.value.decode("utf-8", "replace")
+        if self.notice:
+            sys.stdout.write(self.notice + "\n")
+            sys.stdout.flush()
 
     def upload_fields(self, E):
         _check_E(E)

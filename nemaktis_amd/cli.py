@@ -232,8 +232,15 @@ def run(settings_path, precision=64, wl_batch=1, use_graph=False):
     ic["Mesh dimensions"], ic["Mesh spacings"] = dims, spacings
     wl, qs = _wavelengths_wavevectors(s["Physics settings"]["Coefficients"])
     E = np.zeros((len(wl), len(qs), 4, dims[1], dims[0]), np.complex128)
-    bpm_backend.run_backend_with_mask(json.dumps(s), "", lc, mask, E, precision=precision, wl_batch=wl_batch,
-                                      use_graph=use_graph, strict=True)
+    if use_graph:
+        # graph replay lives in a context (one capture, replayed by the march)
+        with bpm_backend.Context(json.dumps(s), lc, E.size, mask_vals=mask, mask_formula="", precision=precision,
+                                 wl_batch=wl_batch, use_graph=True) as ctx:
+            ctx.propagate()
+            ctx.download(E)
+    else:
+        bpm_backend.run_backend_with_mask(json.dumps(s), "", lc, mask, E, precision=precision, wl_batch=wl_batch,
+                                          strict=True)
     scr = s["Postprocessor settings"]["Screen output"]
     if scr.get("Activate", True):
         base = scr.get("Base name", "")

@@ -66,12 +66,13 @@ void fill_stats(nmk_bpm_stats *out, const Stats &s) {
     }
 }
 
-// maps exceptions to status codes and messages
+// maps exceptions to status codes and messages; on success the message
+// buffer receives the notice (if any) the reference would have printed
 template <class F>
-int guarded(char *err, size_t len, F &&f) {
+int guarded(char *err, size_t len, F &&f, const std::string *notice = nullptr) {
     try {
         f();
-        set_err(err, len, "");
+        set_err(err, len, notice ? *notice : std::string());
         return NMK_BPM_OK;
     } catch (const BpmError &e) {
         set_err(err, len, e.what());
@@ -174,17 +175,21 @@ void nmk_bpm_default_options(nmk_bpm_options *o) {
 int nmk_bpm_run(const char *json, const char *formula, const double *lc, int64_t n_lc, const double *mask,
                 int64_t n_mask, double *E, int64_t n_E, const nmk_bpm_options *opts, nmk_bpm_stats *stats, char *err,
                 size_t errlen) {
+    std::string warning;
     return guarded(err, errlen, [&] {
-        std::string warning;
         Problem p = problem_from(json, formula, lc, n_lc, mask, n_mask, n_E, &warning);
         Options o = to_options(opts);
+        // a one-shot run marches once: a captured graph would be replayed
+        // once and only adds the capture, so one-shot runs launch eagerly
+        // (use_graph is honoured by contexts, which replay their graphs)
+        o.use_graph = 0;
         auto r = make_runner(p, o);
         if (p.beam == BeamKind::None) r->upload_fields(reinterpret_cast<const c64 *>(E));
         Stats st;
         r->propagate(&st);
         r->download(reinterpret_cast<c64 *>(E));
         fill_stats(stats, st);
-    });
+    }, &warning);
 }
 
 int nmk_bpm_run_backend_without_mask(const char *json, const double *lc, int64_t n_lc, double *E, int64_t n_E,
@@ -210,13 +215,14 @@ nmk_bpm_ctx *nmk_bpm_create(const char *json, const char *formula, const double 
                             const double *mask, int64_t n_mask, int64_t n_E, const nmk_bpm_options *opts, char *err,
                             size_t errlen) {
     nmk_bpm_ctx *ctx = nullptr;
+    std::string warning;
     int rc = guarded(err, errlen, [&] {
         std::unique_ptr<nmk_bpm_ctx> c(new nmk_bpm_ctx);
-        c->problem = problem_from(json, formula, lc, n_lc, mask, n_mask, n_E, nullptr);
+        c->problem = problem_from(json, formula, lc, n_lc, mask, n_mask, n_E, &warning);
         c->runner = make_runner(c->problem, to_options(opts));
         c->problem.lc = nullptr;  // the director now lives in HBM
         ctx = c.release();
-    });
+    }, &warning);
     return rc == NMK_BPM_OK ? ctx : nullptr;
 }
 
@@ -278,6 +284,38 @@ int nmk_bpm_estimate_footprint(const char *json, int64_t n_lc, int has_mask, con
         }
         if (bytes) *bytes = b;
     });
+}
+
+int nmk_bpm_interface_weights(const char *json, const char *formula, const double *mask, int64_t n_mask,
+                              double *weights, int64_t n_weights, char *err, size_t errlen) {
+    std::string warning;
+    return guarded(err, errlen, [&] {
+        if (!json) throw BpmError("null settings string");
+        if (!formula || !mask) throw BpmError("a mask and a mask formula are required");
+        const Settings s = parse_settings(json);
+        Problem p;
+        p.Nx = s.Nx;
+        p.Ny = s.Ny;
+        p.Nz = s.Nz;
+        p.dx = s.dx;
+        p.dy = s.dy;
+        p.dz = s.dz;
+        if (p.Nz < 2) throw BpmError("The mesh needs at least two planes along z");
+        const int64_t nv = p.voxels();
+        if (n_mask != nv) throw BpmError("Wrong dimension for the mask pointer array");
+        const int64_t nw = 2 * (int64_t)p.Nx * p.Ny * (p.Nz - 1);
+        if (!weights || n_weights != nw) throw BpmError("the weights array needs 2*Nx*Ny*(Nz-1) values");
+        p.mask.resize((size_t)nv);
+        for (int64_t i = 0; i < nv; i++) p.mask[i] = mask[i] >= 0 ? 1 : 0;
+        std::vector<double> w;
+        try {
+            interface_weights(p, formula, w, &warning);
+        } catch (const ExprError &e) {
+            throw BpmError(std::string("Wrong mask formula: ") + e.what());
+        }
+        if (w.empty()) std::fill(weights, weights + nw, 0.5);
+        else std::copy(w.begin(), w.end(), weights);
+    }, &warning);
 }
 
 int nmk_bpm_eval_expression(const char *expr, double lambda, double *value, char *err, size_t errlen) {

@@ -42,10 +42,17 @@ struct DevBuf {
     }
     void alloc(size_t n) {
         release();
-        count = n;
+        count = 0;
         if (!n) return;
         if (source) ptr = static_cast<P *>(source->allocate(n * sizeof(P)));
         else HIPCHK(hipMalloc((void **)&ptr, n * sizeof(P)));
+        count = n;
+    }
+    // exchanges the allocations of two buffers of the same source
+    void swap(DevBuf &o) {
+        std::swap(ptr, o.ptr);
+        std::swap(count, o.count);
+        std::swap(source, o.source);
     }
     size_t bytes() const { return count * sizeof(P); }
     size_t footprint() const { return (bytes() + kAllocAlign - 1) / kAllocAlign * kAllocAlign; }
@@ -192,10 +199,11 @@ public:
     }
 
     ~RunnerT() override {
-        for (auto &kv : graphs_) {
-            (void)hipGraphExecDestroy(kv.second.exec);
-            (void)hipGraphDestroy(kv.second.graph);
-        }
+        // work queued by a propagation that threw (or by a caller's stream)
+        // may still read or write the buffers released below: finish it first
+        (void)hipStreamSynchronize(st_);
+        if (st2_) (void)hipStreamSynchronize(st2_);
+        drop_graphs();
         for (auto &e : evs_) (void)hipEventDestroy(e);
         for (auto &pr : prof_pool_) {
             (void)hipEventDestroy(pr.first);
@@ -605,9 +613,19 @@ private:
         return cur;
     }
 
+    void drop_graphs() {
+        for (auto &kv : graphs_) {
+            (void)hipGraphExecDestroy(kv.second.exec);
+            (void)hipGraphDestroy(kv.second.graph);
+        }
+        graphs_.clear();
+    }
+
     // validates t, then switches to it (the runner keeps its state when t is
-    // refused); the x-line row buffer only grows, so that re-tuning a runner
-    // does not free buffers a captured graph has used
+    // refused).  The captured graphs hold the buffer pointers of the old
+    // shapes, so they are destroyed before any buffer changes; a larger x-line
+    // row buffer is allocated beside the old one and swapped in only once the
+    // allocation has succeeded.
     void set_tuning(const Tuning &t) {
         const LinePlan lx = g_.px ? make_line_plan(g_.Mx, t.x_chunks) : lpx_;
         const LinePlan ly = g_.py ? make_line_plan(g_.My, t.y_chunks) : lpy_;
@@ -617,22 +635,24 @@ private:
         if (g_.py && ((t.y_pair ? 2 : 1) * ly.P * t.y_cols > 512 ||
                       ysolve_lds_bytes<T>(ly, t.y_cols, t.y_pair) > max_block_lds()))
             throw std::runtime_error("y line solver tuning exceeds the block limits");
+        drop_graphs();
         if (g_.px) {
             const size_t need = 2 * (size_t)sz_.nslots_max * 2 * 3 * g_.My * lx.lmax * lx.P;
-            if (need > xt_.count) xt_.alloc(need);
+            if (need > xt_.count) {
+                DevBuf<C2<T>> grown;
+                grown.source = xt_.source;
+                grown.alloc(need);  // throws with xt_ untouched
+                xt_.swap(grown);    // the old rows are released with `grown`
+            }
         }
         tune_ = t;
         lpx_ = lx;
         lpy_ = ly;
-        for (auto &kv : graphs_) {
-            (void)hipGraphExecDestroy(kv.second.exec);
-            (void)hipGraphDestroy(kv.second.graph);
-        }
-        graphs_.clear();
     }
 
     void retune(const Tuning &t) override {
         HIPCHK(hipStreamSynchronize(st_));
+        if (st2_) HIPCHK(hipStreamSynchronize(st2_));
         set_tuning(t.x_chunks > 0 ? t : default_tuning(o_.precision, g_.Mx, g_.My, sz_.nprops_max));
     }
 

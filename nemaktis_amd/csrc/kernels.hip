@@ -2230,9 +2230,10 @@ void launch_fft2(c64 *data, int Nx, int Ny, int planes, int sign, const c64 *tw_
 
 // ---------------------------------------------------------------------------
 // per-k 2x2 transfer filter (PostprocessorLib/ScreenOutput.cpp:226-302) and
-// the 1/(Nx*Ny) normalisation, applied to both polarisations (:206-219)
+// the 1/(Nx*Ny) normalisation, applied to both polarisations (:206-219);
+// any number of isotropic layers (the transfer product is accumulated in
+// registers, one layer at a time)
 // ---------------------------------------------------------------------------
-constexpr int kMaxIsoLayers = 16;
 
 __global__ __launch_bounds__(256) void k_filter(c64 *__restrict__ E, Geo g, const SlotParams *__restrict__ sp,
                                                 const double *__restrict__ qxy, int nq,
@@ -2321,7 +2322,6 @@ __global__ __launch_bounds__(256) void k_filter(c64 *__restrict__ E, Geo g, cons
 void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const double *qxy, int nq,
                    const double *iso_thick, const double *iso_index, int n_iso, double dx, double dy, double z_foc,
                    double NA, hipStream_t st) {
-    if (n_iso > kMaxIsoLayers) throw std::runtime_error("too many isotropic layers (max 16)");
     hipLaunchKernelGGL(k_filter, dim3(blocks_for(g.Nfull, 256), nslots * nq), dim3(256), 0, st, E, g, sp, qxy, nq,
                        iso_thick, iso_index, n_iso, dx, dy, z_foc, NA);
     NMK_LAUNCH_CHECK();

@@ -228,6 +228,15 @@ void Server::worker(int w) {
             status = std::string(e.what()).rfind("HIP error", 0) == 0 ? 2 : 4;
             error = e.what();
         }
+        // a job that failed part-way may have kernels or copies still queued
+        // on its stream that write into its arena range: the range is handed
+        // to the next job only once that stream has drained
+        const hipError_t drained = hipStreamSynchronize(streams_[w]);
+        if (drained != hipSuccess && state == JOB_DONE) {
+            state = JOB_FAILED;
+            status = 2;
+            error = std::string("HIP error: ") + hipGetErrorString(drained) + " at hipStreamSynchronize";
+        }
         {
             std::lock_guard<std::mutex> lk(mu_);
             ranges_->release(job->region);

@@ -1,7 +1,11 @@
 #include "settings.h"
 
 #include <cmath>
+#include <algorithm>
+#include <exception>
+#include <mutex>
 #include <regex>
+#include <thread>
 
 #include "expr.h"
 #include "json.h"
@@ -155,45 +159,70 @@ Settings parse_settings(const std::string &json_text) {
     return s;
 }
 
-namespace {
-
 // UtilitiesLib/VectorField.cpp:64-113: interface interpolation weights along z
-// from a mask formula, by bisection of the formula's zero crossing in each slab
-void mask_formula_weights(const Problem &p, const std::string &formula, std::vector<double> &w, std::string *warning) {
+// from a mask formula, by bisection of the formula's zero crossing in each
+// slab.  The columns are independent; they are split over host threads as the
+// reference splits them over OpenMP threads (the expression tree is read-only).
+void interface_weights(const Problem &p, const std::string &formula, std::vector<double> &w, std::string *warning) {
     const int Nx = p.Nx, Ny = p.Ny, Nz = p.Nz;
     const int64_t N = (int64_t)Nx * Ny;
     w.assign(2 * N * (int64_t)(Nz - 1), 0.5);
-    Expr e(formula, {"x", "y", "z"});
-    bool failed = false;
-    for (int64_t iperp = 0; iperp < N; iperp++) {
-        int ix = int(iperp % Nx), iy = int(iperp / Nx);
-        double v[3] = {(ix - 0.5 * (Nx - 1)) * p.dx, (iy - 0.5 * (Ny - 1)) * p.dy, 0};
-        auto f = [&](double z) {
-            v[2] = z;
-            return e.eval(v);
-        };
-        for (int iz = 0; iz < Nz - 1; iz++) {
-            int64_t i = iperp + N * iz;
-            double zi = (iz - 0.5 * (Nz - 1)) * p.dz;
-            double zm = zi, zp = (iz + 1 - 0.5 * (Nz - 1)) * p.dz, z = zm;
-            double fm = f(zm), fp = f(zp);
-            if (fm * fp <= 0) {
-                bool m0 = p.mask.empty() ? true : p.mask[i] != 0;
-                bool m1 = p.mask.empty() ? true : p.mask[i + N] != 0;
-                if (m0 == m1) failed = true;
-                while (std::fabs(zp - zm) >= 1e-3 * p.dz) {
-                    z = zm;
-                    fm = f(z);
-                    z = 0.5 * (zp + zm);
-                    double fz = f(z);
-                    if (fz == 0) break;
-                    if (fz * fm < 0) zp = z;
-                    else zm = z;
+    const Expr e(formula, {"x", "y", "z"});
+    auto columns = [&](int64_t beg, int64_t end) {
+        bool failed = false;
+        for (int64_t iperp = beg; iperp < end; iperp++) {
+            int ix = int(iperp % Nx), iy = int(iperp / Nx);
+            double v[3] = {(ix - 0.5 * (Nx - 1)) * p.dx, (iy - 0.5 * (Ny - 1)) * p.dy, 0};
+            auto f = [&](double z) {
+                v[2] = z;
+                return e.eval(v);
+            };
+            for (int iz = 0; iz < Nz - 1; iz++) {
+                int64_t i = iperp + N * iz;
+                double zi = (iz - 0.5 * (Nz - 1)) * p.dz;
+                double zm = zi, zp = (iz + 1 - 0.5 * (Nz - 1)) * p.dz, z = zm;
+                double fm = f(zm), fp = f(zp);
+                if (fm * fp <= 0) {
+                    bool m0 = p.mask.empty() ? true : p.mask[i] != 0;
+                    bool m1 = p.mask.empty() ? true : p.mask[i + N] != 0;
+                    if (m0 == m1) failed = true;
+                    while (std::fabs(zp - zm) >= 1e-3 * p.dz) {
+                        z = zm;
+                        fm = f(z);
+                        z = 0.5 * (zp + zm);
+                        double fz = f(z);
+                        if (fz == 0) break;
+                        if (fz * fm < 0) zp = z;
+                        else zm = z;
+                    }
+                    w[2 * i] = (z - zi) / p.dz;
+                    w[2 * i + 1] = 1 - w[2 * i];
                 }
-                w[2 * i] = (z - zi) / p.dz;
-                w[2 * i + 1] = 1 - w[2 * i];
             }
         }
+        return failed;
+    };
+    const int64_t nthreads = std::max<int64_t>(1, std::min<int64_t>(std::thread::hardware_concurrency(), N / 256));
+    bool failed = false;
+    if (nthreads == 1) {
+        failed = columns(0, N);
+    } else {
+        std::vector<std::thread> pool;
+        std::vector<char> flags((size_t)nthreads, 0);
+        std::exception_ptr first_error;
+        std::mutex mu;
+        for (int64_t t = 0; t < nthreads; t++)
+            pool.emplace_back([&, t] {
+                try {
+                    flags[(size_t)t] = columns(N * t / nthreads, N * (t + 1) / nthreads) ? 1 : 0;
+                } catch (...) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (!first_error) first_error = std::current_exception();
+                }
+            });
+        for (auto &th : pool) th.join();
+        if (first_error) std::rethrow_exception(first_error);
+        for (char c : flags) failed = failed || c;
     }
     if (failed) {
         // the reference prints this and falls back to the default weights
@@ -202,8 +231,6 @@ void mask_formula_weights(const Problem &p, const std::string &formula, std::vec
         w.clear();
     }
 }
-
-}  // namespace
 
 Problem build_problem(const Settings &s, const double *lc, int64_t n_lc, const double *mask, int64_t n_mask,
                       const char *mask_formula, int64_t n_E, std::string *warning) {
@@ -231,7 +258,7 @@ Problem build_problem(const Settings &s, const double *lc, int64_t n_lc, const d
     p.lc = lc;
     if (mask && mask_formula && mask_formula[0] != '\0') {
         try {
-            mask_formula_weights(p, mask_formula, p.weights, warning);
+            interface_weights(p, mask_formula, p.weights, warning);
         } catch (const ExprError &e) {
             throw BpmError(std::string("Wrong mask formula: ") + e.what());
         }

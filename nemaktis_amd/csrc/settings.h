@@ -76,4 +76,11 @@ struct Problem {
 Problem build_problem(const Settings &s, const double *lc, int64_t n_lc, const double *mask, int64_t n_mask,
                       const char *mask_formula, int64_t n_E, std::string *warning);
 
+// Interface interpolation weights of a mask formula (UtilitiesLib/
+// VectorField.cpp:64-113) for the mesh and mask of p: w[2*i], w[2*i+1] for
+// voxel i of planes 0 .. Nz-2.  When the formula's zero crossings disagree
+// with the mask, w is left empty (every weight 0.5) and *warning receives the
+// reference's notice.
+void interface_weights(const Problem &p, const std::string &formula, std::vector<double> &w, std::string *warning);
+
 }  // namespace nmk

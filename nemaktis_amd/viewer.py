@@ -197,7 +197,7 @@ class OpticalFields:
 
     # ---- reference interface -------------------------------------------------------------------------------
     def copy(self):
-        """Returns a hard copy of this OpticalFields object"""
+        """Independent object with the same geometry and its own copy of the field values."""
         new_fields = OpticalFields(
             wavelengths=self._wavelengths, max_NA_objective=self._max_NA_objective,
             max_NA_condenser=self._max_NA_condenser, N_radial_wavevectors=self._N_radial_wavevectors,
@@ -220,7 +220,7 @@ class OpticalFields:
 
     @property
     def focused_vals(self):
-        """Fields after focalisation by the microscope objective, shape (N_wavelengths,N_wavevectors,4,Ny,Nx)."""
+        """Output of the last focus_fields(): complex128 array of the same shape as ``vals``."""
         if self._focused is None:
             out = np.empty(self._shape(), dtype=np.complex128)
             err = _lib.err_buf()
@@ -234,8 +234,8 @@ class OpticalFields:
 
     @property
     def vals(self):
-        """Fields of shape (N_wavelengths,N_wavevectors,4,Ny,Nx); components Ex, Ey for an input polarised
-        along x, then Ex, Ey for an input polarised along y."""
+        """Transverse fields, complex128 (N_wavelengths, N_wavevectors, 4, Ny, Nx): the last axis-2 index
+        runs over (Ex, Ey) of the x-polarised input, then (Ex, Ey) of the y-polarised input."""
         if self._vals is None:
             self._vals = np.empty(self._shape(), dtype=np.complex128)
             self._ctx.download(self._vals)
@@ -261,8 +261,8 @@ class OpticalFields:
         self._NA = max(0, min(self._max_NA_objective, new_NA))
 
     def focus_fields(self, z_focus=None):
-        """Propagate the optical fields through the objective lens to the screen conjugate to the focusing
-        plane (whose altitude inside the sample is set with the parameter z_focus)."""
+        """Refocuses the fields: the objective filter for focal height z_focus (kept for later calls when
+        omitted) and the current NA, applied to every (wavelength, wavevector) plane on the device."""
         if z_focus is not None:
             self._z = z_focus
         h = self._handle()
@@ -313,7 +313,9 @@ class OpticalFields:
 
 
 class FieldViewer:
-    """Recombines optical fields into micrographs as in a real microscope (reference: field_viewer.py:16-470).
+    """Turns the transverse fields into polarised-light micrographs on the device: polariser, waveplates and
+    analyser applied per pixel, incoherent sum over the condenser wavevectors, colour from the spectrum
+    (reference: field_viewer.py:16-470).
 
     ``FieldViewer(optical_fields, cmf=None, gamma=True)``; ``cmf`` is the (N_wavelengths, 3) colour matching
     table of the wavelengths (the ``cmf`` argument of the reference)."""
@@ -353,7 +355,7 @@ class FieldViewer:
 
     @property
     def z_focus(self):
-        """Current vertical position of the focal plane"""
+        """Height z of the plane the objective is focused on."""
         return self._optical_fields.z_focus
 
     @z_focus.setter
@@ -362,7 +364,7 @@ class FieldViewer:
 
     @property
     def NA_objective(self):
-        """Numerical aperture of the microscope's objective"""
+        """Objective NA used for the next render (clipped to the fields' maximum objective NA)."""
         return self._NA_objective
 
     @NA_objective.setter
@@ -401,7 +403,7 @@ class FieldViewer:
                                        0 if cmf is None else cmf.size, err, len(err)), err)
 
     def update_image(self):
-        """Recompute the micrograph from the optical fields data"""
+        """Refocuses the fields if needed, then recomputes the spectrum and the image on the device."""
         self.render()
         of = self._optical_fields
         err = _lib.err_buf()
@@ -412,11 +414,11 @@ class FieldViewer:
                                               len(err)), err)
 
     def get_image(self):
-        """Returns the current micrograph as a numpy array of shape (Ny,Nx,3)."""
+        """Last rendered image: float64 array (Ny, Nx, 3); needs a colour table."""
         if self._cmf is None:
             raise ViewerError("a colour matching table (cmf) is required for images")
         return np.flip(np.tile(self._image, (self.n_tiles_y, self.n_tiles_x, 1)), axis=0)
 
     def get_spectrum(self):
-        """Returns the q-averaged spectrum as a numpy array of shape (Ny,Nx,Nl)."""
+        """Last rendered spectrum, summed over the condenser wavevectors: float64 array (Ny, Nx, N_wavelengths)."""
         return np.flip(np.tile(self._q_averaged_specter, (self.n_tiles_y, self.n_tiles_x, 1)), axis=0)

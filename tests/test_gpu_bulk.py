@@ -64,7 +64,9 @@ def test_bulk_output_fp64_matches_oracle(tmp_path):
 def test_bulk_output_fp32_batched_graph_matches_oracle(tmp_path):
     s, lc = _case(dims=(24, 22, 5), seed=5, folder=str(tmp_path), name="b32")
     print("[gpu case] bulk fp32 graph", flush=True)
-    bpm_backend.propagate(json.dumps(s), lc, precision=32, wl_batch=0, use_graph=True)
+    with bpm_backend.Context(json.dumps(s), lc, int(np.prod(fields.output_shape(s))), precision=32, wl_batch=0,
+                             use_graph=True) as ctx:
+        ctx.propagate()
     _, bulk_want = oracle.propagate_bulk(s, lc)
     _compare(str(tmp_path / "b32.vti"), bulk_want, 1e-4)
 

@@ -228,7 +228,11 @@ def test_builtin_shapes_mid_mesh_batched():
               nhost="1.5", na_objective=0.4)
     lc = fields.random_smooth(300, 260, 4, (0.1, 0.1, 0.1), seed=31, corr_length=0.3)
     want = oracle.propagate(s, lc)
-    err32 = _rel(_gpu(s, lc, precision=32, wl_batch=0, use_graph=True), want)
+    with bpm_backend.Context(json.dumps(s), lc, int(np.prod(fields.output_shape(s))), precision=32, wl_batch=0,
+                             use_graph=True) as ctx:
+        ctx.propagate()
+        got32 = ctx.download(np.empty(fields.output_shape(s), np.complex128))
+    err32 = _rel(got32, want)
     assert err32 < TOL32, err32
     err64 = _rel(_gpu(s, lc, precision=64, wl_batch=0), want)
     assert err64 < TOL64, err64
@@ -283,3 +287,61 @@ def test_xcd_work_orders_are_bitwise_equal():
         assert np.array_equal(o, outs[0])
     want = oracle.propagate(s, lc)
     assert _rel(outs[0], want) < TOL32
+
+
+def test_drop_in_c_symbols_called_directly():
+    """nmk_bpm_run_backend_without_mask / _with_mask through ctypes, as a C
+    caller binds them (PythonBackend/bpm_backend.h:4-13), against the oracle."""
+    import ctypes
+    from nemaktis_amd import _lib
+    lib = _lib.load()
+    s, lc = _random_case((20, 18, 6), seed=41)
+    js = json.dumps(s).encode()
+    lcf = np.ascontiguousarray(lc.reshape(-1))
+    E = np.zeros(fields.output_shape(s), np.complex128)
+    err = _lib.err_buf()
+    print("[gpu case] nmk_bpm_run_backend_without_mask", flush=True)
+    rc = lib.nmk_bpm_run_backend_without_mask(js, _lib.dptr(lcf), lcf.size, _lib.dptr(E.reshape(-1).view(np.float64)),
+                                              E.size, err, len(err))
+    assert rc == 0 and err.value == b"", err.value
+    assert _rel(E, oracle.propagate(s, lc)) < TOL64
+    dims, sp = (20, 20, 9), (0.25, 0.25, 0.25)
+    R = 1.9
+    lcd, mask, formula = fields.droplet(*dims, sp, R)
+    sd = _base(dims, sp, no="1.5", ne="1.7", nhost="1.51", na_objective=0.5)
+    w = _formula_weights(sd, lambda x, y, z: R * R - x * x - y * y - z * z, mask)
+    E2 = np.zeros(fields.output_shape(sd), np.complex128)
+    m = np.ascontiguousarray(mask.reshape(-1))
+    lcdf = np.ascontiguousarray(lcd.reshape(-1))
+    print("[gpu case] nmk_bpm_run_backend_with_mask", flush=True)
+    rc = lib.nmk_bpm_run_backend_with_mask(json.dumps(sd).encode(), formula.encode(), _lib.dptr(lcdf), lcdf.size,
+                                           _lib.dptr(m), m.size, _lib.dptr(E2.reshape(-1).view(np.float64)), E2.size,
+                                           err, len(err))
+    assert rc == 0 and err.value == b"", err.value
+    assert _rel(E2, oracle.propagate(sd, lcd, mask=mask, weights=w)) < TOL64
+
+
+def test_mask_formula_mismatch_notice_and_fallback(capsys):
+    """A mask formula that disagrees with the mask: the reference's notice on
+    stdout and the default weights (UtilitiesLib/VectorField.cpp:107-111)."""
+    dims, sp = (16, 14, 6), (0.25, 0.25, 0.25)
+    lc, mask, _ = fields.droplet(*dims, sp, 1.2)
+    s = _base(dims, sp, no="1.5", ne="1.7", nhost="1.51", na_objective=0.5)
+    E = np.zeros(fields.output_shape(s), np.complex128)
+    bpm_backend.run_backend_with_mask(json.dumps(s), "0.1-z", lc, mask, E, strict=True)
+    out = capsys.readouterr().out
+    assert "Mask values do not corresponds to the given formula, will ignore interpolation correction" in out
+    assert _rel(E, oracle.propagate(s, lc, mask=mask)) < TOL64
+
+
+def test_many_iso_layers_fp64():
+    """More than 16 isotropic layers between the sample and the objective
+    (PostprocessorLib/ScreenOutput.cpp:240-299 loops over any number)."""
+    s, lc = _random_case((16, 14, 4), seed=43)
+    n = 20
+    s["Postprocessor settings"]["Screen output"].update({
+        "Isotropic layer thicknesses": [0.05 + 0.01 * k for k in range(n)],
+        "Isotropic layer refractive indices": [1.3 + 0.01 * k for k in range(n)],
+        "Focalisation z-shift": 0.2})
+    err = _rel(_gpu(s, lc), oracle.propagate(s, lc))
+    assert err < TOL64, err

@@ -184,3 +184,86 @@ def test_footprint_estimate_scales():
 def test_oracle_expression_evaluator_agrees_with_backend(expr, lam):
     import oracle
     assert oracle.eval_expr(expr, **{"lambda": lam}) == pytest.approx(_eval(expr, lam), rel=1e-15)
+
+
+# --------------------------------------------------------------------------- mask-formula weights (host side)
+def _host_weights(s, formula, mask):
+    Nx, Ny, Nz = s["Physics settings"]["Initial conditions"]["Mesh dimensions"]
+    w = np.empty(2 * Nx * Ny * (Nz - 1))
+    err = _lib.err_buf()
+    m = np.ascontiguousarray(mask, dtype=np.float64).reshape(-1)
+    rc = _lib.load().nmk_bpm_interface_weights(json.dumps(s).encode(), formula.encode(), _lib.dptr(m), m.size,
+                                               _lib.dptr(w), w.size, err, len(err))
+    if rc:
+        raise ValueError(err.value.decode())
+    return w.reshape(Nz - 1, Ny, Nx, 2), err.value.decode()
+
+
+def test_backend_interface_weights_closed_form_and_oracle():
+    """The weights the backend computes from a mask formula (host code,
+    UtilitiesLib/VectorField.cpp:64-113) against the closed-form crossings of a
+    droplet R^2 - r^2 (within the 1e-3 dz bisection bracket) and against the
+    oracle's independent numpy bisection (same steps, so equal to rounding)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mask_weights
+    dims, sp, R = (13, 12, 10), (0.3, 0.3, 0.3), 1.5
+    s = fields.make_settings(dims, sp, [0.6])
+    x, y, z = fields.mesh_coords(*dims, *sp)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    mask = R * R - X * X - Y * Y - Z * Z
+    w, notice = _host_weights(s, "%r^2-x^2-y^2-z^2" % R, mask)
+    assert notice == ""
+    w_or, bad = mask_weights.interface_weights(lambda a, b, c: R ** 2 - a ** 2 - b ** 2 - c ** 2, dims, sp, mask)
+    assert not bad
+    np.testing.assert_allclose(w, w_or, rtol=0, atol=1e-12)
+    rho2 = R * R - x[None, :] ** 2 - y[:, None] ** 2
+    zi = (np.arange(dims[2] - 1) - 0.5 * (dims[2] - 1)) * sp[2]
+    for iz in range(dims[2] - 1):
+        for sgn in (1.0, -1.0):
+            with np.errstate(invalid="ignore"):
+                root = sgn * np.sqrt(rho2)
+            inside = (rho2 > 0) & (root > zi[iz]) & (root < zi[iz] + sp[2])
+            assert np.all(np.abs(w[iz, ..., 0][inside] - (root[inside] - zi[iz]) / sp[2]) < 1e-3)
+    assert np.sum(w[..., 0] != 0.5) > 20
+
+
+def test_backend_interface_weights_mismatch_notice():
+    """A formula that disagrees with the mask: all weights 0.5 and the
+    reference's notice (UtilitiesLib/VectorField.cpp:107-111) with status 0."""
+    dims, sp = (6, 5, 6), (0.2, 0.2, 0.2)
+    s = fields.make_settings(dims, sp, [0.6])
+    w, notice = _host_weights(s, "0.05-z", np.ones(dims[::-1]))
+    assert np.all(w == 0.5)
+    assert notice == "Mask values do not corresponds to the given formula, will ignore interpolation correction"
+    with pytest.raises(ValueError, match="Wrong mask formula"):
+        _host_weights(s, "0.05-", np.ones(dims[::-1]))
+
+
+# --------------------------------------------------------------------------- the two drop-in C symbols
+def test_drop_in_symbols_validate_before_any_device_work():
+    """nmk_bpm_run_backend_without_mask / _with_mask called directly through
+    ctypes (the binding INTEGRATION.md gives): input errors come back as
+    NMK_BPM_EINPUT with the reference's message, before any device call."""
+    lib = _lib.load()
+    s, lc, E = _small()
+    js = json.dumps(s).encode()
+    lcf = np.ascontiguousarray(lc.reshape(-1))
+    Ef = E.reshape(-1).view(np.float64)
+    err = _lib.err_buf()
+    rc = lib.nmk_bpm_run_backend_without_mask(js, _lib.dptr(lcf), lcf.size - 1, _lib.dptr(Ef), E.size, err, len(err))
+    assert rc == 1 and b"Wrong dimension for the user pointer array" in err.value
+    rc = lib.nmk_bpm_run_backend_without_mask(js, _lib.dptr(lcf), lcf.size, _lib.dptr(Ef), E.size - 1, err, len(err))
+    assert rc == 1 and b"OpticalFieldCollection" in err.value
+    m = np.ones(8 * 6 * 3)
+    rc = lib.nmk_bpm_run_backend_with_mask(js, b"", _lib.dptr(lcf), lcf.size, _lib.dptr(m), m.size - 2,
+                                           _lib.dptr(Ef), E.size, err, len(err))
+    assert rc == 1 and b"Wrong dimension for the mask pointer array" in err.value
+    rc = lib.nmk_bpm_run_backend_with_mask(js, b"", _lib.dptr(lcf), lcf.size, None, 0, _lib.dptr(Ef), E.size,
+                                           err, len(err))
+    assert rc == 1 and b"mask" in err.value
+
+
+def test_one_shot_calls_refuse_graph_replay():
+    s, lc, E = _small()
+    with pytest.raises(ValueError, match="Context"):
+        bpm_backend.run_backend_without_mask(json.dumps(s), lc, E, use_graph=True)

@@ -419,3 +419,98 @@ def test_filter_index_matched_layer_is_angular_spectrum_propagator():
     H = np.where(np.sqrt(qq) < 0.8, np.exp(1j * (np.sqrt(n * n - qq) - n) * k0 * zf), 0)
     want = np.fft.ifft2(np.fft.fft2(E[0]) * H)
     np.testing.assert_allclose(got[0], want, atol=1e-11)
+
+
+# --------------------------------------------------------------------------- Q-tensor input
+def test_qtensor_of_a_director_reproduces_director_mode():
+    """PermittivityTensorField.cpp:41-65 (Q-tensor branch) against :18-40 (director
+    branch): the caller's Q of a uniaxial director of order 1 is (3 n n^T - I)/2
+    (HighLevelPythonInterface/nemaktis/lc_material.py:667-681), for which
+    (no^2 + de/3) I + (2 de/3) Q = no^2 I + de n n^T exactly.  The two input modes
+    must then give the same permittivity and the same output fields, so a wrong
+    isotropic part or anisotropy factor of the Q branch fails here."""
+    dims = (14, 12, 5)
+    n = fields.random_smooth(*dims, (0.1, 0.1, 0.1), seed=21, corr_length=0.3)
+    Q = 0.5 * (3 * np.einsum("...i,...j->...ij", n, n) - np.eye(3))
+    q6 = np.stack([Q[..., 0, 0], Q[..., 1, 1], Q[..., 2, 2], Q[..., 0, 1], Q[..., 0, 2], Q[..., 1, 2]], -1)
+    kw = dict(wl=(0.55, 0.63), q=((0.0, 0.0), (0.1, -0.05)), no="1.5", ne="1.7", nhost="1.4", na_objective=0.6)
+    s_dir = _settings(dims, **kw)
+    s_q = _settings(dims, field_type="Q-tensor", **kw)
+    mask = np.where(np.arange(dims[0])[None, None, :] < 10, 1.0, -1.0) * np.ones((dims[2], dims[1], 1))
+    a = oracle.Session(s_dir, n, mask=mask)
+    b = oracle.Session(s_q, q6, mask=mask)
+    for w in range(2):
+        a.set_wavelength(w)
+        b.set_wavelength(w)
+        assert abs(a.nref - b.nref) < 1e-14
+        for iz in range(dims[2]):
+            np.testing.assert_allclose(b.eps_plane(iz), a.eps_plane(iz), rtol=0, atol=2e-14)
+    e_dir = oracle.propagate(s_dir, n, mask=mask)
+    e_q = oracle.propagate(s_q, q6, mask=mask)
+    assert np.linalg.norm(e_q - e_dir) / np.linalg.norm(e_dir) < 1e-13
+    # a Q of lower order (S = 0.6) is a different medium: the check is not vacuous
+    e_s = oracle.propagate(s_q, 0.6 * q6, mask=mask)
+    assert np.linalg.norm(e_s - e_dir) / np.linalg.norm(e_dir) > 1e-3
+
+
+# --------------------------------------------------------------------------- mask-formula weights
+def _crossing_weights(z_root, dims, sp):
+    """Closed-form weights: the crossing z* of each slab it falls in, (z* - z_iz)/dz."""
+    Nx, Ny, Nz = dims
+    dz = sp[2]
+    zi = (np.arange(Nz - 1) - 0.5 * (Nz - 1)) * dz
+    w = np.full((Nz - 1, Ny, Nx, 2), 0.5)
+    for iz in range(Nz - 1):
+        inside = (z_root >= zi[iz]) & (z_root <= zi[iz] + dz)
+        w0 = (z_root - zi[iz]) / dz
+        w[iz, ..., 0] = np.where(inside, w0, 0.5)
+        w[iz, ..., 1] = np.where(inside, 1 - w0, 0.5)
+    return w
+
+
+def test_oracle_mask_weights_planar_interface_closed_form():
+    """A planar interface z = z0 inside slab k: every column's weights of slab k
+    are (z0 - z_k)/dz up to the bisection bracket (1e-3 dz), all others 0.5."""
+    import mask_weights
+    dims, sp = (5, 4, 7), (0.2, 0.2, 0.25)
+    z = (np.arange(dims[2]) - 0.5 * (dims[2] - 1)) * sp[2]
+    for z0 in (0.11, -0.37, 0.6):
+        mask = (z0 - z)[:, None, None] * np.ones((1, dims[1], dims[0]))
+        w, bad = mask_weights.interface_weights(lambda x, y, zz: z0 - zz + 0 * x, dims, sp, mask)
+        assert not bad
+        want = _crossing_weights(np.full((dims[1], dims[0]), z0), dims, sp)
+        np.testing.assert_allclose(w, want, rtol=0, atol=1e-3)
+        k = int(np.floor((z0 - z[0]) / sp[2]))
+        assert np.all(np.abs(w[k, ..., 0] - (z0 - z[k]) / sp[2]) < 1e-3)
+        assert np.all(np.delete(w, k, axis=0) == 0.5)
+
+
+def test_oracle_mask_weights_spherical_interface_closed_form():
+    """A droplet R^2 - r^2: crossings at z* = +-sqrt(R^2 - x^2 - y^2)."""
+    import mask_weights
+    dims, sp, R = (11, 10, 12), (0.3, 0.3, 0.3), 1.4
+    x, y, z = fields.mesh_coords(*dims, *sp)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    mask = R * R - X * X - Y * Y - Z * Z
+    w, bad = mask_weights.interface_weights(lambda a, b, c: R * R - a * a - b * b - c * c, dims, sp, mask)
+    assert not bad
+    rho2 = R * R - x[None, :] ** 2 - y[:, None] ** 2
+    with np.errstate(invalid="ignore"):
+        root = np.sqrt(rho2)
+    want = np.full_like(w, 0.5)
+    for sgn in (1.0, -1.0):
+        r = np.where(rho2 > 0, sgn * root, np.nan)
+        cw = _crossing_weights(r, dims, sp)
+        hit = cw[..., 0] != 0.5
+        want[hit] = cw[hit]
+    np.testing.assert_allclose(w, want, rtol=0, atol=1e-3)
+    assert np.sum(w[..., 0] != 0.5) > 20
+
+
+def test_oracle_mask_weights_mismatch_falls_back():
+    """A formula whose crossings disagree with the mask: every weight 0.5
+    (UtilitiesLib/VectorField.cpp:107-111)."""
+    import mask_weights
+    dims, sp = (6, 5, 6), (0.2, 0.2, 0.2)
+    w, bad = mask_weights.interface_weights(lambda x, y, z: 0.05 - z + 0 * x, dims, sp, np.ones(dims[::-1]))
+    assert bad and np.all(w == 0.5)

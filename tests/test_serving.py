@@ -150,3 +150,27 @@ def test_short_poisson_trace():
     assert res["jobs"] == 6
     assert res["p99_ms"] >= res["p50_ms"] > 0
     assert res["updates_per_s"] > 0
+
+
+def test_job_failing_after_its_march_leaves_its_neighbour_intact(tmp_path):
+    """A job whose bulk output cannot be written fails after its kernels ran
+    (its arena range is handed back only once its stream has drained); the job
+    beside it equals its standalone run bit for bit."""
+    good = _job((96, 96, 12), seed=12)
+    bad_s = json.loads(_job((64, 64, 10), seed=13)[0])
+    blocker = tmp_path / "not_a_folder"
+    blocker.write_text("a file where the results folder should be")
+    bad_s["Algorithm settings"]["General"]["Results folder name"] = str(blocker)
+    bad_s["Postprocessor settings"]["Bulk output"] = {"Activate": True, "Base name": "x"}
+    bad_lc = fields.random_smooth(64, 64, 10, (0.1, 0.1, 0.1), seed=13, corr_length=0.5)
+    with serving.Server(max_active=2) as srv:
+        print("[serving] failing bulk job beside a good job", flush=True)
+        jb = srv.submit(json.dumps(bad_s), bad_lc, precision=32)
+        jg = srv.submit(good[0], good[1], precision=32)
+        ib, ig = jb.wait(), jg.wait()
+        jg2 = srv.submit(good[0], good[1], precision=32)
+        ig2 = jg2.wait()
+    assert ib["state"] == "failed", ib
+    assert ig["state"] == "done" and ig2["state"] == "done"
+    ref = bpm_backend.propagate(good[0], good[1], precision=32, wl_batch=0)
+    assert np.array_equal(jg.E, ref) and np.array_equal(jg2.E, ref)
