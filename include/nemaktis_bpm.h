@@ -149,8 +149,13 @@ const char *nmk_bpm_version(void);
  * range of the job's footprint are free; a job that does not fit may be
  * overtaken by later jobs that do, at most max_bypass times.  Admitted jobs
  * run concurrently, each on its own stream with its state resident in its
- * arena range.  E_field_vals is read at admission ("Beam profile": "None")
- * and written when the job ends; it must stay valid until then. */
+ * arena range.  Submission only validates (settings, sizes, the mask
+ * formula's syntax, footprint): the host work of size O(volume) -- the
+ * per-voxel mask, the interface weights of a mask formula and the director
+ * upload -- runs on the job's worker once it is admitted, so lc_field_vals,
+ * mask_vals and E_field_vals are borrowed until the job has finished.
+ * E_field_vals is read at admission ("Beam profile": "None") and written
+ * when the job ends. */
 typedef struct nmk_srv nmk_srv;
 
 typedef struct {
@@ -175,6 +180,9 @@ typedef struct {
     double updates;      /* voxel-slice updates of the job */
     int64_t footprint_bytes;
     char error[256];
+    double t_ready_ms;   /* -1 until the job's host preparation (mask, interface
+                            weights) is done and its inputs are on the device */
+    char notice[128];    /* what the reference prints on success, if anything */
 } nmk_srv_job_info;
 
 typedef struct {

@@ -49,7 +49,8 @@ class SrvConfig(ctypes.Structure):
 class SrvJobInfo(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("t_submit_ms", ctypes.c_double),
                 ("t_admit_ms", ctypes.c_double), ("t_end_ms", ctypes.c_double), ("device_ms", ctypes.c_double),
-                ("updates", ctypes.c_double), ("footprint_bytes", ctypes.c_int64), ("error", ctypes.c_char * 256)]
+                ("updates", ctypes.c_double), ("footprint_bytes", ctypes.c_int64), ("error", ctypes.c_char * 256),
+                ("t_ready_ms", ctypes.c_double), ("notice", ctypes.c_char * 128)]
 
     STATES = ["queued", "running", "done", "failed", "cancelled"]
 
@@ -57,7 +58,8 @@ class SrvJobInfo(ctypes.Structure):
         return {"state": self.STATES[self.state], "status": self.status, "t_submit_ms": self.t_submit_ms,
                 "t_admit_ms": self.t_admit_ms, "t_end_ms": self.t_end_ms, "device_ms": self.device_ms,
                 "updates": self.updates, "footprint_bytes": self.footprint_bytes,
-                "error": self.error.decode("utf-8", "replace")}
+                "error": self.error.decode("utf-8", "replace"), "t_ready_ms": self.t_ready_ms,
+                "notice": self.notice.decode("utf-8", "replace")}
 
 
 class SrvCounters(ctypes.Structure):

@@ -387,6 +387,8 @@ int nmk_srv_wait(nmk_srv *srv, int64_t id, double timeout_ms, nmk_srv_job_info *
         info->updates = ji.updates;
         info->footprint_bytes = ji.footprint;
         set_err(info->error, sizeof(info->error), ji.error);
+        info->t_ready_ms = ji.t_ready_ms;
+        set_err(info->notice, sizeof(info->notice), ji.notice);
     }
     return r;
 }

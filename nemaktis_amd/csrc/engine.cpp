@@ -144,26 +144,51 @@ public:
         plan_x_ = make_fft_plan(p.Nx);
         plan_y_ = make_fft_plan(p.Ny);
 
-        // inputs resident in HBM
-        lc_.alloc((size_t)p.voxels() * p.field_dim);
-        HIPCHK(hipMemcpyAsync(lc_.ptr, p.lc, lc_.bytes(), hipMemcpyHostToDevice, st_));
-        if (!p.mask.empty()) {
-            mask_.alloc(p.mask.size());
-            HIPCHK(hipMemcpyAsync(mask_.ptr, p.mask.data(), mask_.bytes(), hipMemcpyHostToDevice, st_));
-        }
-        if (!p.weights.empty()) {
-            weights_.alloc(p.weights.size());
-            HIPCHK(hipMemcpyAsync(weights_.ptr, p.weights.data(), weights_.bytes(), hipMemcpyHostToDevice, st_));
-        }
-        std::vector<double> qxy;
-        for (int i = 0; i < sz_.nq; i++) {
-            qxy.push_back(p.qx[i]);
-            qxy.push_back(p.qy[i]);
-        }
-        q_.alloc(qxy.size());
-        HIPCHK(hipMemcpyAsync(q_.ptr, qxy.data(), q_.bytes(), hipMemcpyHostToDevice, st_));
+        // every buffer is allocated before anything is uploaded, so that a
+        // run that does not fit its allocator fails before any device work
+        const bool dev_in = o.dev.lc != nullptr;  // inputs and output already on the device
+        if (dev_in && p.bulk) throw BpmError("bulk output needs host arrays (its planes are written to a file)");
+        if (!dev_in) lc_.alloc((size_t)p.voxels() * p.field_dim);
+        if (dev_in ? o.dev.mask != nullptr : !p.mask.empty()) mask_.alloc((size_t)p.voxels());
+        if (!dev_in && !p.weights.empty()) weights_.alloc(p.weights.size());
+        q_.alloc(2 * (size_t)sz_.nq);
         iso_t_.alloc(std::max<size_t>(1, p.iso_thick.size()));
         iso_n_.alloc(std::max<size_t>(1, p.iso_index.size()));
+        twx_.alloc(p.Nx);
+        twy_.alloc(p.Ny);
+        sp_.alloc(sz_.nslots_max);
+        pp_.alloc(sz_.nprops_max);
+        partials_.alloc((size_t)kNrefBlocks * sz_.nslots_max);
+        cf_.alloc(2 * (size_t)sz_.nslots_max * cf_slot_size(g_.Mx, g_.My));
+        set_tuning(o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, g_.Mx, g_.My, sz_.nprops_max));
+        for (auto &b : fb_) b.alloc((size_t)sz_.nprops_max * 2 * sz_.NP);
+        if (!dev_in) E_.alloc((size_t)sz_.nwl * sz_.nq * 4 * sz_.Nfull);
+        if (p.bulk) {
+            // every plane of every propagation stays on the device until the
+            // run writes its file (the reference holds them in host memory)
+            bulk_.alloc((size_t)sz_.nwl * sz_.nq * 2 * p.Nz * 2 * sz_.Nfull);
+            bt_.alloc((size_t)sz_.nprops_max * 2 * sz_.NP);
+        }
+        lc_p_ = dev_in ? o.dev.lc : lc_.ptr;
+        mask_p_ = mask_.ptr;
+        w_p_ = dev_in ? o.dev.weights : weights_.ptr;
+        E_p_ = dev_in ? o.dev.E : E_.ptr;
+        if (!o.profile && !o.no_side_stream) HIPCHK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
+
+        // inputs resident in HBM
+        if (!dev_in) HIPCHK(hipMemcpyAsync(lc_.ptr, p.lc, lc_.bytes(), hipMemcpyHostToDevice, st_));
+        if (mask_.ptr) {
+            if (dev_in) launch_mask_flags(o.dev.mask, mask_.ptr, p.voxels(), st_);
+            else HIPCHK(hipMemcpyAsync(mask_.ptr, p.mask.data(), mask_.bytes(), hipMemcpyHostToDevice, st_));
+        }
+        if (weights_.ptr)
+            HIPCHK(hipMemcpyAsync(weights_.ptr, p.weights.data(), weights_.bytes(), hipMemcpyHostToDevice, st_));
+        hq_.clear();
+        for (int i = 0; i < sz_.nq; i++) {
+            hq_.push_back(p.qx[i]);
+            hq_.push_back(p.qy[i]);
+        }
+        HIPCHK(hipMemcpyAsync(q_.ptr, hq_.data(), q_.bytes(), hipMemcpyHostToDevice, st_));
         if (!p.iso_thick.empty()) {
             HIPCHK(hipMemcpyAsync(iso_t_.ptr, p.iso_thick.data(), p.iso_thick.size() * 8, hipMemcpyHostToDevice, st_));
             HIPCHK(hipMemcpyAsync(iso_n_.ptr, p.iso_index.data(), p.iso_index.size() * 8, hipMemcpyHostToDevice, st_));
@@ -175,26 +200,10 @@ public:
                 long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
                 h[k] = mk((double)std::cos(a), (double)std::sin(a));
             }
-            d.alloc(n);
             HIPCHK(hipMemcpyAsync(d.ptr, h.data(), d.bytes(), hipMemcpyHostToDevice, st_));
         };
         twiddles(p.Nx, twx_, htwx_);
         twiddles(p.Ny, twy_, htwy_);
-
-        sp_.alloc(sz_.nslots_max);
-        pp_.alloc(sz_.nprops_max);
-        partials_.alloc((size_t)kNrefBlocks * sz_.nslots_max);
-        cf_.alloc(2 * (size_t)sz_.nslots_max * cf_slot_size(g_.Mx, g_.My));
-        if (!o.profile) HIPCHK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
-        set_tuning(o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, g_.Mx, g_.My, sz_.nprops_max));
-        for (auto &b : fb_) b.alloc((size_t)sz_.nprops_max * 2 * sz_.NP);
-        E_.alloc((size_t)sz_.nwl * sz_.nq * 4 * sz_.Nfull);
-        if (p.bulk) {
-            // every plane of every propagation stays on the device until the
-            // run writes its file (the reference holds them in host memory)
-            bulk_.alloc((size_t)sz_.nwl * sz_.nq * 2 * p.Nz * 2 * sz_.Nfull);
-            bt_.alloc((size_t)sz_.nprops_max * 2 * sz_.NP);
-        }
         HIPCHK(hipStreamSynchronize(st_));
     }
 
@@ -215,7 +224,7 @@ public:
     }
 
     hipStream_t stream() const override { return st_; }
-    const c64 *device_output() const override { return E_.ptr; }
+    const c64 *device_output() const override { return E_p_; }
 
     int64_t footprint() const override {
         size_t b = lc_.footprint() + weights_.footprint() + q_.footprint() + iso_t_.footprint() + iso_n_.footprint() +
@@ -227,49 +236,56 @@ public:
     }
 
     void upload_fields(const c64 *E_host) override {
-        HIPCHK(hipMemcpyAsync(E_.ptr, E_host, E_.bytes(), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemcpyAsync(E_p_, E_host, out_bytes(), hipMemcpyHostToDevice, st_));
         HIPCHK(hipStreamSynchronize(st_));
     }
 
     void download(c64 *E_host) override {
-        HIPCHK(hipMemcpyAsync(E_host, E_.ptr, E_.bytes(), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipMemcpyAsync(E_host, E_p_, out_bytes(), hipMemcpyDeviceToHost, st_));
         HIPCHK(hipStreamSynchronize(st_));
     }
 
     void propagate(Stats *stats) override {
-        hipEvent_t e0 = ev(0), e1 = ev(1);
-        HIPCHK(hipEventRecord(e0, st_));
+        // no_timing: no events at all (the run creates no HIP objects); the
+        // stage times stay 0 and the stream is synchronised instead
+        const bool timing = !o_.no_timing;
+        auto mark = [&](size_t i) {
+            if (!timing) return (hipEvent_t) nullptr;
+            hipEvent_t e = ev(i);
+            HIPCHK(hipEventRecord(e, st_));
+            return e;
+        };
+        auto elapsed = [&](hipEvent_t a, hipEvent_t b) {
+            float t = 0;
+            if (a && b) HIPCHK(hipEventElapsedTime(&t, a, b));
+            return (double)t;
+        };
+        hipEvent_t e0 = mark(0);
         double march_ms = 0, setup_ms = 0, out_ms = 0;
         if (stats) *stats = Stats();
         stats_ = stats;
         for (int w0 = 0; w0 < sz_.nwl; w0 += sz_.batch) {
             const int nslots = std::min(sz_.batch, sz_.nwl - w0);
             const int nprops = nslots * sz_.nq * 2;
-            hipEvent_t a = ev(2), b = ev(3), c = ev(4), d = ev(5);
-            HIPCHK(hipEventRecord(a, st_));
+            hipEvent_t a = mark(2);
             setup_batch(w0, nslots, nprops);
-            HIPCHK(hipEventRecord(b, st_));
+            hipEvent_t b = mark(3);
             C2<T> *state = march(nslots, nprops);
-            HIPCHK(hipEventRecord(c, st_));
+            hipEvent_t c = mark(4);
             output_batch(w0, nslots, nprops, state);
-            HIPCHK(hipEventRecord(d, st_));
-            HIPCHK(hipEventSynchronize(d));
+            hipEvent_t d = mark(5);
+            if (d) HIPCHK(hipEventSynchronize(d));
+            else HIPCHK(hipStreamSynchronize(st_));
             resolve_profile();
-            float t;
-            HIPCHK(hipEventElapsedTime(&t, a, b));
-            setup_ms += t;
-            HIPCHK(hipEventElapsedTime(&t, b, c));
-            march_ms += t;
-            HIPCHK(hipEventElapsedTime(&t, c, d));
-            out_ms += t;
+            setup_ms += elapsed(a, b);
+            march_ms += elapsed(b, c);
+            out_ms += elapsed(c, d);
         }
-        HIPCHK(hipEventRecord(e1, st_));
-        HIPCHK(hipEventSynchronize(e1));
+        hipEvent_t e1 = mark(1);
+        if (e1) HIPCHK(hipEventSynchronize(e1));
         if (p_.bulk) write_bulk();
         if (stats) {
-            float t;
-            HIPCHK(hipEventElapsedTime(&t, e0, e1));
-            stats->total_ms = t;
+            stats->total_ms = elapsed(e0, e1);
             stats->setup_ms = setup_ms;
             stats->march_ms = march_ms;
             stats->output_ms = out_ms;
@@ -316,6 +332,13 @@ private:
     DevBuf<C2<T>> cf_, xt_;
     DevBuf<C2<T>> fb_[4];
     DevBuf<c64> bulk_;   // bulk output planes (p.bulk)
+    // inputs and output as the kernels see them: the runner's own buffers, or
+    // the caller's device arrays (Options::dev)
+    const double *lc_p_ = nullptr, *w_p_ = nullptr;
+    const uint8_t *mask_p_ = nullptr;
+    c64 *E_p_ = nullptr;
+    std::vector<double> hq_;
+    size_t out_bytes() const { return (size_t)sz_.nwl * sz_.nq * 4 * sz_.Nfull * sizeof(c64); }
     DevBuf<C2<T>> bt_;   // one boundary-shifted plane of the bulk output
     LinePlan lpx_{}, lpy_{};
     Tuning tune_{};
@@ -400,7 +423,7 @@ private:
         }
         timed(KF_SETUP, [&] {
             HIPCHK(hipMemcpyAsync(sp_.ptr, hsp_.data(), nslots * sizeof(SlotParams), hipMemcpyHostToDevice, st_));
-            launch_nref(lc_.ptr, mask_.ptr, g_, sp_.ptr, nslots, partials_.ptr, kNrefBlocks, st_);
+            launch_nref(lc_p_, mask_p_, g_, sp_.ptr, nslots, partials_.ptr, kNrefBlocks, st_);
             HIPCHK(hipMemcpyAsync(hsp_.data(), sp_.ptr, nslots * sizeof(SlotParams), hipMemcpyDeviceToHost, st_));
             HIPCHK(hipStreamSynchronize(st_));
         });
@@ -458,10 +481,10 @@ private:
             HIPCHK(hipMemcpyAsync(pp_.ptr, hpp_.data(), nprops * sizeof(PropParams), hipMemcpyHostToDevice, st_));
             C2<T> *F = fb_[0].ptr;
             if (p.beam == BeamKind::None) {
-                const c64 *Ein = E_.ptr + (size_t)w0 * sz_.nq * 4 * sz_.Nfull;
+                const c64 *Ein = E_p_ + (size_t)w0 * sz_.nq * 4 * sz_.Nfull;
                 launch_pack_fields<T>(Ein, g_, pp_.ptr, nprops, sz_.nq, F, st_);
             } else {
-                launch_init_fields<T>(lc_.ptr, mask_.ptr, g_, sp_.ptr, pp_.ptr, nprops,
+                launch_init_fields<T>(lc_p_, mask_p_, g_, sp_.ptr, pp_.ptr, nprops,
                                       p.beam == BeamKind::Gaussian ? 1 : 0, p.waist, p.dx, p.dy, F, st_);
             }
         });
@@ -523,11 +546,13 @@ private:
             HIPCHK(hipEventRecord(x, s));
             return x;
         };
+        // the event that slab k's coefficients are ready (only when they run on
+        // the second stream; on the march stream the order is the stream's)
         auto coef = [&](int k) {
             timed(KF_COEFS, [&] {
-                launch_slab_coefs<T>(lc_.ptr, mask_.ptr, weights_.ptr, g_, sp_.ptr, nslots, k, CF(k), XT(k), lpx_, cs);
+                launch_slab_coefs<T>(lc_p_, mask_p_, w_p_, g_, sp_.ptr, nslots, k, CF(k), XT(k), lpx_, cs);
             });
-            return record(cs);
+            return overlap ? record(cs) : (hipEvent_t) nullptr;
         };
         if (overlap) HIPCHK(hipStreamWaitEvent(cs, record(st_), 0));
         std::vector<hipEvent_t> ready(nsl, nullptr);
@@ -549,12 +574,12 @@ private:
                          : plane_bytes <= (size_t(32) << 20) ? 6
                          : sizeof(T) == 8 ? 10 : 0;
         const int ord = ord0 | (sz_.nq << 4);  // group size of order bit 3: the pairs of one slot
-        HIPCHK(hipStreamWaitEvent(st_, ready[0], 0));
+        if (ready[0]) HIPCHK(hipStreamWaitEvent(st_, ready[0], 0));
         // bulk plane 0: the entrance fields after the input interface
         if (bulk_.ptr) {
             const int beam = p_.beam == BeamKind::None ? 2 : p_.beam == BeamKind::Gaussian ? 1 : 0;
-            launch_bulk_entrance(lc_.ptr, mask_.ptr, g_, sp_.ptr, pp_.ptr, nprops, beam, p_.waist, p_.dx, p_.dy,
-                                 E_.ptr, sz_.nq, bulk_.ptr, st_);
+            launch_bulk_entrance(lc_p_, mask_p_, g_, sp_.ptr, pp_.ptr, nprops, beam, p_.waist, p_.dx, p_.dy,
+                                 E_p_, sz_.nq, bulk_.ptr, st_);
         }
         timed(KF_BOUNDARY, [&] { launch_boundary<T>(other, cur, nullptr, CF(0), g_, pp_.ptr, nprops, 1, unshifted_[1], st_); });
         for (int k = 0; k < nsl; k++) {
@@ -597,7 +622,7 @@ private:
             }
             // P_k, Sh | Sh, P_{k+1}
             if (k + 1 < nsl) {
-                HIPCHK(hipStreamWaitEvent(st_, ready[k + 1], 0));
+                if (ready[k + 1]) HIPCHK(hipStreamWaitEvent(st_, ready[k + 1], 0));
                 timed(KF_BOUNDARY, [&] { launch_boundary<T>(cur, other, CF(k), CF(k + 1), g_, pp_.ptr, nprops, 2, unshifted_[2], st_); });
             } else {
                 timed(KF_BOUNDARY, [&] { launch_boundary<T>(cur, other, CF(k), nullptr, g_, pp_.ptr, nprops, 1, unshifted_[1], st_); });
@@ -657,9 +682,9 @@ private:
     }
 
     void output_batch(int w0, int nslots, int nprops, const C2<T> *state) {
-        c64 *Eb = E_.ptr + (size_t)w0 * sz_.nq * 4 * sz_.Nfull;
+        c64 *Eb = E_p_ + (size_t)w0 * sz_.nq * 4 * sz_.Nfull;
         timed(KF_OUTPUT, [&] {
-            launch_output<T>(state, lc_.ptr, mask_.ptr, g_, sp_.ptr, pp_.ptr, nprops, sz_.nq, Eb, st_);
+            launch_output<T>(state, lc_p_, mask_p_, g_, sp_.ptr, pp_.ptr, nprops, sz_.nq, Eb, st_);
         });
         timed(KF_FILTER, [&] {
             const int planes = nslots * sz_.nq * 4;
@@ -728,9 +753,17 @@ int64_t estimate_footprint(const Problem &p, const Options &o) {
     const int Mx = p.Nx > 3 ? p.Nx - 1 : p.Nx, My = p.Ny > 3 ? p.Ny - 1 : p.Ny;
     int64_t b = 0;
     auto add = [&](int64_t bytes) { b += (bytes + (int64_t)kAllocAlign - 1) / (int64_t)kAllocAlign * (int64_t)kAllocAlign; };
-    add(p.voxels() * p.field_dim * 8);
-    add((int64_t)p.mask.size());
-    add((int64_t)p.weights.size() * 8);
+    const bool dev_in = o.dev.lc != nullptr;  // the caller's device arrays: no director, weights or output copies
+    if (!dev_in) add(p.voxels() * p.field_dim * 8);
+    // a problem whose mask preparation is deferred holds its mask and (with a
+    // formula) the interface weights once prepared: counted at full size
+    const int64_t nw = 2 * (int64_t)p.Nx * p.Ny * std::max(0, p.Nz - 1);
+    if (dev_in) {
+        if (o.dev.mask) add(p.voxels());
+    } else {
+        add(p.mask_src ? p.voxels() : (int64_t)p.mask.size());
+        add((p.mask_src && !p.mask_formula.empty() ? nw : (int64_t)p.weights.size()) * 8);
+    }
     add((int64_t)s.nq * 2 * 8);
     add((int64_t)std::max<size_t>(1, p.iso_thick.size()) * 8);
     add((int64_t)std::max<size_t>(1, p.iso_index.size()) * 8);
@@ -746,7 +779,7 @@ int64_t estimate_footprint(const Problem &p, const Options &o) {
         add(2 * (int64_t)s.nslots_max * 2 * 3 * My * lp.lmax * lp.P * fsz);
     }
     for (int i = 0; i < 4; i++) add((int64_t)s.nprops_max * 2 * s.NP * fsz);
-    add((int64_t)s.nwl * s.nq * 4 * s.Nfull * 16);
+    if (!dev_in) add((int64_t)s.nwl * s.nq * 4 * s.Nfull * 16);
     if (p.bulk) {
         add((int64_t)s.nwl * s.nq * 2 * p.Nz * 2 * s.Nfull * 16);
         add((int64_t)s.nprops_max * 2 * s.NP * fsz);

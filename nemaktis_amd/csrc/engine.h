@@ -35,6 +35,17 @@ public:
     virtual void release(void *p) = 0;
 };
 
+// caller-owned device arrays a runner uses in place of its own buffers (all
+// or none): the director (caller layout), the mask as float64 (>= 0 is LC,
+// may be null), the interface weights (may be null) and the output E, which
+// also holds the input fields of "Beam profile": "None"
+struct DeviceIO {
+    const double *lc = nullptr;
+    const double *mask = nullptr;
+    const double *weights = nullptr;
+    c64 *E = nullptr;
+};
+
 struct Options {
     int precision = 64;  // 64: complex128 march; 32: complex64 march, FP64 coefficients
     int wl_batch = 1;    // wavelengths marched together (<= 0: all)
@@ -43,6 +54,9 @@ struct Options {
     int profile = 0;     // time every kernel family with events (no graph)
     Tuning tuning;       // x_chunks == 0: default shapes
     DeviceAllocator *allocator = nullptr;  // nullptr: hipMalloc / hipFree
+    DeviceIO dev;                // dev.lc set: inputs and output are the caller's device arrays
+    bool no_timing = false;      // no HIP events (stage times stay 0)
+    bool no_side_stream = false; // no second stream (coefficients on the march stream)
 };
 
 // per-kernel-family device time of one propagate() (profile mode); the slab

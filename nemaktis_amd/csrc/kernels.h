@@ -7,6 +7,10 @@
 
 namespace nmk {
 
+// LC flags of the voxels from a float64 mask on the device (mask >= 0 is LC,
+// UtilitiesLib/VectorField.cpp:61-62)
+void launch_mask_flags(const double *mask, uint8_t *flags, int64_t n, hipStream_t st);
+
 // sum of n_eff over every voxel of each slot's wavelength -> sp[slot].nref
 void launch_nref(const double *lc, const uint8_t *mask, const Geo &g, SlotParams *sp, int nslots, double *partials,
                  int nblocks, hipStream_t st);

@@ -78,6 +78,19 @@ __device__ __forceinline__ double eps_voxel(const double *__restrict__ lc, const
     return neff;
 }
 
+__global__ __launch_bounds__(256) void k_mask_flags(const double *__restrict__ mask, uint8_t *__restrict__ flags,
+                                                    int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        flags[i] = mask[i] >= 0 ? 1 : 0;
+}
+
+void launch_mask_flags(const double *mask, uint8_t *flags, int64_t n, hipStream_t st) {
+    if (n <= 0) return;
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_mask_flags, dim3(blocks), dim3(256), 0, st, mask, flags, n);
+    NMK_LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------------------
 // nref = mean n_eff over every voxel (PermittivityTensorField.cpp:94-98);
 // fixed block ranges and fixed reduction trees make it deterministic

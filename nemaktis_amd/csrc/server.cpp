@@ -130,11 +130,9 @@ int64_t Server::submit(const std::string &json, const char *mask_formula, const 
     auto job = std::make_shared<Job>();
     job->info.t_submit_ms = now_ms();
     if (!lc || n_lc < 0 || !E || n_E < 0) throw BpmError("serving: missing input or output array");
-    job->lc.assign(lc, lc + n_lc);
     const Settings s = parse_settings(json);
-    std::string warning;
-    job->problem = build_problem(s, job->lc.data(), n_lc, mask, mask ? n_mask : 0, mask ? mask_formula : nullptr,
-                                 n_E, &warning);
+    job->problem = build_problem(s, lc, n_lc, mask, mask ? n_mask : 0, mask ? mask_formula : nullptr, n_E, nullptr,
+                                 /*defer_mask=*/true);
     job->opt = opt;
     job->opt.device = -1;
     job->opt.profile = 0;
@@ -186,14 +184,23 @@ void Server::schedule_locked() {
 }
 
 void Server::run_job(Job &job, hipStream_t st) {
+    // host preparation of the job: per-voxel mask and interface weights
+    std::string notice;
+    prepare_mask(job.problem, &notice);
     SliceAllocator slice(arena_ + job.region, (size_t)job.info.footprint);
     Options o = job.opt;
     o.allocator = &slice;
     std::unique_ptr<Runner> r = make_runner(job.problem, o, st);
     // inputs are on the device now; the host copies are no longer needed
-    std::vector<double>().swap(job.lc);
     job.problem.lc = nullptr;
+    std::vector<uint8_t>().swap(job.problem.mask);
+    std::vector<double>().swap(job.problem.weights);
     if (job.problem.beam == BeamKind::None) r->upload_fields(job.E);
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        job.info.t_ready_ms = now_ms();
+        job.info.notice = notice;
+    }
     Stats s;
     r->propagate(&s);
     r->download(job.E);
@@ -245,7 +252,6 @@ void Server::worker(int w) {
             job->info.status = status;
             job->info.error = error;
             job->info.t_end_ms = now_ms();
-            std::vector<double>().swap(job->lc);
             job->problem = Problem();
             if (state == JOB_DONE) st_.completed++;
             else st_.failed++;
@@ -282,7 +288,6 @@ bool Server::cancel(int64_t id, bool *known) {
     queue_.erase(std::remove(queue_.begin(), queue_.end(), it->second), queue_.end());
     j.info.state = JOB_CANCELLED;
     j.info.t_end_ms = now_ms();
-    std::vector<double>().swap(j.lc);
     j.problem = Problem();
     st_.cancelled++;
     schedule_locked();

@@ -8,6 +8,7 @@
 // resident in a slice of one pre-allocated HBM arena.
 #pragma once
 
+#include <chrono>
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
@@ -37,11 +38,13 @@ struct JobInfo {
     int status = 0;            // nmk_bpm_status of a failed job
     double t_submit_ms = 0;    // host steady clock, ms since the server started
     double t_admit_ms = -1;
+    double t_ready_ms = -1;    // host preparation done and inputs on the device
     double t_end_ms = -1;
     double device_ms = 0;      // device time of the propagation
     double updates = 0;        // voxel-slice updates of the job
     int64_t footprint = 0;     // admitted bytes
     std::string error;
+    std::string notice;        // what the reference prints on success (mask formula mismatch)
 };
 
 struct ServerStats {
@@ -73,11 +76,14 @@ public:
     Server(const Server &) = delete;
     Server &operator=(const Server &) = delete;
 
-    // Validates the job (settings, array sizes, footprint against the arena)
-    // and queues it; throws BpmError on invalid input or a job that can never
-    // fit.  lc and mask are copied; E (complex128, caller layout) must stay
-    // valid until the job has finished: it is read first with "Beam profile":
-    // "None" and receives the output fields.
+    // Validates the job (settings, array sizes, the mask formula's syntax,
+    // footprint against the arena) and queues it; throws BpmError on invalid
+    // input or a job that can never fit.  Nothing of size O(volume) happens
+    // here: the worker that runs the job builds its mask and interface
+    // weights and uploads lc when the job is admitted.  lc, mask and E
+    // (complex128, caller layout) must therefore stay valid until the job has
+    // finished: E is read first with "Beam profile": "None" and receives the
+    // output fields.
     int64_t submit(const std::string &json, const char *mask_formula, const double *lc, int64_t n_lc,
                    const double *mask, int64_t n_mask, c64 *E, int64_t n_E, const Options &opt);
     // 0: the job has finished (done, failed or cancelled); 1: timed out
@@ -91,8 +97,7 @@ public:
 private:
     struct Job {
         int64_t id = 0;
-        std::vector<double> lc;
-        Problem problem;
+        Problem problem;  // lc and mask borrowed from the caller until the job ends
         Options opt;
         c64 *E = nullptr;
         size_t region = 0;

@@ -232,8 +232,23 @@ void interface_weights(const Problem &p, const std::string &formula, std::vector
     }
 }
 
+void prepare_mask(Problem &p, std::string *warning) {
+    if (!p.mask_src) return;
+    const int64_t nv = p.voxels();
+    p.mask.resize((size_t)nv);
+    for (int64_t i = 0; i < nv; i++) p.mask[i] = p.mask_src[i] >= 0 ? 1 : 0;
+    if (!p.mask_formula.empty()) {
+        try {
+            interface_weights(p, p.mask_formula, p.weights, warning);
+        } catch (const ExprError &e) {
+            throw BpmError(std::string("Wrong mask formula: ") + e.what());
+        }
+    }
+    p.mask_src = nullptr;
+}
+
 Problem build_problem(const Settings &s, const double *lc, int64_t n_lc, const double *mask, int64_t n_mask,
-                      const char *mask_formula, int64_t n_E, std::string *warning) {
+                      const char *mask_formula, int64_t n_E, std::string *warning, bool defer_mask) {
     Problem p;
     p.Nx = s.Nx;
     p.Ny = s.Ny;
@@ -251,18 +266,19 @@ Problem build_problem(const Settings &s, const double *lc, int64_t n_lc, const d
     }
     if (mask) {
         if (n_mask != nv) throw BpmError("Wrong dimension for the mask pointer array");
-        p.mask.resize((size_t)nv);
-        for (int64_t i = 0; i < nv; i++) p.mask[i] = mask[i] >= 0 ? 1 : 0;
+        p.mask_src = mask;
+        p.mask_formula = mask_formula ? mask_formula : "";
     }
     if (lc == nullptr && n_lc > 0) throw BpmError("Null LC field pointer");
     p.lc = lc;
-    if (mask && mask_formula && mask_formula[0] != '\0') {
+    if (mask && !p.mask_formula.empty()) {
         try {
-            interface_weights(p, mask_formula, p.weights, warning);
+            Expr(p.mask_formula, {"x", "y", "z"});  // syntax check now, evaluation with the mask
         } catch (const ExprError &e) {
             throw BpmError(std::string("Wrong mask formula: ") + e.what());
         }
     }
+    if (mask && !defer_mask) prepare_mask(p, warning);
     // PythonBackend/bpm_backend.cpp:29-31
     if (fd != 3 && fd != 6) throw BpmError("Unexpected dimension for the LC orientational field, should be 3 or 6.");
     p.field_dim = (int)fd;

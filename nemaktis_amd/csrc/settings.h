@@ -66,6 +66,11 @@ struct Problem {
     const double *lc = nullptr;
     std::vector<uint8_t> mask;      // empty: every voxel is LC
     std::vector<double> weights;    // empty: every interface weight is 0.5
+    // a masked problem validated with its mask preparation deferred: the
+    // caller's mask (borrowed) and formula, turned into `mask` / `weights` by
+    // prepare_mask
+    const double *mask_src = nullptr;
+    std::string mask_formula;
     int64_t voxels() const { return (int64_t)Nx * Ny * Nz; }
     int64_t n_out_values() const { return (int64_t)wl.size() * (int64_t)qx.size() * 4 * Nx * Ny; }
 };
@@ -73,8 +78,14 @@ struct Problem {
 // Builds the problem from the settings and the caller's arrays, applying the
 // reference's size checks in the reference's order.  mask == nullptr selects
 // the unmasked entry point.
+// With defer_mask the checks run (including the formula's syntax) but the
+// per-voxel mask and the interface weights are left to prepare_mask, so that
+// a caller that queues the problem (the serving loop) does that host work
+// later, off the submitting thread.
 Problem build_problem(const Settings &s, const double *lc, int64_t n_lc, const double *mask, int64_t n_mask,
-                      const char *mask_formula, int64_t n_E, std::string *warning);
+                      const char *mask_formula, int64_t n_E, std::string *warning, bool defer_mask = false);
+// builds the deferred mask and interface weights of a problem (no-op without)
+void prepare_mask(Problem &p, std::string *warning);
 
 // Interface interpolation weights of a mask formula (UtilitiesLib/
 // VectorField.cpp:64-113) for the mesh and mask of p: w[2*i], w[2*i+1] for

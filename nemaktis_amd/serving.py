@@ -49,6 +49,7 @@ class Job:
             raise TimeoutError("job %d still %s" % (self.id, d["state"]))
         self.info = d
         self._inputs = None
+        self.server._live.pop(self.id, None)
         return d
 
     def cancel(self):
@@ -70,6 +71,8 @@ class Server:
         self.h = self.lib.nmk_srv_create(ctypes.byref(cfg), err, len(err))
         if not self.h:
             raise ServingError(err.value.decode())
+        # the server borrows every job's arrays until the job has finished
+        self._live = {}
 
     def submit(self, json_str, lc_vals, mask_vals=None, mask_formula="", precision=32, wl_batch=0, use_graph=False,
                tuning=None, E=None, E_shape=None):
@@ -96,6 +99,7 @@ class Server:
                                       ctypes.byref(opts), err, len(err))
         if jid < 0:
             raise ServingError(err.value.decode())
+        self._live[jid] = (lc, mask, E)
         return Job(self, jid, E, (lc, mask))
 
     def stats(self):
@@ -105,8 +109,9 @@ class Server:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.nmk_srv_destroy(self.h)
+            self.lib.nmk_srv_destroy(self.h)  # cancels queued jobs, waits for the running ones
             self.h = None
+            self._live = {}
 
     def __enter__(self):
         return self
@@ -182,7 +187,14 @@ def percentiles(values, qs=(50, 99)):
 def run_trace(server, specs, cache=None, precision=32, wl_batch=0, keep_outputs=False, log=None, use_graph=False):
     """Replays the arrivals of ``specs`` against ``server`` (directors taken
     from / put into ``cache`` before the clock starts) and returns the
-    latency percentiles and the aggregate throughput."""
+    latency percentiles and the aggregate throughput.
+
+    Every time is on the server's clock and measured from the job's scheduled
+    arrival (the trace start plus ``arrival_s``), so a late submission counts
+    against the job: latency = end - arrival, split into queue (arrival to
+    admission), preparation (admission to inputs on the device: mask,
+    interface weights, director upload), device (the propagation) and the
+    rest (output download)."""
     import json as _json
     cache = {} if cache is None else cache
     prepared = []
@@ -191,6 +203,7 @@ def run_trace(server, specs, cache=None, precision=32, wl_batch=0, keep_outputs=
         prepared.append((s, _json.dumps(job_settings(s, sp)), lc, mask, formula))
     jobs = []
     t0 = time.perf_counter()
+    srv0 = server.stats()["now_ms"]
     for s, js, lc, mask, formula in prepared:
         delay = s["arrival_s"] - (time.perf_counter() - t0)
         if delay > 0:
@@ -205,22 +218,29 @@ def run_trace(server, specs, cache=None, precision=32, wl_batch=0, keep_outputs=
         info = j.wait()
         if info["state"] != "done":
             raise ServingError("job %d %s: %s" % (s["job"], info["state"], info["error"]))
+        info["arrival_ms"] = srv0 + 1e3 * s["arrival_s"]
         infos.append(info)
         if log:
-            log("job %d done: latency %.1f ms (queued %.1f ms, device %.1f ms)" % (
-                s["job"], info["t_end_ms"] - info["t_submit_ms"], info["t_admit_ms"] - info["t_submit_ms"],
-                info["device_ms"]))
-    lat = [i["t_end_ms"] - i["t_submit_ms"] for i in infos]
-    wait = [i["t_admit_ms"] - i["t_submit_ms"] for i in infos]
-    t_first = min(i["t_submit_ms"] for i in infos)
+            log("job %d done: latency %.1f ms (queue %.1f, prep %.1f, device %.1f)" % (
+                s["job"], info["t_end_ms"] - info["arrival_ms"], info["t_admit_ms"] - info["arrival_ms"],
+                info["t_ready_ms"] - info["t_admit_ms"], info["device_ms"]))
+    lat = [i["t_end_ms"] - i["arrival_ms"] for i in infos]
+    queue = [i["t_admit_ms"] - i["arrival_ms"] for i in infos]
+    prep = [i["t_ready_ms"] - i["t_admit_ms"] for i in infos]
+    dev = [i["device_ms"] for i in infos]
+    lag = [i["t_submit_ms"] - i["arrival_ms"] for i in infos]
+    t_first = min(i["arrival_ms"] for i in infos)
     t_last = max(i["t_end_ms"] for i in infos)
     upd = sum(i["updates"] for i in infos)
-    p = percentiles(lat)
+    p, pq, pp, pd = percentiles(lat), percentiles(queue), percentiles(prep), percentiles(dev)
     st = server.stats()
     out = {"jobs": len(infos), "p50_ms": p[50], "p99_ms": p[99], "mean_ms": float(np.mean(lat)),
-           "queue_wait_p50_ms": percentiles(wait)[50], "makespan_s": (t_last - t_first) * 1e-3,
+           "latency_from": "scheduled arrival",
+           "queue_p50_ms": pq[50], "queue_p99_ms": pq[99], "prep_p50_ms": pp[50], "prep_p99_ms": pp[99],
+           "device_p50_ms": pd[50], "device_p99_ms": pd[99], "submit_lag_max_ms": float(max(lag)),
+           "makespan_s": (t_last - t_first) * 1e-3,
            "updates": upd, "updates_per_s": upd / max(1e-9, (t_last - t_first) * 1e-3),
-           "device_busy_ms": float(sum(i["device_ms"] for i in infos)), "peak_active": st["peak_active"],
+           "device_busy_ms": float(sum(dev)), "peak_active": st["peak_active"],
            "arena_peak_bytes": st["arena_peak_bytes"], "arena_bytes": st["arena_bytes"]}
     if keep_outputs:
         out["outputs"] = [j.E for _, j in jobs]

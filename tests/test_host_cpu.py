@@ -267,3 +267,4 @@ def test_one_shot_calls_refuse_graph_replay():
     s, lc, E = _small()
     with pytest.raises(ValueError, match="Context"):
         bpm_backend.run_backend_without_mask(json.dumps(s), lc, E, use_graph=True)
+
