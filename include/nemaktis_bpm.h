@@ -101,6 +101,32 @@ int nmk_bpm_run(const char *json_str, const char *mask_formula, const double *lc
                 const double *mask_vals, int64_t n_mask_vals, double *E_field_vals, int64_t n_E_vals,
                 const nmk_bpm_options *opts, nmk_bpm_stats *stats, char *err_msg, size_t err_len);
 
+/* No counterpart in the reference.  Variant of nmk_bpm_run for callers that
+ * cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The inputs and the output are the
+ * caller's device arrays, in the layouts of nmk_bpm_run:
+ *   lc_field_vals_dev  director / Q-tensor (n_lc_vals doubles)
+ *   mask_vals_dev      float64 mask (>= 0 is LC), NULL: unmasked
+ *   weights_dev        interface weights, 2*Nx*Ny*(Nz-1) doubles as
+ *                      nmk_bpm_interface_weights computes them, NULL: all 0.5
+ *   E_field_vals_dev   output fields (complex128 as (re, im) pairs); holds the
+ *                      input fields when "Beam profile" is "None"
+ * Every other buffer (slab coefficients, march state, line rows, FFT twiddles,
+ * parameter blocks) is carved from one static device array of
+ * NMK_BPM_NOALLOC_SCRATCH_BYTES on each GPU the library runs on, held until
+ * the process ends.  The run allocates no device memory and creates no
+ * streams or events: one wavelength at a time, eager launches on `stream`
+ * (a hipStream_t; NULL: the per-thread default stream); it returns once the
+ * output is written.  A run whose buffers exceed the scratch is refused with
+ * NMK_BPM_ENOMEM before any device work; bulk output is refused with
+ * NMK_BPM_EINPUT.  All calls on one GPU share its scratch array: calls that
+ * overlap on one GPU race, so the caller serialises them. */
+#define NMK_BPM_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int nmk_bpm_run_noalloc(const char *json_str, const double *lc_field_vals_dev, int64_t n_lc_vals,
+                        const double *mask_vals_dev, int64_t n_mask_vals, const double *weights_dev,
+                        int64_t n_weights, double *E_field_vals_dev, int64_t n_E_vals, int32_t precision,
+                        void *stream, char *err_msg, size_t err_len);
+
 /* Context API: inputs uploaded once and kept resident in HBM, so that repeated
  * propagations (benchmarks, parameter sweeps) time the device work alone. */
 typedef struct nmk_bpm_ctx nmk_bpm_ctx;

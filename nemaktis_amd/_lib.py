@@ -95,7 +95,7 @@ WAVEPLATES = {"No": 0, "Quarter-wave": 1, "Half-wave": 2, "Tint-sensitive": 3}
 
 EXPORTED = [
     "nmk_bpm_run_backend_without_mask", "nmk_bpm_run_backend_with_mask", "nmk_bpm_default_options",
-    "nmk_bpm_run", "nmk_bpm_create", "nmk_bpm_upload_fields", "nmk_bpm_propagate", "nmk_bpm_download",
+    "nmk_bpm_run", "nmk_bpm_run_noalloc", "nmk_bpm_create", "nmk_bpm_upload_fields", "nmk_bpm_propagate", "nmk_bpm_download",
     "nmk_bpm_destroy", "nmk_bpm_set_tuning", "nmk_bpm_estimate_footprint", "nmk_bpm_interface_weights",
     "nmk_bpm_eval_expression", "nmk_bpm_version",
     "nmk_srv_default_config", "nmk_srv_create", "nmk_srv_submit", "nmk_srv_wait", "nmk_srv_cancel",
@@ -120,6 +120,8 @@ def load():
     lib.nmk_bpm_default_options.argtypes = [ctypes.POINTER(Options)]
     lib.nmk_bpm_run.argtypes = [cp, cp, c_double_p, i64, c_double_p, i64, c_double_p, i64,
                                 ctypes.POINTER(Options), ctypes.POINTER(Stats), cp, sz]
+    lib.nmk_bpm_run_noalloc.argtypes = [cp, ctypes.c_void_p, i64, ctypes.c_void_p, i64, ctypes.c_void_p, i64,
+                                        ctypes.c_void_p, i64, ctypes.c_int32, ctypes.c_void_p, cp, sz]
     lib.nmk_bpm_create.argtypes = [cp, cp, c_double_p, i64, c_double_p, i64, i64, ctypes.POINTER(Options), cp, sz]
     lib.nmk_bpm_create.restype = ctypes.c_void_p
     lib.nmk_bpm_upload_fields.argtypes = [ctypes.c_void_p, c_double_p, i64, cp, sz]

@@ -22,7 +22,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + 
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 
-HIP_SOURCES = ["kernels.hip", "viewer_kernels.hip"]
+HIP_SOURCES = ["kernels.hip", "viewer_kernels.hip", "noalloc.hip"]
 CXX_SOURCES = ["json.cpp", "expr.cpp", "settings.cpp", "vti.cpp", "engine.cpp", "server.cpp", "viewer.cpp", "capi.cpp"]
 
 

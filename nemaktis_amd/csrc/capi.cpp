@@ -12,6 +12,7 @@
 
 #include "engine.h"
 #include "expr.h"
+#include "noalloc.h"
 #include "server.h"
 #include "settings.h"
 #include "viewer.h"
@@ -79,6 +80,9 @@ int guarded(char *err, size_t len, F &&f, const std::string *notice = nullptr) {
         return NMK_BPM_EINPUT;
     } catch (const std::bad_alloc &) {
         set_err(err, len, "host out of memory");
+        return NMK_BPM_ENOMEM;
+    } catch (const ScratchExhausted &e) {
+        set_err(err, len, e.what());
         return NMK_BPM_ENOMEM;
     } catch (const std::runtime_error &e) {
         std::string m = e.what();
@@ -209,6 +213,32 @@ int nmk_bpm_run_backend_with_mask(const char *json, const char *formula, const d
     nmk_bpm_options o;
     nmk_bpm_default_options(&o);
     return nmk_bpm_run(json, formula ? formula : "", lc, n_lc, mask, n_mask, E, n_E, &o, nullptr, err, errlen);
+}
+
+int nmk_bpm_run_noalloc(const char *json, const double *lc_dev, int64_t n_lc, const double *mask_dev,
+                        int64_t n_mask, const double *weights_dev, int64_t n_weights, double *E_dev, int64_t n_E,
+                        int32_t precision, void *stream, char *err, size_t errlen) {
+    return guarded(err, errlen, [&] {
+        if (!json) throw BpmError("null settings string");
+        if (!lc_dev || !E_dev) throw BpmError("null device array");
+        if (precision != 32 && precision != 64) throw BpmError("precision must be 32 or 64");
+        const Settings s = parse_settings(json);
+        // the checks of nmk_bpm_run; the mask stays on the device (no formula:
+        // its interface weights come precomputed in weights_dev)
+        Problem p = build_problem(s, lc_dev, n_lc, mask_dev, mask_dev ? n_mask : 0, mask_dev ? "" : nullptr, n_E,
+                                  nullptr, /*defer_mask=*/true);
+        p.mask_src = nullptr;
+        p.lc = nullptr;
+        if (weights_dev && n_weights != 2 * (int64_t)p.Nx * p.Ny * (p.Nz - 1))
+            throw BpmError("the weights array needs 2*Nx*Ny*(Nz-1) values");
+        if (p.bulk) throw BpmError("the no-allocation entry point does not write bulk output");
+        DeviceIO io;
+        io.lc = lc_dev;
+        io.mask = mask_dev;
+        io.weights = weights_dev;
+        io.E = reinterpret_cast<c64 *>(E_dev);
+        run_noalloc(p, precision, io, static_cast<hipStream_t>(stream));
+    });
 }
 
 nmk_bpm_ctx *nmk_bpm_create(const char *json, const char *formula, const double *lc, int64_t n_lc,

@@ -268,3 +268,21 @@ def test_one_shot_calls_refuse_graph_replay():
     with pytest.raises(ValueError, match="Context"):
         bpm_backend.run_backend_without_mask(json.dumps(s), lc, E, use_graph=True)
 
+
+def test_noalloc_entry_point_refusals_before_device_work():
+    """nmk_bpm_run_noalloc checks its inputs and the scratch size before any
+    device call (the device pointers below are never dereferenced)."""
+    lib = _lib.load()
+    err = _lib.err_buf()
+    fake = ctypes.c_void_p(1 << 20)
+    s = fields.make_settings((1024, 1024, 3), (0.1,) * 3, [0.6])
+    n_lc, n_E = 1024 * 1024 * 3 * 3, 4 * 1024 * 1024
+    rc = lib.nmk_bpm_run_noalloc(json.dumps(s).encode(), fake, n_lc, None, 0, None, 0, fake, n_E, 64, None, err,
+                                 len(err))
+    assert rc == 3 and b"bytes of device scratch" in err.value, err.value
+    rc = lib.nmk_bpm_run_noalloc(json.dumps(s).encode(), fake, n_lc - 1, None, 0, None, 0, fake, n_E, 64, None, err,
+                                 len(err))
+    assert rc == 1 and b"Wrong dimension for the user pointer array" in err.value
+    rc = lib.nmk_bpm_run_noalloc(json.dumps(s).encode(), None, n_lc, None, 0, None, 0, fake, n_E, 64, None, err,
+                                 len(err))
+    assert rc == 1 and b"null device array" in err.value
