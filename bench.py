@@ -113,13 +113,12 @@ def probe(kind, steps):
         log("imaging probe: %d refocus + render steps, last %s" % (steps, of.timings()))
         of.close()
         return
-    from nemaktis_amd import fields, autotune, bpm_backend
+    from nemaktis_amd import fields, bpm_backend
+    tuning = None  # built-in launch shapes, as the benchmark's legs
     if kind == "march-droplet":
         s, lc, mask, formula = fields.workload("droplet256")
-        tuning = None
     else:
         s, lc, mask, formula = fields.workload("random512")
-        tuning, _ = autotune.tune(s, lc, precision=32, wl_batch=0, use_cache=True, log=log)
     js = json.dumps(s)
     n_E = int(np.prod(fields.output_shape(s)))
     with bpm_backend.Context(js, lc, n_E, mask_vals=mask, mask_formula=formula or "", precision=32, wl_batch=0,

@@ -108,6 +108,14 @@ Sizes sizes_of(const Problem &p, const Options &o) {
 
 constexpr int kNrefBlocks = 1024;
 
+// line solves of at most 256 chunks of 16 (FP32) or 8 (FP64) points
+void check_mesh_limits(const Problem &p, const Options &o) {
+    const int max_n = o.precision == 32 ? 4097 : 2049;
+    if (p.Nx > max_n || p.Ny > max_n)
+        throw BpmError("Mesh dimensions above " + std::to_string(max_n) + " points per axis are not supported (" +
+                       std::to_string(o.precision) + "-bit path)");
+}
+
 int pow2_at_least(int v) {
     int p = 1;
     while (p < v) p *= 2;
@@ -138,9 +146,7 @@ public:
         g_.field_dim = p.field_dim;
         g_.Nfull = sz_.Nfull;
         g_.NP = sz_.NP;
-        if (!fft_supported(p.Nx) || !fft_supported(p.Ny))
-            throw BpmError("Mesh dimensions must factor into primes <= 13 for the screen FFT");
-        if (p.Nx > 2048 || p.Ny > 2048) throw BpmError("Mesh dimensions above 2048 points are not supported");
+        check_mesh_limits(p, o);
         plan_x_ = make_fft_plan(p.Nx);
         plan_y_ = make_fft_plan(p.Ny);
 
@@ -711,7 +717,7 @@ Tuning default_tuning(int precision, int Mx, int My, int nprops) {
         const int64_t lines = (int64_t)std::max(1, other) * 2 * std::max(2, nprops);
         int par = 1;
         while (par < 64 && (int64_t)par * lines < 131072) par *= 2;
-        return std::min(128, std::max(16, std::max(pow2_at_least((M + lcap - 1) / lcap), par)));
+        return std::min(256, std::max(16, std::max(pow2_at_least((M + lcap - 1) / lcap), par)));
     };
     const size_t budget = max_block_lds();
     auto xlds = [&](int chunks, int rows, int nc) {
@@ -748,6 +754,7 @@ Tuning default_tuning(int precision, int Mx, int My, int nprops) {
 
 int64_t estimate_footprint(const Problem &p, const Options &o) {
     // mirrors the allocations of RunnerT, buffer by buffer
+    check_mesh_limits(p, o);
     const Sizes s = sizes_of(p, o);
     const int64_t fsz = o.precision == 32 ? 8 : 16;
     const int Mx = p.Nx > 3 ? p.Nx - 1 : p.Nx, My = p.Ny > 3 ? p.Ny - 1 : p.Ny;

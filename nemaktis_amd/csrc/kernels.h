@@ -98,12 +98,22 @@ void launch_bulk_plane(const C2<T> *F, const Geo &g, const SlotParams *sp, const
 // screen filter: batched 2-D FFT of howmany planes (Ny x Nx) and the per-k
 // 2x2 transfer filter
 // radices of the stages, 4 bits each from the lowest (at most 16 stages)
+// direct: the length has a prime factor above 13 or does not fit the LDS
+// plans (more than 2048 points); its lines are transformed by a direct DFT
 struct FftPlan {
     int n, nst;
     uint64_t radices;
+    int direct;
 };
 FftPlan make_fft_plan(int n);
+// true when the length factors into the radices of the LDS plans (2..13)
 bool fft_supported(int n);
+// unnormalised DFT of lines of length n, sign -1 forward, +1 backward, with
+// the twiddle table tw[m] = exp(-2 pi i m / n): `lines` lines per plane, line
+// i of plane p starting at p*plane_size + i*line_stride, elements elem_stride
+// apart; src may equal dst
+void launch_dft_lines(const c64 *src, c64 *dst, int n, int lines, int planes, int64_t plane_size,
+                      int64_t line_stride, int64_t elem_stride, int sign, const c64 *tw, hipStream_t st);
 void launch_fft2(c64 *data, int Nx, int Ny, int planes, int sign, const c64 *tw_x, const c64 *tw_y,
                  const FftPlan &px, const FftPlan &py, hipStream_t st);
 // the two passes of launch_fft2: rows (src may equal dst) and columns in place

@@ -2118,6 +2118,11 @@ bool fft_supported(int n) {
 FftPlan make_fft_plan(int n) {
     FftPlan pl{};
     pl.n = n;
+    if (n < 1) throw std::runtime_error("FFT length must be positive");
+    if (!fft_supported(n) || n > 2048) {
+        pl.direct = 1;
+        return pl;
+    }
     int m = n;
     pl.nst = 0;
     for (int r : {4, 2, 3, 5, 7, 11, 13})
@@ -2175,8 +2180,65 @@ __global__ __launch_bounds__(256) void k_fft_cols(c64 *__restrict__ data, int Nx
     }
 }
 
+// Direct DFT of lines (lengths the LDS plans do not take): a block holds C
+// lines and the twiddle table in LDS; output k of a line is
+// sum_j x_j tw[(j k) mod n] (conjugated twiddles for sign +1), with the twiddle
+// index advanced by k per term (no products or divisions in the loop).
+// Rows (elem_stride 1) run k fastest over one line per block, columns
+// (line_stride 1) the C adjacent lines fastest, so loads and stores coalesce.
+__global__ __launch_bounds__(256) void k_dft_lines(const c64 *src, c64 *dst, int n, int lines, int C,
+                                                   int groups, int64_t plane_size, int64_t line_stride,
+                                                   int64_t elem_stride, int sign, const c64 *__restrict__ tw) {
+    extern __shared__ c64 dbuf[];
+    c64 *stw = dbuf + (int64_t)C * n;
+    const int64_t plane = blockIdx.x / groups;
+    const int l0 = (int)(blockIdx.x % groups) * C;
+    const int nl = min(C, lines - l0);
+    const bool kfast = elem_stride == 1;
+    const int64_t base = plane * plane_size + (int64_t)l0 * line_stride;
+    const int tot = nl * n;
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        const int i = kfast ? t / n : t % nl, k = kfast ? t % n : t / nl;
+        dbuf[i * n + k] = src[base + i * line_stride + k * elem_stride];
+    }
+    for (int t = threadIdx.x; t < n; t += blockDim.x) stw[t] = sign > 0 ? conj(tw[t]) : tw[t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        const int i = kfast ? t / n : t % nl, k = kfast ? t % n : t / nl;
+        const c64 *x = dbuf + i * n;
+        c64 acc = mk(0.0, 0.0);
+        int idx = 0;
+        for (int j = 0; j < n; j++) {
+            acc = cmad(acc, x[j], stw[idx]);
+            idx += k;
+            if (idx >= n) idx -= n;
+        }
+        dst[base + i * line_stride + k * elem_stride] = acc;
+    }
+}
+
+void launch_dft_lines(const c64 *src, c64 *dst, int n, int lines, int planes, int64_t plane_size,
+                      int64_t line_stride, int64_t elem_stride, int sign, const c64 *tw, hipStream_t st) {
+    if (n < 1 || lines < 1 || planes < 1) return;
+    // columns: up to 8 adjacent lines per block (whole 128-byte rows); rows: one
+    int C = elem_stride == 1 ? 1 : 8;
+    while (C > 1 && (size_t)(C + 1) * n * sizeof(c64) > 64 * 1024) C /= 2;
+    const size_t lds = (size_t)(C + 1) * n * sizeof(c64);
+    allow_kernel_lds(reinterpret_cast<const void *>(k_dft_lines), lds);
+    const int groups = (lines + C - 1) / C;
+    const int64_t blocks = (int64_t)groups * planes;
+    if (blocks > 0x7fffffff) throw std::runtime_error("DFT: too many lines");
+    hipLaunchKernelGGL(k_dft_lines, dim3((unsigned)blocks), dim3(256), lds, st, src, dst, n, lines, C, groups,
+                       plane_size, line_stride, elem_stride, sign, tw);
+    NMK_LAUNCH_CHECK();
+}
+
 void launch_fft_rows(const c64 *src, c64 *dst, int Nx, int Ny, int planes, int sign, const c64 *tw_x,
                      const FftPlan &px, hipStream_t st) {
+    if (px.direct) {
+        launch_dft_lines(src, dst, Nx, Ny, planes, (int64_t)Nx * Ny, Nx, 1, sign, tw_x, st);
+        return;
+    }
     const int C = fft_lines_per_block(Nx, 16);
     const int64_t nlines = (int64_t)Ny * planes;
     const unsigned grid = (unsigned)((nlines + C - 1) / C);
@@ -2207,6 +2269,10 @@ void launch_fft_rows(const c64 *src, c64 *dst, int Nx, int Ny, int planes, int s
 
 void launch_fft_cols(c64 *data, int Nx, int Ny, int planes, int sign, const c64 *tw_y, const FftPlan &py,
                      hipStream_t st) {
+    if (py.direct) {
+        launch_dft_lines(data, data, Ny, Nx, planes, (int64_t)Nx * Ny, 1, Nx, sign, tw_y, st);
+        return;
+    }
     const int C = fft_cols_per_block(Ny);
     const int ngroups = (Nx + C - 1) / C;
     const int64_t nblocks = (int64_t)ngroups * planes;

@@ -62,8 +62,7 @@ void condenser_wavevectors(ViewGeometry &g) {
 
 void validate_geometry(ViewGeometry &g) {
     if (g.Nx < 1 || g.Ny < 1) throw BpmError("mesh_dimensions should be positive");
-    if (!fft_supported(g.Nx) || !fft_supported(g.Ny) || g.Nx > 2048 || g.Ny > 2048)
-        throw BpmError("mesh dimensions must have prime factors <= 13 and at most 2048 points per axis");
+    if (g.Nx > 4097 || g.Ny > 4097) throw BpmError("mesh dimensions above 4097 points per axis are not supported");
     if (g.wl.empty()) throw BpmError("at least one wavelength is required");
     for (double l : g.wl)
         if (!(l > 0)) throw BpmError("wavelengths must be positive");

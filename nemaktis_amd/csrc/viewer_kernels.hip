@@ -8,6 +8,7 @@
 #include <string>
 
 #include "fft_device.h"
+#include "kernels.h"
 #include "viewer.h"
 
 namespace nmk {
@@ -123,8 +124,27 @@ __global__ __launch_bounds__(NT) void k_view_columns(c64 *__restrict__ data, con
     }
 }
 
+// fields of the 4 components of every (wavelength, wavevector) times its
+// filter plane (the column pass of lengths without an LDS plan)
+__global__ __launch_bounds__(256) void k_view_apply_filter(c64 *__restrict__ data, const c64 *__restrict__ filt,
+                                                           int64_t N) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= N) return;
+    const int64_t plane = blockIdx.y;  // (wavelength, wavevector) * 4 + component
+    data[plane * N + m] = data[plane * N + m] * filt[(plane >> 2) * N + m];
+}
+
 void launch_view_columns(c64 *data, const c64 *filt, int Nx, int Ny, int nlq, const c64 *tw_y, const FftPlan &py,
                          hipStream_t st) {
+    if (py.direct) {
+        // direct DFT of the columns, the filter, and the inverse DFT
+        const int64_t N = (int64_t)Nx * Ny;
+        launch_dft_lines(data, data, Ny, Nx, nlq * 4, N, 1, Nx, -1, tw_y, st);
+        hipLaunchKernelGGL(k_view_apply_filter, dim3(blocks_of(N, 256), nlq * 4), dim3(256), 0, st, data, filt, N);
+        NMK_VIEW_LAUNCH_CHECK();
+        launch_dft_lines(data, data, Ny, Nx, nlq * 4, N, 1, Nx, +1, tw_y, st);
+        return;
+    }
     // power-of-two columns of at most 256 points: eight adjacent columns per
     // block of 512 threads, so that a row of the block is a whole 128-byte line
     const bool wide = fft_pow2(py) && 8 * Ny <= 2048 && Nx >= 8;

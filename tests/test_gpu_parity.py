@@ -345,3 +345,26 @@ def test_many_iso_layers_fp64():
         "Focalisation z-shift": 0.2})
     err = _rel(_gpu(s, lc), oracle.propagate(s, lc))
     assert err < TOL64, err
+
+
+@pytest.mark.parametrize("dims", [(17, 16, 4), (34, 19, 4), (16, 97, 3)])
+def test_mesh_sizes_with_large_prime_factors_fp64(dims):
+    """Screen FFT of lengths with prime factors 17, 19 and 97 (direct DFT on the
+    device; FFTW plans any length, BeamPropagationLib/OpticalFieldCollection.cpp:59-71)
+    against the oracle (Bluestein)."""
+    s = _base(dims, wavevectors=[(0.0, 0.0), (0.07, -0.04)], no="1.5", ne="1.7", na_objective=0.6)
+    lc = fields.random_smooth(*dims, (0.1, 0.1, 0.1), seed=47, corr_length=0.3)
+    err = _rel(_gpu(s, lc), oracle.propagate(s, lc))
+    assert err < TOL64, err
+    err32 = _rel(_gpu(s, lc, precision=32), oracle.propagate(s, lc))
+    assert err32 < TOL32, err32
+
+
+@pytest.mark.parametrize("dims", [(3000, 6, 3), (8, 2600, 3)])
+def test_lines_longer_than_2048_points_fp32(dims):
+    """Lines of 2999 and 2599 unknowns (256 chunks per line, reduced systems in
+    LDS) and screen transforms of 3000 and 2600 points (direct DFT), FP32."""
+    s = _base(dims, no="1.5", ne="1.7", na_objective=0.5)
+    lc = fields.random_smooth(*dims, (0.1, 0.1, 0.1), seed=53, corr_length=0.3)
+    err = _rel(_gpu(s, lc, precision=32), oracle.propagate(s, lc))
+    assert err < TOL32, err

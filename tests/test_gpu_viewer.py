@@ -37,6 +37,8 @@ CASES = [
     (1, 40, 0.1, 3.9, [0.55, 0.6], 1.0, 0.5, 0.3, 3, True, 0.0),      # 1-D Koehler along y
     (256, 256, 25.5, 25.5, [0.6], 1.0, 0.4, 0.0, 1, False, 0.0),     # 8 columns per block
     (520, 64, 51.9, 6.3, [0.6], 1.0, 0.4, 0.0, 1, False, 0.0),       # rows of 520, columns of 64
+    (17, 34, 1.6, 3.3, [0.55, 0.6], 1.0, 0.5, 0.2, 2, False, 0.0),    # prime 17: direct DFT rows and columns
+    (97, 2500, 9.6, 249.9, [0.6], 1.0, 0.4, 0.0, 1, False, 0.0),     # prime 97, 2500 > 2048: direct DFT
 ]
 
 

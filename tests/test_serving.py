@@ -1,7 +1,7 @@
 """Serving loop on the GPU: isolation (each result is the same alone and in
 the mix), admission by footprint under a small arena, refusal of a job that
-can never fit, cancellation of a queued job, and a job that fails at
-admission without disturbing the others.  Failure conditions are all forced
+can never fit, cancellation of a queued job, refusal of invalid jobs, and a
+job that fails after its march without disturbing the job beside it.  Failure conditions are all forced
 from the host (sizes, settings); no kernel is made to fail.
 
 Tolerance of "the same alone and in the mix": bit for bit (the kernels are
@@ -125,18 +125,19 @@ def test_cancel_queued_job():
     assert i1["state"] == "done"
 
 
-def test_invalid_and_failing_jobs_leave_the_others_alone():
+def test_invalid_jobs_are_refused_and_leave_the_others_alone():
     good = _job((64, 64, 8), seed=8)
-    # 17 points: accepted by the settings checks, refused by the screen FFT
-    # when the job is admitted
-    bad = _job((17, 16, 6), seed=9)
+    # 4099 points per line: beyond the line solvers (at most 4097 points per
+    # axis), refused at submission like malformed settings
+    bad = _job((4099, 4, 3), seed=9)
     with serving.Server(max_active=2) as srv:
         with pytest.raises(serving.ServingError):
             srv.submit("{not json", good[1])
-        j_bad = srv.submit(bad[0], bad[1], precision=32)
+        with pytest.raises(serving.ServingError, match="4097"):
+            srv.submit(bad[0], bad[1], precision=32)
         j_good = srv.submit(good[0], good[1], precision=32)
-        ib, ig = j_bad.wait(), j_good.wait()
-    assert ib["state"] == "failed" and "FFT" in ib["error"], ib
+        ig = j_good.wait()
+        assert srv.stats()["submitted"] == 1
     assert ig["state"] == "done"
     ref = bpm_backend.propagate(good[0], good[1], precision=32, wl_batch=0)
     assert np.array_equal(j_good.E, ref)
