@@ -6,10 +6,13 @@ director (1 um correlation), no = 1.52, ne = 1.74, wavelength 0.55 um, normal
 incidence, 2 input polarisations.  One step = one full propagation of that
 job (permittivity and coefficient precompute, input/output Fresnel, the
 99-slab z-march and the FFT screen filter) with the director already
-resident in HBM.  `value` is the FP32 path (complex64 march, FP64
-coefficients, hipGraph-replayed z-march); the FP64 plain path (complex128
-march, one wavelength at a time, eager launches) is timed beside it, as is
-the FP64 CPU oracle on a bounded sample of the same workload.
+resident in HBM.  `value` is the FP64 path (complex128 march, the reference's
+arithmetic, one wavelength at a time, z-march replayed as a hipGraph); beside
+it: the same path with eager launches, the FP32 batched path (complex64 march,
+FP64 coefficients) with its error against FP64, the roofline of the dominant
+kernel of each, the FP64 CPU oracle on the same workload (median of 3), the
+one-shot drop-in call timed end to end, and the serving, imaging, bulk-output
+and droplet-sweep legs.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -91,6 +94,7 @@ def probe(kind, steps):
     bench.py --probe KIND): 'copy' copies 256 MiB device to device `steps`
     times (counter calibration), 'march' runs the FP32 workload, 'march-droplet'
     the 256x256x80 droplet sweep (294 propagations, FP32, built-in shapes),
+    'march64' the FP64 path of the workload (one wavelength at a time),
     'imaging' the refocus + micrograph step, all with eager launches."""
     if kind == "copy":
         import torch
@@ -115,14 +119,17 @@ def probe(kind, steps):
         return
     from nemaktis_amd import fields, bpm_backend
     tuning = None  # built-in launch shapes, as the benchmark's legs
+    precision, wl_batch = 32, 0
     if kind == "march-droplet":
         s, lc, mask, formula = fields.workload("droplet256")
     else:
         s, lc, mask, formula = fields.workload("random512")
+        if kind == "march64":
+            precision, wl_batch = 64, 1
     js = json.dumps(s)
     n_E = int(np.prod(fields.output_shape(s)))
-    with bpm_backend.Context(js, lc, n_E, mask_vals=mask, mask_formula=formula or "", precision=32, wl_batch=0,
-                             use_graph=False, tuning=tuning) as ctx:
+    with bpm_backend.Context(js, lc, n_E, mask_vals=mask, mask_formula=formula or "", precision=precision,
+                             wl_batch=wl_batch, use_graph=False, tuning=tuning) as ctx:
         for _ in range(steps):
             st = ctx.propagate()
     log("march probe: %d propagations, last %.2f ms, tuning %s" % (steps, st["total_ms"], tuning))
@@ -327,17 +334,118 @@ def run_profile_leg(js, lc, n_E, precision, wl_batch, steps, tuning=None):
     return fam_ms, fam_n
 
 
-def cpu_baseline(settings_fn, lc_full, nz_sample):
+def host_facts():
+    """CPU model, hardware threads available to this process (affinity and
+    OMP_NUM_THREADS) and memory of the host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    mem_gb = None
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal:"):
+                    mem_gb = round(int(line.split()[1]) / 1024 ** 2, 1)
+                    break
+    except OSError:
+        pass
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return {"cpu_model": model, "threads": threads, "memory_gb": mem_gb}
+
+
+def cpu_baseline(settings_fn, lc_full, nz_sample, repeats=3):
+    """The FP64 oracle (the repository's CPU restatement, OpenMP on every
+    available host thread) on the same director: median of `repeats` runs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from nemaktis_amd import fields
     s = settings_fn(nz_sample)
     lc = np.ascontiguousarray(lc_full[:nz_sample])
-    threads = os.cpu_count() or 1
+    facts = host_facts()
+    times, out = [], None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        out = oracle.propagate(s, lc, nthreads=facts["threads"])
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    return fields.updates_per_run(s) / dt, facts, times, out
+
+
+def drop_in_leg(js, lc, shape, precision):
+    """The one-shot backend call as nemaktis makes it (host arrays in, host
+    array out), timed end to end, beside its parts on a context: creation
+    (allocation and director upload), the device work, the download."""
+    from nemaktis_amd import bpm_backend
+    E = np.zeros(shape, np.complex128)
+    st = {}
+    bpm_backend.run_backend_without_mask(js, lc, E, precision=precision, strict=True)  # first call loads the code
     t0 = time.perf_counter()
-    oracle.propagate(s, lc, nthreads=threads)
-    dt = time.perf_counter() - t0
-    return fields.updates_per_run(s) / dt, threads, dt
+    bpm_backend.run_backend_without_mask(js, lc, E, precision=precision, strict=True, stats=st)
+    wall = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ctx = bpm_backend.Context(js, lc, E.size, precision=precision)
+    create = time.perf_counter() - t0
+    try:
+        t0 = time.perf_counter()
+        dev = ctx.propagate()
+        prop = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ctx.download(E)
+        down = time.perf_counter() - t0
+    finally:
+        ctx.close()
+    return {"call_ms": 1e3 * wall, "device_ms": st.get("total_ms"),
+            "context_create_ms": 1e3 * create, "director_gb": lc.nbytes / 1e9,
+            "upload_gbs": lc.nbytes / 1e9 / create, "propagate_wall_ms": 1e3 * prop,
+            "propagate_device_ms": dev["total_ms"], "download_ms": 1e3 * down,
+            "download_gbs": E.nbytes / 1e9 / max(down, 1e-9),
+            "note": "one-shot run_backend_without_mask on host arrays (copy-in of the director, eager launches, "
+                    "copy-out of E); context_create = allocation + director upload"}
+
+
+def roofline_of(js, lc, n_E, precision, wl_batch, steps, Nx, Ny, nprops, nq, fd, nw, tuning=None):
+    """Dominant march kernel family of the path, its average launch time
+    (HIP events on its launch stream, eager), and its bandwidth two ways:
+    the workload's algorithmic bytes per voxel-slice update (64 / 128 B of
+    field traffic + 24 B of director per slot, shared by the slot's 2 Nq
+    propagations) times the updates one launch advances (every propagation
+    at every point of the full grid: nprops * Nx * Ny), and the kernel's own
+    unique bytes per launch (its field, coefficient and line-row planes)."""
+    fam_ms, fam_n = run_profile_leg(js, lc, n_E, precision, wl_batch, steps, tuning)
+    march_fams = [f for f in FAMILIES if f not in ("setup", "output", "filter") and fam_n[f]]
+    dom = max(march_fams, key=lambda f: fam_ms[f])
+    avg_ms = fam_ms[dom] / fam_n[dom]
+    per_update = (64 if precision == 32 else 128) + 24.0 / (2 * nq)
+    units = nprops * Nx * Ny
+    algo = per_update * units
+    achieved = algo / (avg_ms * 1e-3) / 1e9
+    c = 8 if precision == 32 else 16
+    local = algorithmic_bytes(dom, c, nprops, nprops // (2 * nq), (Nx - 1) * (Ny - 1), Nx * Ny, fd, nw)
+    tag = "fp32" if precision == 32 else "fp64"
+    traffic, traffic_src = measured_traffic(KERNELS.get(dom, dom), os.path.join(ROOT, "profiles",
+                                                                              "hbm_traffic_random512_%s.txt" % tag))
+    return {"bound": "hbm", "kernel": KERNELS.get(dom, dom), "family": dom, "precision": precision,
+            "avg_launch_ms": avg_ms, "launches": fam_n[dom], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "traffic_source": traffic_src,
+            "algorithmic_bytes_per_launch": algo,
+            "algorithmic_basis": "%.1f B per voxel-slice update x %d updates per launch (%d propagations x %dx%d)"
+                                 % (per_update, units, nprops, Nx, Ny),
+            "kernel_local_bytes_per_launch": local,
+            "kernel_local_frac": round(local / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5) if local else None,
+            "family_share": {f: round(fam_ms[f] / max(1e-9, sum(fam_ms.values())), 4) for f in FAMILIES}}
 
 
 def main():
@@ -347,17 +455,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-nz", type=int, default=100, help="planes of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="built-in line-solver shapes instead of the autotuner")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--tune", action="store_true", help="autotuned line-solver shapes instead of the built-in ones")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop trace")
-    ap.add_argument("--serve-jobs", type=int, default=8)
-    ap.add_argument("--serve-rate", type=float, default=1.0, help="mean job arrivals per second")
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=0.5, help="mean job arrivals per second")
+    ap.add_argument("--serve-only", action="store_true", help="run the serving trace alone (one JSON line)")
     ap.add_argument("--serve-active", type=int, default=4, help="jobs on the device at once")
     ap.add_argument("--no-imaging", action="store_true", help="skip the refocus + micrograph leg")
     ap.add_argument("--no-bulk", action="store_true", help="skip the bulk-output leg")
     ap.add_argument("--no-droplet", action="store_true", help="skip the 294-propagation droplet sweep leg")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the one-shot drop-in call leg")
     ap.add_argument("--bulk-dir", default="/tmp/nemaktis_amd_bench", help="folder of the bulk-output file")
-    ap.add_argument("--probe", choices=["copy", "march", "march-droplet", "imaging"],
+    ap.add_argument("--probe", choices=["copy", "march", "march64", "march-droplet", "imaging"],
                     help="short run for a hardware-counter pass")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -372,6 +482,9 @@ def main():
         return
     from nemaktis_amd import _lib, fields
     _lib.load()
+    if args.serve_only:
+        print(json.dumps({"serving": serve_leg(args.serve_jobs, args.serve_rate, 5, args.serve_active)}), flush=True)
+        return
     Nx = Ny = 512
     Nz = 100
     sp = (0.1, 0.1, 0.1)
@@ -387,54 +500,65 @@ def main():
     n_E = int(np.prod(shape))
     updates = fields.updates_per_run(s)
 
-    tuning = None
-    if not args.no_tune:
+    tuning32 = tuning64 = None
+    if args.tune:
         from nemaktis_amd import autotune
-        log("autotuning the line-solver shapes (FP32, shallow copy of the job) ...")
-        tuning, _ = autotune.tune(s, lc, precision=32, wl_batch=0, use_cache=True, log=log)
+        log("autotuning the line-solver shapes (shallow copies of the job) ...")
+        tuning64, _ = autotune.tune(s, lc, precision=64, wl_batch=1, use_cache=True, log=log, use_graph=True)
+        tuning32, _ = autotune.tune(s, lc, precision=32, wl_batch=0, use_cache=True, log=log)
 
-    log("FP32 batched path (graph) ...")
-    wall32, dev32, st32, out32 = run_gpu_leg(js, lc, n_E, shape, 32, 0, True, args.steps, args.warmup, tuning)
-    ms32 = 1e3 * wall32 / args.steps
-    value = updates * args.steps / wall32
+    # headline: the FP64 path (complex128 march, the reference's arithmetic),
+    # one wavelength at a time, z-march replayed as a hipGraph
+    log("FP64 path (graph) ...")
+    wall64, dev64, st64, out64 = run_gpu_leg(js, lc, n_E, shape, 64, 1, True, args.steps, args.warmup, tuning64)
+    ms64 = 1e3 * wall64 / args.steps
+    value = updates * args.steps / wall64
 
-    fp64 = None
-    if not args.no_fp64:
-        log("FP64 plain path ...")
-        k64 = max(1, args.steps // 2)
-        tuning64 = None
-        if not args.no_tune:
-            from nemaktis_amd import autotune
-            tuning64, _ = autotune.tune(s, lc, precision=64, wl_batch=1, use_cache=True, log=log, use_graph=False)
-        wall64, dev64, st64, out64 = run_gpu_leg(js, lc, n_E, shape, 64, 1, False, k64, 1, tuning64)
+    log("FP64 plain path (eager launches) ...")
+    k64e = max(1, args.steps // 2)
+    walle, deve, _, oute = run_gpu_leg(js, lc, n_E, shape, 64, 1, False, k64e, 1, tuning64)
+    fp64_eager = {"value": updates * k64e / walle, "ms_per_step": 1e3 * walle / k64e, "steps": k64e,
+                  "bit_equal_to_graph": bool(np.array_equal(oute, out64))}
+
+    fp32 = None
+    if not args.no_fp32:
+        log("FP32 batched path (graph) ...")
+        wall32, dev32, _, out32 = run_gpu_leg(js, lc, n_E, shape, 32, 0, True, args.steps, args.warmup, tuning32)
         errs = rel_errors(out32, out64)
-        fp64 = {"value": updates * k64 / wall64, "ms_per_step": 1e3 * wall64 / k64,
-                "fp32_speedup": (updates * args.steps / wall32) / (updates * k64 / wall64),
-                "fp32_rel_l2_max": float(errs.max()), "fp32_rel_l2_median": float(np.median(errs)),
-                "line_solver_shapes": tuning64}
+        fp32 = {"value": updates * args.steps / wall32, "ms_per_step": 1e3 * wall32 / args.steps,
+                "device_ms_per_step_median": float(np.median(dev32)),
+                "speedup_vs_fp64_graph": ms64 / (1e3 * wall32 / args.steps),
+                "speedup_vs_fp64_eager": fp64_eager["ms_per_step"] / (1e3 * wall32 / args.steps),
+                "rel_l2_vs_fp64_max": float(errs.max()), "rel_l2_vs_fp64_median": float(np.median(errs)),
+                "dtype": "c64 (FP64 permittivity, coefficient precompute and phase bookkeeping)"}
 
-    log("profiled pass (kernel-family event timing) ...")
-    fam_ms, fam_n = run_profile_leg(js, lc, n_E, 32, 0, max(1, min(args.steps, 3)), tuning)
-    march_fams = [f for f in FAMILIES if f not in ("setup", "output", "filter") and fam_n[f]]
-    dom = max(march_fams, key=lambda f: fam_ms[f])
-    avg_ms = fam_ms[dom] / fam_n[dom]
-    Mx, My = Nx - 1, Ny - 1
-    algo = algorithmic_bytes(dom, 8, 2, 1, Mx * My, Nx * Ny, 3, 2)
-    achieved = algo / (avg_ms * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic(KERNELS.get(dom, dom))
-    roofline = {"bound": "hbm", "kernel": KERNELS.get(dom, dom), "family": dom, "avg_launch_ms": avg_ms,
-                "launches": fam_n[dom], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "traffic_source": traffic_src, "algorithmic_bytes_per_launch": algo,
-                "family_share": {f: round(fam_ms[f] / max(1e-9, sum(fam_ms.values())), 4) for f in FAMILIES}}
+    log("profiled passes (kernel-family event timing) ...")
+    prof_steps = max(1, min(args.steps, 3))
+    roofline = roofline_of(js, lc, n_E, 64, 1, prof_steps, Nx, Ny, 2, 1, 3, 2, tuning64)
+    roofline["step_achieved"] = round(updates * (128 + 12.0) / (ms64 * 1e-3) / 1e9, 2)
+    roofline["step_frac"] = round(roofline["step_achieved"] / HBM_PEAK_GBS, 5)
+    if fp32:
+        r32 = roofline_of(js, lc, n_E, 32, 0, prof_steps, Nx, Ny, 2, 1, 3, 2, tuning32)
+        r32["step_achieved"] = round(updates * (64 + 12.0) / (fp32["ms_per_step"] * 1e-3) / 1e9, 2)
+        r32["step_frac"] = round(r32["step_achieved"] / HBM_PEAK_GBS, 5)
+        fp32["roofline"] = r32
 
     cpu = None
     if not args.no_cpu:
-        log("CPU oracle sample ...")
-        v, threads, dt = cpu_baseline(settings_for, lc, args.cpu_nz)
-        cpu = {"value": v, "unit": "voxel-slice updates/s", "cores": threads, "kind": "port",
-               "sample": "FP64 oracle, 512x512x%d slice of the same director (%d slabs, 2 polarisations), "
-                         "%.1f s wall" % (args.cpu_nz, args.cpu_nz - 1, dt)}
+        log("CPU oracle (median of 3) ...")
+        v, facts, times, cpu_out = cpu_baseline(settings_for, lc, args.cpu_nz)
+        cpu = {"value": v, "unit": "voxel-slice updates/s", "cores": facts["threads"], "kind": "port",
+               "sample": "FP64 oracle on the same workload, 512x512x%d (%d slabs, 2 polarisations), median of %d "
+                         "runs (%s s)" % (args.cpu_nz, args.cpu_nz - 1, len(times),
+                                          ", ".join("%.2f" % t for t in times)),
+               "cpu_model": facts["cpu_model"], "memory_gb": facts["memory_gb"], "build": "g++ -O3 -fopenmp"}
+        if args.cpu_nz == Nz:
+            cpu["fp64_gpu_rel_l2_vs_oracle_max"] = float(rel_errors(out64, cpu_out).max())
+
+    dropin = None
+    if not args.no_dropin:
+        log("one-shot drop-in call ...")
+        dropin = {"fp64": drop_in_leg(js, lc, shape, 64)}
 
     imaging = None
     if not args.no_imaging:
@@ -444,7 +568,7 @@ def main():
     bulk = None
     if not args.no_bulk:
         log("bulk-output leg ...")
-        bulk = bulk_leg(s, lc, tuning, args.bulk_dir)
+        bulk = bulk_leg(s, lc, tuning32, args.bulk_dir)
 
     droplet = None
     if not args.no_droplet:
@@ -457,17 +581,21 @@ def main():
         serve = serve_leg(args.serve_jobs, args.serve_rate, 5, args.serve_active)
 
     line = {
-        "metric": "voxel-slice updates/s at 512x512x100 (FP32 path; FP64 plain path beside it); "
+        "metric": "voxel-slice updates/s at 512x512x100 (FP64 path; FP32 batched path beside it); "
                   "p50/p99 micrograph latency of the serving trace",
         "value": value, "unit": "voxel-slice updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms32, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c64 (FP64 coefficient precompute)", "data": "synthetic",
+        "warmup": args.warmup, "ms_per_step": ms64, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128 (f64)", "data": "synthetic",
         "config": {"workload": "random-smooth director 512x512x100, dx=0.1um, lambda=0.55um, no=1.52, "
                                "ne=1.74, normal incidence, 2 polarisations", "updates_per_step": updates,
-                   "device_ms_per_step_median": float(np.median(dev32)), "line_solver_shapes": tuning},
-        "fp64_plain": fp64,
+                   "path": "FP64, one wavelength at a time, z-march replayed as a hipGraph, built-in launch shapes"
+                           if tuning64 is None else "FP64, graph, autotuned shapes",
+                   "device_ms_per_step_median": float(np.median(dev64)), "line_solver_shapes": tuning64},
+        "fp64_eager": fp64_eager,
+        "fp32_batched": fp32,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "drop_in_call": dropin,
         "serving": serve,
         "imaging": imaging,
         "bulk_output": bulk,
@@ -475,6 +603,8 @@ def main():
     }
     if cpu:
         line["vs_cpu"] = value / cpu["value"]
+        if fp32:
+            fp32["vs_cpu"] = fp32["value"] / cpu["value"]
     print(json.dumps(line), flush=True)
 
 

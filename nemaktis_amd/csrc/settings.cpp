@@ -170,6 +170,7 @@ void interface_weights(const Problem &p, const std::string &formula, std::vector
     const Expr e(formula, {"x", "y", "z"});
     auto columns = [&](int64_t beg, int64_t end) {
         bool failed = false;
+        std::vector<double> fplane((size_t)Nz);  // the formula on the planes of one column
         for (int64_t iperp = beg; iperp < end; iperp++) {
             int ix = int(iperp % Nx), iy = int(iperp / Nx);
             double v[3] = {(ix - 0.5 * (Nx - 1)) * p.dx, (iy - 0.5 * (Ny - 1)) * p.dy, 0};
@@ -177,11 +178,14 @@ void interface_weights(const Problem &p, const std::string &formula, std::vector
                 v[2] = z;
                 return e.eval(v);
             };
+            // each plane bounds two slabs: evaluated once (the same values the
+            // slab-by-slab evaluation gives)
+            for (int iz = 0; iz < Nz; iz++) fplane[iz] = f((iz - 0.5 * (Nz - 1)) * p.dz);
             for (int iz = 0; iz < Nz - 1; iz++) {
                 int64_t i = iperp + N * iz;
                 double zi = (iz - 0.5 * (Nz - 1)) * p.dz;
                 double zm = zi, zp = (iz + 1 - 0.5 * (Nz - 1)) * p.dz, z = zm;
-                double fm = f(zm), fp = f(zp);
+                double fm = fplane[iz], fp = fplane[iz + 1];
                 if (fm * fp <= 0) {
                     bool m0 = p.mask.empty() ? true : p.mask[i] != 0;
                     bool m1 = p.mask.empty() ? true : p.mask[i + N] != 0;
