@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libnemaktis_bpm.so")
+# NEMAKTIS_AMD_LIB selects another build of the library (A/B comparisons)
+LIB_PATH = os.environ.get("NEMAKTIS_AMD_LIB") or os.path.join(HERE, "lib", "libnemaktis_bpm.so")
 
 _lib = None
 
