@@ -15,9 +15,10 @@ for) comes back to the host.  There is no CPU fallback.
 Differences from the reference, by construction:
   * VTI import/export (``OpticalFields(vti_file=...)``, ``save_to_vti``) and
     the GUI (``FieldViewer.plot``) are not provided.
-  * ``cmf`` must be given for colour images: the default table comes from
-    dtmm's data files (``dtmm.color.load_tcmf``), which are not shipped here.
-    Without it the spectrum is still computed and ``get_image`` raises.
+  * Without ``cmf`` the colour table is ``colour.default_cmf`` (CIE 1931
+    functions by a published analytic fit, a 6504 K black body for D65,
+    normalised to the D65 white point) instead of dtmm's tabulated
+    ``dtmm.color.load_tcmf``, which is not shipped here.
   * ``FieldViewer.NA_objective`` returns the objective NA (the reference
     declares its setter with ``@z_focus.setter``, so its getter returns the
     focus altitude).
@@ -26,7 +27,7 @@ import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, colour
 
 
 class ViewerError(RuntimeError):
@@ -318,7 +319,8 @@ class FieldViewer:
     (reference: field_viewer.py:16-470).
 
     ``FieldViewer(optical_fields, cmf=None, gamma=True)``; ``cmf`` is the (N_wavelengths, 3) colour matching
-    table of the wavelengths (the ``cmf`` argument of the reference)."""
+    table of the wavelengths (the ``cmf`` argument of the reference); ``None`` selects
+    ``colour.default_cmf`` of the fields' wavelengths."""
 
     polariser = True
     analyser = True
@@ -346,7 +348,9 @@ class FieldViewer:
         Nx, Ny = optical_fields.get_mesh_dimensions()
         self._q_averaged_specter = np.zeros((Ny, Nx, Nl))
         self._image = np.zeros((Ny, Nx, 3))
-        self._cmf = None if cmf is None else np.ascontiguousarray(np.asarray(cmf, dtype=np.float64).reshape(Nl, 3))
+        if cmf is None:
+            cmf = colour.default_cmf(optical_fields.get_wavelengths())
+        self._cmf = np.ascontiguousarray(np.asarray(cmf, dtype=np.float64).reshape(Nl, 3))
         self._koehler_1D = optical_fields._koehler_1D
         self._N_radial_wavevectors = optical_fields._N_radial_wavevectors
 

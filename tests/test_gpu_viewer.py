@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import viewer_oracle as vo
-from nemaktis_amd import bpm_backend, fields, viewer
+from nemaktis_amd import bpm_backend, colour, fields, viewer
 
 pytestmark = pytest.mark.gpu
 
@@ -163,9 +163,31 @@ def test_view_from_context_equals_view_from_host_fields():
 def test_view_errors_are_reported():
     of = _make(CASES[0])
     fv = viewer.FieldViewer(of)
-    fv.update_image()
-    with pytest.raises(viewer.ViewerError, match="cmf"):
-        fv.get_image()
     fv.lower_waveplate = "Full-wave"
     with pytest.raises(ValueError):
         fv.update_image()
+
+
+def test_image_with_the_default_colour_table():
+    """Without cmf the viewer renders with colour.default_cmf of the fields'
+    wavelengths (the table itself is pinned on the CPU); a fully
+    transmitting medium (uniform x-polarised field, no polariser or
+    analyser) renders as white."""
+    of = _make(CASES[2], seed=5)
+    wl = of.get_wavelengths()
+    fv = viewer.FieldViewer(of)
+    fv.analyser = False
+    fv.update_image()
+    want = vo.colour(vo.spectrum(of.focused_vals, wl, dict(analyser=False), CASES[2][8], CASES[2][7]),
+                     colour.default_cmf(wl), 1.0, gamma=True)
+    np.testing.assert_allclose(np.flip(fv.get_image(), axis=0), want, rtol=0, atol=1e-11)
+    flat = _make(CASES[0], seed=6)
+    v = np.zeros(flat.vals.shape, np.complex128)
+    v[:, :, 0] = 1.0  # x input -> unit Ex
+    v[:, :, 3] = 1.0  # y input -> unit Ey
+    flat.vals = v
+    fw = viewer.FieldViewer(flat)
+    fw.polariser = fw.analyser = False
+    fw.update_image()
+    # D65 white through the sRGB matrix (7 significant digits) and the transfer function
+    np.testing.assert_allclose(fw.get_image(), 1.0, atol=1e-5)
