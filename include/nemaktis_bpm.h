@@ -160,6 +160,16 @@ int nmk_bpm_interface_weights(const char *json_str, const char *mask_formula, co
                               int64_t n_mask_vals, double *weights, int64_t n_weights, char *err_msg,
                               size_t err_len);
 
+/* The same weights computed on the current HIP device, as a masked run
+ * computes them (one thread per column, the formula compiled to a postfix
+ * program; arithmetic operation for operation the host version's, elementary
+ * functions from the device math library).  No counterpart in the reference,
+ * which evaluates the formula on the host; for callers that keep the weights
+ * and for checking the device evaluation against the host one. */
+int nmk_bpm_interface_weights_device(const char *json_str, const char *mask_formula, const double *mask_vals,
+                                     int64_t n_mask_vals, double *weights, int64_t n_weights, char *err_msg,
+                                     size_t err_len);
+
 /* Evaluates an index expression (muParser syntax, variable `lambda` in um) as
  * the backend does for "no", "ne", ... (PhysicsLib/PhysicsCoefficients.cpp:27-103). */
 int nmk_bpm_eval_expression(const char *expr, double lambda, double *value, char *err_msg, size_t err_len);

@@ -98,6 +98,7 @@ EXPORTED = [
     "nmk_bpm_run_backend_without_mask", "nmk_bpm_run_backend_with_mask", "nmk_bpm_default_options",
     "nmk_bpm_run", "nmk_bpm_run_noalloc", "nmk_bpm_create", "nmk_bpm_upload_fields", "nmk_bpm_propagate", "nmk_bpm_download",
     "nmk_bpm_destroy", "nmk_bpm_set_tuning", "nmk_bpm_estimate_footprint", "nmk_bpm_interface_weights",
+    "nmk_bpm_interface_weights_device",
     "nmk_bpm_eval_expression", "nmk_bpm_version",
     "nmk_srv_default_config", "nmk_srv_create", "nmk_srv_submit", "nmk_srv_wait", "nmk_srv_cancel",
     "nmk_srv_stats", "nmk_srv_destroy",
@@ -133,6 +134,7 @@ def load():
     lib.nmk_bpm_estimate_footprint.argtypes = [cp, i64, ctypes.c_int, ctypes.POINTER(Options),
                                                ctypes.POINTER(i64), cp, sz]
     lib.nmk_bpm_interface_weights.argtypes = [cp, cp, c_double_p, i64, c_double_p, i64, cp, sz]
+    lib.nmk_bpm_interface_weights_device.argtypes = [cp, cp, c_double_p, i64, c_double_p, i64, cp, sz]
     lib.nmk_bpm_eval_expression.argtypes = [cp, ctypes.c_double, c_double_p, cp, sz]
     lib.nmk_bpm_version.restype = ctypes.c_char_p
     vp = ctypes.c_void_p

@@ -188,6 +188,7 @@ int nmk_bpm_run(const char *json, const char *formula, const double *lc, int64_t
         // (use_graph is honoured by contexts, which replay their graphs)
         o.use_graph = 0;
         auto r = make_runner(p, o);
+        if (!r->notice().empty()) warning = r->notice();
         if (p.beam == BeamKind::None) r->upload_fields(reinterpret_cast<const c64 *>(E));
         Stats st;
         r->propagate(&st);
@@ -250,6 +251,7 @@ nmk_bpm_ctx *nmk_bpm_create(const char *json, const char *formula, const double 
         std::unique_ptr<nmk_bpm_ctx> c(new nmk_bpm_ctx);
         c->problem = problem_from(json, formula, lc, n_lc, mask, n_mask, n_E, &warning);
         c->runner = make_runner(c->problem, to_options(opts));
+        if (!c->runner->notice().empty()) warning = c->runner->notice();
         c->problem.lc = nullptr;  // the director now lives in HBM
         ctx = c.release();
     }, &warning);
@@ -310,14 +312,16 @@ int nmk_bpm_estimate_footprint(const char *json, int64_t n_lc, int has_mask, con
         if (has_mask) {
             // the mask and (upper bound) the interface weights of a mask formula
             auto al = [](int64_t v) { return (v + (int64_t)kAllocAlign - 1) / (int64_t)kAllocAlign * (int64_t)kAllocAlign; };
-            b += al(nv) + al(2 * (int64_t)p.Nx * p.Ny * std::max(0, p.Nz - 1) * 8);
+            b += al(nv) + al(2 * (int64_t)p.Nx * p.Ny * std::max(0, p.Nz - 1) * 8) + al(sizeof(int));
         }
         if (bytes) *bytes = b;
     });
 }
 
-int nmk_bpm_interface_weights(const char *json, const char *formula, const double *mask, int64_t n_mask,
-                              double *weights, int64_t n_weights, char *err, size_t errlen) {
+namespace {
+
+int interface_weights_call(const char *json, const char *formula, const double *mask, int64_t n_mask,
+                           double *weights, int64_t n_weights, char *err, size_t errlen, bool device) {
     std::string warning;
     return guarded(err, errlen, [&] {
         if (!json) throw BpmError("null settings string");
@@ -339,13 +343,32 @@ int nmk_bpm_interface_weights(const char *json, const char *formula, const doubl
         for (int64_t i = 0; i < nv; i++) p.mask[i] = mask[i] >= 0 ? 1 : 0;
         std::vector<double> w;
         try {
-            interface_weights(p, formula, w, &warning);
+            if (device) {
+                const Expr e(formula, {"x", "y", "z"});
+                p.weights_on_device = e.compile(p.weight_prog);
+                if (!p.weights_on_device) throw BpmError("the mask formula is too long for a device program");
+                device_interface_weights(p, w, &warning);
+            } else {
+                interface_weights(p, formula, w, &warning);
+            }
         } catch (const ExprError &e) {
             throw BpmError(std::string("Wrong mask formula: ") + e.what());
         }
         if (w.empty()) std::fill(weights, weights + nw, 0.5);
         else std::copy(w.begin(), w.end(), weights);
     }, &warning);
+}
+
+}  // namespace
+
+int nmk_bpm_interface_weights(const char *json, const char *formula, const double *mask, int64_t n_mask,
+                              double *weights, int64_t n_weights, char *err, size_t errlen) {
+    return interface_weights_call(json, formula, mask, n_mask, weights, n_weights, err, errlen, false);
+}
+
+int nmk_bpm_interface_weights_device(const char *json, const char *formula, const double *mask, int64_t n_mask,
+                                     double *weights, int64_t n_weights, char *err, size_t errlen) {
+    return interface_weights_call(json, formula, mask, n_mask, weights, n_weights, err, errlen, true);
 }
 
 int nmk_bpm_eval_expression(const char *expr, double lambda, double *value, char *err, size_t errlen) {

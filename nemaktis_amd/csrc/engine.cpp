@@ -135,7 +135,7 @@ public:
         }
         sz_ = sizes_of(p, o);
         bind_source(o.allocator, bulk_, bt_, lc_, weights_, q_, iso_t_, iso_n_, partials_, mask_, twx_, twy_, E_, sp_, pp_, cf_,
-                    xt_, fb_[0], fb_[1], fb_[2], fb_[3]);
+                    xt_, fb_[0], fb_[1], fb_[2], fb_[3], flag_);
         g_.Nx = p.Nx;
         g_.Ny = p.Ny;
         g_.Nz = p.Nz;
@@ -156,7 +156,13 @@ public:
         if (dev_in && p.bulk) throw BpmError("bulk output needs host arrays (its planes are written to a file)");
         if (!dev_in) lc_.alloc((size_t)p.voxels() * p.field_dim);
         if (dev_in ? o.dev.mask != nullptr : !p.mask.empty()) mask_.alloc((size_t)p.voxels());
-        if (!dev_in && !p.weights.empty()) weights_.alloc(p.weights.size());
+        const bool dev_weights = !dev_in && p.weights_on_device && !p.mask.empty();
+        if (dev_weights) {
+            weights_.alloc(2 * (size_t)sz_.Nfull * (size_t)(p.Nz - 1));
+            flag_.alloc(1);
+        } else if (!dev_in && !p.weights.empty()) {
+            weights_.alloc(p.weights.size());
+        }
         q_.alloc(2 * (size_t)sz_.nq);
         iso_t_.alloc(std::max<size_t>(1, p.iso_thick.size()));
         iso_n_.alloc(std::max<size_t>(1, p.iso_index.size()));
@@ -187,8 +193,15 @@ public:
             if (dev_in) launch_mask_flags(o.dev.mask, mask_.ptr, p.voxels(), st_);
             else HIPCHK(hipMemcpyAsync(mask_.ptr, p.mask.data(), mask_.bytes(), hipMemcpyHostToDevice, st_));
         }
-        if (weights_.ptr)
+        if (dev_weights) {
+            // the interface weights of the mask formula, from the LC flags just uploaded
+            HIPCHK(hipMemsetAsync(flag_.ptr, 0, sizeof(int), st_));
+            launch_interface_weights(p.weight_prog, mask_.ptr, p.Nx, p.Ny, p.Nz, p.dx, p.dy, p.dz, weights_.ptr,
+                                     flag_.ptr, st_);
+            HIPCHK(hipMemcpyAsync(&mismatch_, flag_.ptr, sizeof(int), hipMemcpyDeviceToHost, st_));
+        } else if (weights_.ptr) {
             HIPCHK(hipMemcpyAsync(weights_.ptr, p.weights.data(), weights_.bytes(), hipMemcpyHostToDevice, st_));
+        }
         hq_.clear();
         for (int i = 0; i < sz_.nq; i++) {
             hq_.push_back(p.qx[i]);
@@ -211,6 +224,11 @@ public:
         twiddles(p.Nx, twx_, htwx_);
         twiddles(p.Ny, twy_, htwy_);
         HIPCHK(hipStreamSynchronize(st_));
+        if (dev_weights && mismatch_) {
+            // the reference prints the notice and keeps every weight at 0.5
+            w_p_ = nullptr;
+            notice_ = kMaskMismatchNotice;
+        }
     }
 
     ~RunnerT() override {
@@ -230,13 +248,14 @@ public:
     }
 
     hipStream_t stream() const override { return st_; }
+    const std::string &notice() const override { return notice_; }
     const c64 *device_output() const override { return E_p_; }
 
     int64_t footprint() const override {
         size_t b = lc_.footprint() + weights_.footprint() + q_.footprint() + iso_t_.footprint() + iso_n_.footprint() +
                    partials_.footprint() + mask_.footprint() + twx_.footprint() + twy_.footprint() +
                    E_.footprint() + sp_.footprint() + pp_.footprint() + cf_.footprint() + xt_.footprint() +
-                   bulk_.footprint() + bt_.footprint();
+                   bulk_.footprint() + bt_.footprint() + flag_.footprint();
         for (auto &f : fb_) b += f.footprint();
         return (int64_t)b;
     }
@@ -331,6 +350,9 @@ private:
     FftPlan plan_x_, plan_y_;
     DevBuf<double> lc_, weights_, q_, iso_t_, iso_n_, partials_;
     DevBuf<uint8_t> mask_;
+    DevBuf<int> flag_;      // mismatch flag of the device interface weights
+    int mismatch_ = 0;
+    std::string notice_;
     DevBuf<c64> twx_, twy_, E_;
     std::vector<c64> htwx_, htwy_;
     DevBuf<SlotParams> sp_;
@@ -769,7 +791,9 @@ int64_t estimate_footprint(const Problem &p, const Options &o) {
         if (o.dev.mask) add(p.voxels());
     } else {
         add(p.mask_src ? p.voxels() : (int64_t)p.mask.size());
-        add((p.mask_src && !p.mask_formula.empty() ? nw : (int64_t)p.weights.size()) * 8);
+        const bool formula = p.mask_src ? !p.mask_formula.empty() : p.weights_on_device && !p.mask.empty();
+        add((formula ? nw : (int64_t)p.weights.size()) * 8);
+        if (formula) add(sizeof(int));  // mismatch flag of the device weights
     }
     add((int64_t)s.nq * 2 * 8);
     add((int64_t)std::max<size_t>(1, p.iso_thick.size()) * 8);
@@ -792,6 +816,30 @@ int64_t estimate_footprint(const Problem &p, const Options &o) {
         add((int64_t)s.nprops_max * 2 * s.NP * fsz);
     }
     return b;
+}
+
+void device_interface_weights(const Problem &p, std::vector<double> &w, std::string *warning) {
+    if (!p.weights_on_device || p.mask.size() != (size_t)p.voxels())
+        throw BpmError("device interface weights need a compiled mask formula and the LC flags");
+    const size_t nw = 2 * (size_t)p.Nx * p.Ny * (size_t)std::max(0, p.Nz - 1);
+    w.assign(nw, 0.5);
+    if (!nw) return;
+    DevBuf<uint8_t> mask;
+    DevBuf<double> dw;
+    DevBuf<int> flag;
+    mask.alloc(p.mask.size());
+    dw.alloc(nw);
+    flag.alloc(1);
+    int mismatch = 0;
+    HIPCHK(hipMemcpy(mask.ptr, p.mask.data(), mask.bytes(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(flag.ptr, 0, sizeof(int)));
+    launch_interface_weights(p.weight_prog, mask.ptr, p.Nx, p.Ny, p.Nz, p.dx, p.dy, p.dz, dw.ptr, flag.ptr, nullptr);
+    HIPCHK(hipMemcpy(w.data(), dw.ptr, dw.bytes(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&mismatch, flag.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    if (mismatch) {
+        if (warning) *warning = kMaskMismatchNotice;
+        w.clear();
+    }
 }
 
 std::unique_ptr<Runner> make_runner(const Problem &p, const Options &o, hipStream_t stream) {

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "device_common.h"
 #include "settings.h"
@@ -90,6 +91,9 @@ public:
     // device cannot run them)
     virtual void retune(const Tuning &t) = 0;
     virtual hipStream_t stream() const = 0;
+    // notice the run produced while preparing its inputs (the reference's
+    // printed messages that do not stop it), empty if none
+    virtual const std::string &notice() const = 0;
 };
 
 std::unique_ptr<Runner> make_runner(const Problem &p, const Options &o, hipStream_t stream = nullptr);
@@ -98,5 +102,10 @@ std::unique_ptr<Runner> make_runner(const Problem &p, const Options &o, hipStrea
 // buffer rounded up to kAllocAlign); used for serving admission
 constexpr size_t kAllocAlign = 256;
 int64_t estimate_footprint(const Problem &p, const Options &o);
+
+// interface weights of p's compiled mask formula (p.weights_on_device) and
+// LC flags (p.mask) computed on the current device, downloaded into w (left
+// empty, with the reference's notice in *warning, on a mismatch)
+void device_interface_weights(const Problem &p, std::vector<double> &w, std::string *warning);
 
 }  // namespace nmk

@@ -27,7 +27,7 @@ struct Expr::Node {
                 if (op == "-") return a - b;
                 if (op == "*") return a * b;
                 if (op == "/") return a / b;
-                if (op == "^") return std::pow(a, b);
+                if (op == "^") return kids[1]->kind == Num && kids[1]->value == 2.0 ? a * a : std::pow(a, b);
                 if (op == "<") return a < b;
                 if (op == ">") return a > b;
                 if (op == "<=") return a <= b;
@@ -291,6 +291,77 @@ Expr::Expr(Expr &&) noexcept = default;
 Expr &Expr::operator=(Expr &&) noexcept = default;
 
 double Expr::eval(const double *values) const { return root_->eval(values); }
+
+namespace {
+
+int fun1_code(const std::string &f) {
+    static const char *names[] = {"sin", "cos", "tan", "asin", "acos", "atan", "sinh", "cosh", "tanh", "asinh",
+                                  "acosh", "atanh", "log2", "log10", "log", "exp", "sqrt", "abs", "sign", "rint"};
+    for (int i = 0; i < int(sizeof(names) / sizeof(names[0])); i++)
+        if (f == names[i]) return i;
+    if (f == "ln") return EF_LOG;
+    return -1;
+}
+
+int bin_code(const std::string &op) {
+    static const struct {
+        const char *op;
+        int code;
+    } t[] = {{"+", EX_ADD}, {"-", EX_SUB}, {"*", EX_MUL}, {"/", EX_DIV}, {"^", EX_POW}, {"<", EX_LT},
+             {">", EX_GT}, {"<=", EX_LE}, {">=", EX_GE}, {"==", EX_EQ}, {"!=", EX_NE}, {"&&", EX_AND},
+             {"||", EX_OR}};
+    for (const auto &e : t)
+        if (op == e.op) return e.code;
+    return -1;
+}
+
+// appends the postfix form of n; depth: stack slots in use before n
+bool emit(const Expr::Node &n, ExprProgram &p, int depth) {
+    auto push = [&](int code, int arg, double value, int new_depth) {
+        if (p.n >= kExprMaxOps || new_depth > kExprMaxStack) return false;
+        p.ops[p.n++] = ExprOp{code, arg, value};
+        if (new_depth > p.depth) p.depth = new_depth;
+        return true;
+    };
+    using K = Expr::Node;
+    switch (n.kind) {
+        case K::Num: return push(EX_CONST, 0, n.value, depth + 1);
+        case K::Var: return push(EX_VAR, n.var, 0, depth + 1);
+        case K::Neg: return emit(*n.kids[0], p, depth) && push(EX_NEG, 0, 0, depth + 1);
+        case K::Tern:
+            return emit(*n.kids[0], p, depth) && emit(*n.kids[1], p, depth + 1) && emit(*n.kids[2], p, depth + 2) &&
+                   push(EX_TERN, 0, 0, depth + 1);
+        case K::Bin: {
+            const int code = bin_code(n.op);
+            if (code < 0) return false;
+            if (code == EX_POW && n.kids[1]->kind == K::Num && n.kids[1]->value == 2.0)
+                return emit(*n.kids[0], p, depth) && push(EX_SQR, 0, 0, depth + 1);
+            return emit(*n.kids[0], p, depth) && emit(*n.kids[1], p, depth + 1) && push(code, 0, 0, depth + 1);
+        }
+        case K::Fun: {
+            const int na = int(n.kids.size());
+            for (int i = 0; i < na; i++)
+                if (!emit(*n.kids[i], p, depth + i)) return false;
+            if (n.op == "atan2") return push(EX_ATAN2, 0, 0, depth + 1);
+            if (n.op == "sum") return push(EX_SUM, na, 0, depth + 1);
+            if (n.op == "avg") return push(EX_AVG, na, 0, depth + 1);
+            if (n.op == "min") return push(EX_MIN, na, 0, depth + 1);
+            if (n.op == "max") return push(EX_MAX, na, 0, depth + 1);
+            const int f = fun1_code(n.op);
+            return f >= 0 && na == 1 && push(EX_FUN1, f, 0, depth + 1);
+        }
+    }
+    return false;
+}
+
+}  // namespace
+
+bool Expr::compile(ExprProgram &out) const {
+    ExprProgram p;
+    if (!emit(*root_, p, 0)) return false;
+    out = p;
+    return true;
+}
 
 double eval_index_expression(const std::string &text, double lambda) {
     Expr e(text, {"lambda"});

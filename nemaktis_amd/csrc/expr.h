@@ -14,6 +14,8 @@
 #include <string>
 #include <vector>
 
+#include "expr_program.h"
+
 namespace nmk {
 
 struct ExprError : std::runtime_error {
@@ -31,6 +33,10 @@ public:
 
     double eval(const double *values) const;
     double eval() const { return eval(nullptr); }
+
+    // the postfix program of the expression (expr_program.h); false when it
+    // needs more than kExprMaxOps operations or kExprMaxStack stack slots
+    bool compile(ExprProgram &out) const;
 
     struct Node;
 

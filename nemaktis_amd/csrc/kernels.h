@@ -4,12 +4,17 @@
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
+#include "expr_program.h"
 
 namespace nmk {
 
 // LC flags of the voxels from a float64 mask on the device (mask >= 0 is LC,
 // UtilitiesLib/VectorField.cpp:61-62)
 void launch_mask_flags(const double *mask, uint8_t *flags, int64_t n, hipStream_t st);
+// interface weights of a compiled mask formula: w (2*Nx*Ny*(Nz-1) values),
+// *mismatch |= 1 when the formula's crossings disagree with the LC flags
+void launch_interface_weights(const ExprProgram &prog, const uint8_t *mask, int Nx, int Ny, int Nz, double dx,
+                              double dy, double dz, double *w, int *mismatch, hipStream_t st);
 
 // sum of n_eff over every voxel of each slot's wavelength -> sp[slot].nref
 void launch_nref(const double *lc, const uint8_t *mask, const Geo &g, SlotParams *sp, int nslots, double *partials,

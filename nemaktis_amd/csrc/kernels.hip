@@ -91,6 +91,69 @@ void launch_mask_flags(const double *mask, uint8_t *flags, int64_t n, hipStream_
     NMK_LAUNCH_CHECK();
 }
 
+// Interface weights of a mask formula (UtilitiesLib/VectorField.cpp:64-113),
+// one thread per column (ix, iy): the formula at every plane (each value
+// computed once and carried to the next slab) and, for each slab where it
+// changes sign or vanishes, the crossing bracketed by bisection until the
+// bracket is narrower than 1e-3 dz; w[2i], w[2i+1] of voxel i = ix + Nx (iy +
+// Ny iz) become ((z - z_iz) / dz, 1 - that), the other slabs keep 0.5.  A
+// crossing between two planes of equal mask value sets *mismatch (the caller
+// then drops every weight, as the reference does).  The arithmetic is the
+// host version's (settings.cpp), operation for operation.
+__global__ __launch_bounds__(256) void k_interface_weights(ExprProgram prog, const uint8_t *__restrict__ mask, int Nx,
+                                                           int Ny, int Nz, double dx, double dy, double dz,
+                                                           double *__restrict__ w, int *__restrict__ mismatch) {
+#pragma clang fp contract(off)
+    const int64_t N = (int64_t)Nx * Ny;
+    const int64_t iperp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (iperp >= N) return;
+    const int ix = int(iperp % Nx), iy = int(iperp / Nx);
+    double v[3] = {(ix - 0.5 * (Nx - 1)) * dx, (iy - 0.5 * (Ny - 1)) * dy, 0};
+    auto f = [&](double z) {
+        v[2] = z;
+        return expr_run(prog, v);
+    };
+    bool failed = false;
+    double fnext = f((0 - 0.5 * (Nz - 1)) * dz);
+    for (int iz = 0; iz < Nz - 1; iz++) {
+        const int64_t i = iperp + N * iz;
+        const double zi = (iz - 0.5 * (Nz - 1)) * dz;
+        double zm = zi, zp = (iz + 1 - 0.5 * (Nz - 1)) * dz, z = zm;
+        double fm = fnext;
+        const double fp = f(zp);
+        fnext = fp;
+        double w0 = 0.5, w1 = 0.5;
+        if (fm * fp <= 0) {
+            const bool m0 = mask ? mask[i] != 0 : true, m1 = mask ? mask[i + N] != 0 : true;
+            if (m0 == m1) failed = true;
+            // each pass halves the bracket: about 10 passes
+            for (int it = 0; it < 64 && fabs(zp - zm) >= 1e-3 * dz; it++) {
+                z = zm;
+                fm = f(z);
+                z = 0.5 * (zp + zm);
+                const double fz = f(z);
+                if (fz == 0) break;
+                if (fz * fm < 0) zp = z;
+                else zm = z;
+            }
+            w0 = (z - zi) / dz;
+            w1 = 1 - w0;
+        }
+        w[2 * i] = w0;
+        w[2 * i + 1] = w1;
+    }
+    if (failed) atomicOr(mismatch, 1);
+}
+
+void launch_interface_weights(const ExprProgram &prog, const uint8_t *mask, int Nx, int Ny, int Nz, double dx,
+                              double dy, double dz, double *w, int *mismatch, hipStream_t st) {
+    const int64_t N = (int64_t)Nx * Ny;
+    if (N <= 0 || Nz < 2) return;
+    hipLaunchKernelGGL(k_interface_weights, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, prog, mask, Nx, Ny,
+                       Nz, dx, dy, dz, w, mismatch);
+    NMK_LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------------------
 // nref = mean n_eff over every voxel (PermittivityTensorField.cpp:94-98);
 // fixed block ranges and fixed reduction trees make it deterministic

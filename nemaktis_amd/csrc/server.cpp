@@ -191,6 +191,7 @@ void Server::run_job(Job &job, hipStream_t st) {
     Options o = job.opt;
     o.allocator = &slice;
     std::unique_ptr<Runner> r = make_runner(job.problem, o, st);
+    if (notice.empty()) notice = r->notice();
     // inputs are on the device now; the host copies are no longer needed
     job.problem.lc = nullptr;
     std::vector<uint8_t>().swap(job.problem.mask);

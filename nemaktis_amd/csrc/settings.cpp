@@ -168,6 +168,8 @@ void interface_weights(const Problem &p, const std::string &formula, std::vector
     const int64_t N = (int64_t)Nx * Ny;
     w.assign(2 * N * (int64_t)(Nz - 1), 0.5);
     const Expr e(formula, {"x", "y", "z"});
+    ExprProgram prog;
+    const bool compiled = e.compile(prog);
     auto columns = [&](int64_t beg, int64_t end) {
         bool failed = false;
         std::vector<double> fplane((size_t)Nz);  // the formula on the planes of one column
@@ -176,7 +178,7 @@ void interface_weights(const Problem &p, const std::string &formula, std::vector
             double v[3] = {(ix - 0.5 * (Nx - 1)) * p.dx, (iy - 0.5 * (Ny - 1)) * p.dy, 0};
             auto f = [&](double z) {
                 v[2] = z;
-                return e.eval(v);
+                return compiled ? expr_run(prog, v) : e.eval(v);
             };
             // each plane bounds two slabs: evaluated once (the same values the
             // slab-by-slab evaluation gives)
@@ -230,20 +232,36 @@ void interface_weights(const Problem &p, const std::string &formula, std::vector
     }
     if (failed) {
         // the reference prints this and falls back to the default weights
-        if (warning)
-            *warning = "Mask values do not corresponds to the given formula, will ignore interpolation correction";
+        if (warning) *warning = kMaskMismatchNotice;
         w.clear();
     }
 }
+
+const char *const kMaskMismatchNotice =
+    "Mask values do not corresponds to the given formula, will ignore interpolation correction";
 
 void prepare_mask(Problem &p, std::string *warning) {
     if (!p.mask_src) return;
     const int64_t nv = p.voxels();
     p.mask.resize((size_t)nv);
-    for (int64_t i = 0; i < nv; i++) p.mask[i] = p.mask_src[i] >= 0 ? 1 : 0;
+    const double *src = p.mask_src;
+    uint8_t *dst = p.mask.data();
+    auto flags = [src, dst](int64_t beg, int64_t end) {
+        for (int64_t i = beg; i < end; i++) dst[i] = src[i] >= 0 ? 1 : 0;
+    };
+    const int64_t nthreads = std::max<int64_t>(1, std::min<int64_t>(std::thread::hardware_concurrency(), nv >> 22));
+    if (nthreads == 1) {
+        flags(0, nv);
+    } else {
+        std::vector<std::thread> pool;
+        for (int64_t t = 0; t < nthreads; t++) pool.emplace_back(flags, nv * t / nthreads, nv * (t + 1) / nthreads);
+        for (auto &th : pool) th.join();
+    }
     if (!p.mask_formula.empty()) {
         try {
-            interface_weights(p, p.mask_formula, p.weights, warning);
+            const Expr e(p.mask_formula, {"x", "y", "z"});
+            p.weights_on_device = e.compile(p.weight_prog);
+            if (!p.weights_on_device) interface_weights(p, p.mask_formula, p.weights, warning);
         } catch (const ExprError &e) {
             throw BpmError(std::string("Wrong mask formula: ") + e.what());
         }

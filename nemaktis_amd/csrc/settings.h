@@ -14,6 +14,8 @@
 #include <utility>
 #include <vector>
 
+#include "expr_program.h"
+
 namespace nmk {
 
 // error raised for any invalid input; the message mirrors the reference's
@@ -71,6 +73,10 @@ struct Problem {
     // prepare_mask
     const double *mask_src = nullptr;
     std::string mask_formula;
+    // the interface weights of the mask formula are left to the device engine
+    // (weights stays empty): the compiled formula
+    bool weights_on_device = false;
+    ExprProgram weight_prog;
     int64_t voxels() const { return (int64_t)Nx * Ny * Nz; }
     int64_t n_out_values() const { return (int64_t)wl.size() * (int64_t)qx.size() * 4 * Nx * Ny; }
 };
@@ -84,8 +90,14 @@ struct Problem {
 // later, off the submitting thread.
 Problem build_problem(const Settings &s, const double *lc, int64_t n_lc, const double *mask, int64_t n_mask,
                       const char *mask_formula, int64_t n_E, std::string *warning, bool defer_mask = false);
-// builds the deferred mask and interface weights of a problem (no-op without)
+// builds the deferred LC flags of a problem (no-op without) and compiles its
+// mask formula for the device; a formula too long for a device program has
+// its interface weights computed here on host threads
 void prepare_mask(Problem &p, std::string *warning);
+
+// the notice the reference prints when the mask formula's crossings disagree
+// with the mask (UtilitiesLib/VectorField.cpp:107-111)
+extern const char *const kMaskMismatchNotice;
 
 // Interface interpolation weights of a mask formula (UtilitiesLib/
 // VectorField.cpp:64-113) for the mesh and mask of p: w[2*i], w[2*i+1] for

@@ -239,6 +239,39 @@ def test_backend_interface_weights_mismatch_notice():
         _host_weights(s, "0.05-", np.ones(dims[::-1]))
 
 
+GRAMMAR_FORMULA = "(z>0.05 ? 0.6 : 0.7)^2-x^2-y^2-z^2+0*min(x,y)+0*avg(x,y,z)+0*abs(sqrt(x^2+1))"
+
+
+def _grammar_fn(x, y, z):
+    return (np.where(z > 0.05, 0.6, 0.7) ** 2 - x ** 2 - y ** 2 - z ** 2 + 0 * np.minimum(x, y) + 0 * ((x + y + z) / 3)
+            + 0 * np.abs(np.sqrt(x ** 2 + 1)))
+
+
+def test_host_weights_compiled_and_tree_evaluations_agree():
+    """The mask formula is evaluated by its compiled postfix program (the one
+    the device runs); a formula too long for a program falls back to the
+    expression tree.  Both give the oracle's weights: a formula with a
+    ternary, comparisons and functions (exact operations only, so equal to
+    the bit), and a plane written short (compiled) and padded with 40 zero
+    terms (over the program's size: tree)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mask_weights
+    dims, sp = (14, 11, 9), (0.1, 0.1, 0.1)
+    s = fields.make_settings(dims, sp, [0.6])
+    x, y, z = fields.mesh_coords(*dims, *sp)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    mask = _grammar_fn(X, Y, Z)
+    w, notice = _host_weights(s, GRAMMAR_FORMULA, mask)
+    w_or, bad = mask_weights.interface_weights(_grammar_fn, dims, sp, mask)
+    assert notice == "" and not bad
+    assert np.array_equal(w, w_or)
+    assert np.sum(w[..., 0] != 0.5) > 20
+    plane = 0.13 - Z + 0 * X
+    short, _ = _host_weights(s, "0.13-z", plane)
+    long_, _ = _host_weights(s, "0.13-z" + "+0*x" * 40, plane)
+    assert np.array_equal(short, long_) and np.sum(short[..., 0] != 0.5) == dims[0] * dims[1]
+
+
 # --------------------------------------------------------------------------- the two drop-in C symbols
 def test_drop_in_symbols_validate_before_any_device_work():
     """nmk_bpm_run_backend_without_mask / _with_mask called directly through
