@@ -161,7 +161,7 @@ int nmk_bpm_interface_weights(const char *json_str, const char *mask_formula, co
                               size_t err_len);
 
 /* The same weights computed on the current HIP device, as a masked run
- * computes them (one thread per column, the formula compiled to a postfix
+ * computes them (one thread per column and slab, the formula compiled to a postfix
  * program; arithmetic operation for operation the host version's, elementary
  * functions from the device math library).  No counterpart in the reference,
  * which evaluates the formula on the host; for callers that keep the weights

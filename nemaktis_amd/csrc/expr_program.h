@@ -2,7 +2,7 @@
 // by the same inline interpreter on the host and on the device.  The mask
 // formula of a masked run is evaluated at every plane of every column (the
 // interface weights, UtilitiesLib/VectorField.cpp:64-113); on the device one
-// thread takes a column.
+// thread takes one slab of one column.
 //
 // Arithmetic is IEEE double with one rounding per operation (no contraction
 // across operations: every result goes through the evaluation stack), so
@@ -65,18 +65,36 @@ __host__ __device__ inline double expr_fun1(int f, double a) {
     return 0;
 }
 
-// value of the program for the variable values v (in the order the
-// expression was compiled with)
-__host__ __device__ inline double expr_run(const ExprProgram &p, const double *v) {
-    double st[kExprMaxStack];
+// evaluation stack of one evaluation: a local array, or (on the device) a
+// thread's column of a block-wide LDS array (slot k of thread t at k*stride+t,
+// so that a wave's accesses fall on consecutive banks)
+struct ExprArrayStack {
+    double *s;
+    __host__ __device__ double &operator[](int k) const { return s[k]; }
+};
+struct ExprStridedStack {
+    double *s;
+    int stride;
+    __host__ __device__ double &operator[](int k) const { return s[k * stride]; }
+};
+
+// value of the program for the variables (x0, x1, x2) (in the order the
+// expression was compiled with; unused ones are ignored); st holds at least
+// p.depth slots
+template <class Stack>
+__host__ __device__ inline double expr_eval(const ExprProgram &p, double x0, double x1, double x2, Stack st) {
     int sp = 0;
     for (int i = 0; i < p.n; i++) {
         const ExprOp op = p.ops[i];
         switch (op.code) {
             case EX_CONST: st[sp++] = op.value; break;
-            case EX_VAR: st[sp++] = v[op.arg]; break;
+            case EX_VAR: st[sp++] = op.arg == 0 ? x0 : op.arg == 1 ? x1 : x2; break;
             case EX_NEG: st[sp - 1] = -st[sp - 1]; break;
-            case EX_SQR: st[sp - 1] = st[sp - 1] * st[sp - 1]; break;
+            case EX_SQR: {
+                const double a = st[sp - 1];
+                st[sp - 1] = a * a;
+                break;
+            }
             case EX_FUN1: st[sp - 1] = expr_fun1(op.arg, st[sp - 1]); break;
             case EX_TERN: {
                 const double c = st[sp - 3], a = st[sp - 2], b = st[sp - 1];
@@ -97,7 +115,10 @@ __host__ __device__ inline double expr_run(const ExprProgram &p, const double *v
             case EX_MAX: {
                 const int n = op.arg;
                 double m = st[sp - n];
-                for (int k = sp - n; k < sp; k++) m = op.code == EX_MIN ? (st[k] < m ? st[k] : m) : (st[k] > m ? st[k] : m);
+                for (int k = sp - n; k < sp; k++) {
+                    const double x = st[k];
+                    m = op.code == EX_MIN ? (x < m ? x : m) : (x > m ? x : m);
+                }
                 sp -= n;
                 st[sp++] = m;
                 break;
@@ -126,6 +147,12 @@ __host__ __device__ inline double expr_run(const ExprProgram &p, const double *v
         }
     }
     return sp > 0 ? st[0] : 0;
+}
+
+// host convenience: the three variables from v[0..2], a local stack
+inline double expr_run(const ExprProgram &p, const double *v) {
+    double s[kExprMaxStack];
+    return expr_eval(p, v[0], v[1], v[2], ExprArrayStack{s});
 }
 
 }  // namespace nmk

@@ -92,36 +92,32 @@ void launch_mask_flags(const double *mask, uint8_t *flags, int64_t n, hipStream_
 }
 
 // Interface weights of a mask formula (UtilitiesLib/VectorField.cpp:64-113),
-// one thread per column (ix, iy): the formula at every plane (each value
-// computed once and carried to the next slab) and, for each slab where it
-// changes sign or vanishes, the crossing bracketed by bisection until the
-// bracket is narrower than 1e-3 dz; w[2i], w[2i+1] of voxel i = ix + Nx (iy +
-// Ny iz) become ((z - z_iz) / dz, 1 - that), the other slabs keep 0.5.  A
-// crossing between two planes of equal mask value sets *mismatch (the caller
-// then drops every weight, as the reference does).  The arithmetic is the
-// host version's (settings.cpp), operation for operation.
+// one thread per (column, slab): the formula at the slab's two planes and, if
+// it changes sign or vanishes between them, the crossing bracketed by
+// bisection until the bracket is narrower than 1e-3 dz; w[2i], w[2i+1] of
+// voxel i = ix + Nx (iy + Ny iz) become ((z - z_iz) / dz, 1 - that), other
+// slabs keep 0.5.  A crossing between two planes of equal mask value sets
+// *mismatch (the caller then drops every weight, as the reference does).
+// The arithmetic is the host version's (settings.cpp), operation for
+// operation; a plane's value is the same whichever slab evaluates it.
 __global__ __launch_bounds__(256) void k_interface_weights(ExprProgram prog, const uint8_t *__restrict__ mask, int Nx,
                                                            int Ny, int Nz, double dx, double dy, double dz,
                                                            double *__restrict__ w, int *__restrict__ mismatch) {
 #pragma clang fp contract(off)
-    const int64_t N = (int64_t)Nx * Ny;
-    const int64_t iperp = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (iperp >= N) return;
-    const int ix = int(iperp % Nx), iy = int(iperp / Nx);
-    double v[3] = {(ix - 0.5 * (Nx - 1)) * dx, (iy - 0.5 * (Ny - 1)) * dy, 0};
-    auto f = [&](double z) {
-        v[2] = z;
-        return expr_run(prog, v);
-    };
+    extern __shared__ double expr_stack[];  // [prog.depth][blockDim.x]
+    const ExprStridedStack stk{expr_stack + threadIdx.x, (int)blockDim.x};
+    const int64_t N = (int64_t)Nx * Ny, n = N * (Nz - 1);
     bool failed = false;
-    double fnext = f((0 - 0.5 * (Nz - 1)) * dz);
-    for (int iz = 0; iz < Nz - 1; iz++) {
-        const int64_t i = iperp + N * iz;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int iz = int(i / N);
+        const int64_t iperp = i - (int64_t)iz * N;
+        const int iy = int(iperp / Nx), ix = int(iperp - (int64_t)iy * Nx);
+        const double x = (ix - 0.5 * (Nx - 1)) * dx, y = (iy - 0.5 * (Ny - 1)) * dy;
+        auto f = [&](double z) { return expr_eval(prog, x, y, z, stk); };
         const double zi = (iz - 0.5 * (Nz - 1)) * dz;
         double zm = zi, zp = (iz + 1 - 0.5 * (Nz - 1)) * dz, z = zm;
-        double fm = fnext;
+        double fm = f(zm);
         const double fp = f(zp);
-        fnext = fp;
         double w0 = 0.5, w1 = 0.5;
         if (fm * fp <= 0) {
             const bool m0 = mask ? mask[i] != 0 : true, m1 = mask ? mask[i + N] != 0 : true;
@@ -147,10 +143,13 @@ __global__ __launch_bounds__(256) void k_interface_weights(ExprProgram prog, con
 
 void launch_interface_weights(const ExprProgram &prog, const uint8_t *mask, int Nx, int Ny, int Nz, double dx,
                               double dy, double dz, double *w, int *mismatch, hipStream_t st) {
-    const int64_t N = (int64_t)Nx * Ny;
-    if (N <= 0 || Nz < 2) return;
-    hipLaunchKernelGGL(k_interface_weights, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, prog, mask, Nx, Ny,
-                       Nz, dx, dy, dz, w, mismatch);
+    const int64_t n = (int64_t)Nx * Ny * (Nz - 1);
+    if (n <= 0) return;
+    if (prog.depth < 1 || prog.depth > kExprMaxStack) throw std::runtime_error("interface weights: bad formula program");
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 1 << 18);
+    const size_t lds = (size_t)prog.depth * 256 * sizeof(double);
+    hipLaunchKernelGGL(k_interface_weights, dim3(blocks), dim3(256), lds, st, prog, mask, Nx, Ny, Nz, dx, dy, dz, w,
+                       mismatch);
     NMK_LAUNCH_CHECK();
 }
 

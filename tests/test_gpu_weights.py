@@ -1,5 +1,5 @@
 """Interface weights of a mask formula computed on the device (the masked
-runs' path: one thread per column, the formula compiled to a postfix program)
+runs' path: one thread per column and slab, the formula compiled to a postfix program)
 against the host computation (UtilitiesLib/VectorField.cpp:64-113 restated in
 settings.cpp), which the CPU tests pin against the oracle and closed forms.
 
