@@ -368,3 +368,22 @@ def test_lines_longer_than_2048_points_fp32(dims):
     lc = fields.random_smooth(*dims, (0.1, 0.1, 0.1), seed=53, corr_length=0.3)
     err = _rel(_gpu(s, lc, precision=32), oracle.propagate(s, lc))
     assert err < TOL32, err
+
+
+@pytest.mark.parametrize("dims,precision", [((4097, 5, 3), 32), ((5, 4097, 3), 32), ((2049, 5, 3), 64),
+                                            ((5, 2049, 3), 64)])
+def test_maximum_line_lengths(dims, precision):
+    """The longest lines each path takes: 4097 points (4096 unknowns, 256
+    chunks of 16) in FP32 and 2049 (256 chunks of 8) in FP64, along x and
+    along y, against the oracle; one point more is refused when the settings
+    are read."""
+    s = _base(dims, no="1.5", ne="1.7", na_objective=0.5)
+    lc = fields.random_smooth(*dims, (0.1, 0.1, 0.1), seed=59, corr_length=0.3)
+    print("[gpu case] maximum line length", dims, precision, flush=True)
+    err = _rel(_gpu(s, lc, precision=precision), oracle.propagate(s, lc))
+    assert err < (TOL32 if precision == 32 else TOL64), err
+    big = tuple(d + 1 if d > 5 else d for d in dims)
+    s2 = _base(big, no="1.5", ne="1.7", na_objective=0.5)
+    lc2 = fields.random_smooth(*big, (0.1, 0.1, 0.1), seed=59, corr_length=0.3)
+    with pytest.raises(bpm_backend.BpmBackendError, match=str(max(dims))):
+        _gpu(s2, lc2, precision=precision)

@@ -175,3 +175,20 @@ def test_job_failing_after_its_march_leaves_its_neighbour_intact(tmp_path):
     assert ig["state"] == "done" and ig2["state"] == "done"
     ref = bpm_backend.propagate(good[0], good[1], precision=32, wl_batch=0)
     assert np.array_equal(jg.E, ref) and np.array_equal(jg2.E, ref)
+
+
+def test_mask_formula_mismatch_notice_in_the_job_record():
+    """A job whose mask formula disagrees with its mask: the reference's notice
+    (UtilitiesLib/VectorField.cpp:107-111, found by the device weights) in the
+    job record, and the default weights (the same output as no formula)."""
+    dims, sp = (48, 40, 8), (0.2, 0.2, 0.2)
+    lc, mask, _ = fields.droplet(*dims, sp, 2.5)
+    s = json.dumps(fields.make_settings(dims, sp, [0.6], nhost="1.51", na_objective=0.4))
+    with serving.Server(max_active=2) as srv:
+        print("[serving] mismatching mask formula", flush=True)
+        j = srv.submit(s, lc, mask_vals=mask, mask_formula="0.05-z", precision=32)
+        info = j.wait()
+    assert info["state"] == "done", info
+    assert info["notice"] == "Mask values do not corresponds to the given formula, will ignore interpolation correction"
+    ref = bpm_backend.propagate(s, lc, mask_vals=mask, mask_formula="", precision=32, wl_batch=0)
+    assert np.array_equal(j.E, ref)
