@@ -334,7 +334,6 @@ int interface_weights_call(const char *json, const char *formula, const double *
         p.dx = s.dx;
         p.dy = s.dy;
         p.dz = s.dz;
-        if (p.Nz < 2) throw BpmError("The mesh needs at least two planes along z");
         const int64_t nv = p.voxels();
         if (n_mask != nv) throw BpmError("Wrong dimension for the mask pointer array");
         const int64_t nw = 2 * (int64_t)p.Nx * p.Ny * (p.Nz - 1);

@@ -520,7 +520,7 @@ private:
 
     // the z-march over all slabs; returns the buffer holding the final state
     C2<T> *march(int nslots, int nprops) {
-        if (o_.use_graph && !o_.profile) {
+        if (o_.use_graph && !o_.profile && p_.Nz > 1) {
             auto key = std::make_pair(nslots, nprops);
             auto it = graphs_.find(key);
             if (it == graphs_.end()) {
@@ -582,6 +582,16 @@ private:
             });
             return overlap ? record(cs) : (hipEvent_t) nullptr;
         };
+        if (nsl == 0) {
+            // a single plane: no slab to march, the entrance fields go to the
+            // output as they are (BPMIteration.cpp:127 runs its loop Nz-1 times)
+            if (bulk_.ptr) {
+                const int beam = p_.beam == BeamKind::None ? 2 : p_.beam == BeamKind::Gaussian ? 1 : 0;
+                launch_bulk_entrance(lc_p_, mask_p_, g_, sp_.ptr, pp_.ptr, nprops, beam, p_.waist, p_.dx, p_.dy,
+                                     E_p_, sz_.nq, bulk_.ptr, st_);
+            }
+            return fb_[0].ptr;
+        }
         if (overlap) HIPCHK(hipStreamWaitEvent(cs, record(st_), 0));
         std::vector<hipEvent_t> ready(nsl, nullptr);
         ready[0] = coef(0);
@@ -712,7 +722,13 @@ private:
     void output_batch(int w0, int nslots, int nprops, const C2<T> *state) {
         c64 *Eb = E_p_ + (size_t)w0 * sz_.nq * 4 * sz_.Nfull;
         timed(KF_OUTPUT, [&] {
-            launch_output<T>(state, lc_p_, mask_p_, g_, sp_.ptr, pp_.ptr, nprops, sz_.nq, Eb, st_);
+            if (p_.Nz > 1) {
+                launch_output<T>(state, lc_p_, mask_p_, g_, sp_.ptr, pp_.ptr, nprops, sz_.nq, Eb, st_);
+            } else {
+                const int beam = p_.beam == BeamKind::None ? 2 : p_.beam == BeamKind::Gaussian ? 1 : 0;
+                launch_output_entrance(lc_p_, mask_p_, g_, sp_.ptr, pp_.ptr, nprops, beam, p_.waist, p_.dx, p_.dy, Eb,
+                                       st_);
+            }
         });
         timed(KF_FILTER, [&] {
             const int planes = nslots * sz_.nq * 4;

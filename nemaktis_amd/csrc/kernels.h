@@ -88,6 +88,12 @@ template <class T>
 void launch_output(const C2<T> *F, const double *lc, const uint8_t *mask, const Geo &g, const SlotParams *sp,
                    const PropParams *pp, int nprops, int nq, c64 *E, hipStream_t st);
 
+// output of a mesh of one plane: each full-grid point's entrance field (beam 0
+// uniform, 1 Gaussian, 2 the caller's fields already in E) through the output
+// interface, in place in E
+void launch_output_entrance(const double *lc, const uint8_t *mask, const Geo &g, const SlotParams *sp,
+                            const PropParams *pp, int nprops, int beam, double waist, double dx, double dy, c64 *E,
+                            hipStream_t st);
 // plane 0 of the bulk output on the full grid: beam through the input
 // interface (beam 0 uniform, 1 Gaussian) or the caller's fields Ein (2)
 void launch_bulk_entrance(const double *lc, const uint8_t *mask, const Geo &g, const SlotParams *sp,

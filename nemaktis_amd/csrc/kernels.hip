@@ -2127,6 +2127,52 @@ void launch_output(const C2<T> *F, const double *lc, const uint8_t *mask, const 
     NMK_LAUNCH_CHECK();
 }
 
+// Output of a mesh of one plane (no slab marched, BPMIteration.cpp:127): every
+// full-grid point, duplicate column and row included, keeps its own entrance
+// field (the beam through the input interface with that point's
+// permittivity, or for beam 2 the caller's fields, read in place from E) and
+// goes through the output interface (FresnelOperator.cpp:45-69).
+__global__ __launch_bounds__(256) void k_output_entrance(const double *__restrict__ lc,
+                                                         const uint8_t *__restrict__ mask, Geo g,
+                                                         const SlotParams *__restrict__ sp,
+                                                         const PropParams *__restrict__ pp, int beam, double waist,
+                                                         double dx, double dy, c64 *__restrict__ E) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.Nfull) return;
+    const int prop = blockIdx.y;
+    const PropParams p = pp[prop];
+    const SlotParams s = sp[p.slot];
+    c64 *e0 = E + ((int64_t)2 * prop) * g.Nfull, *e1 = e0 + g.Nfull;
+    c64 a, b;
+    if (beam == 2) {
+        a = e0[m];
+        b = e1[m];
+    } else {
+        double ar, br;
+        entrance_field(lc, mask, g, s, p, (int)(m % g.Nx), (int)(m / g.Nx), beam == 1, waist, dx, dy, ar, br);
+        a = mk(ar, 0.0);
+        b = mk(br, 0.0);
+    }
+    const c64 ph = mk(p.ph_re, p.ph_im);
+    a = ph * a;
+    b = ph * b;
+    double e[12];
+    eps_voxel(lc, mask, g.field_dim, m + g.Nfull * (g.Nz - 1), s, e);
+    const double n = s.nout_fresnel, nxx = e[9], nyy = e[10], nxy = e[11];
+    const double det = (n + nxx) * (n + nyy) - nxy * nxy;
+    const double cxx = 1. - n * (n + nyy) / det, cyy = 1. - n * (n + nxx) / det, cxy = n * nxy / det;
+    e0[m] = 2.0 * (cxx * a + cxy * b);
+    e1[m] = 2.0 * (cyy * b + cxy * a);
+}
+
+void launch_output_entrance(const double *lc, const uint8_t *mask, const Geo &g, const SlotParams *sp,
+                            const PropParams *pp, int nprops, int beam, double waist, double dx, double dy, c64 *E,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(k_output_entrance, dim3(blocks_for(g.Nfull, 256), nprops), dim3(256), 0, st, lc, mask, g, sp,
+                       pp, beam, waist, dx, dy, E);
+    NMK_LAUNCH_CHECK();
+}
+
 // ---------------------------------------------------------------------------
 // one transverse plane of the bulk output (BPMIteration.cpp:111-120, 153-168):
 // march state -> full grid (duplicate column/row copied) with the phase of the

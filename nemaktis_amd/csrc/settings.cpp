@@ -279,7 +279,6 @@ Problem build_problem(const Settings &s, const double *lc, int64_t n_lc, const d
     p.dy = s.dy;
     p.dz = s.dz;
     const int64_t nv = p.voxels();
-    if (p.Nz < 2) throw BpmError("The mesh needs at least two planes along z");
     // UtilitiesLib/VectorField.cpp:26-62: field dimension from the array size
     int64_t fd = nv > 0 ? n_lc / nv : 0;
     if (n_lc != nv * fd) {

@@ -866,7 +866,7 @@ struct Oracle {
 };
 
 int validate(const OracleParams *p) {
-    if (!p || p->Nx < 1 || p->Ny < 1 || p->Nz < 2) return -1;
+    if (!p || p->Nx < 1 || p->Ny < 1 || p->Nz < 1) return -1;
     if (p->field_dim != 3 && p->field_dim != 6) return -2;
     if (p->n_wl < 1 || p->n_q < 1) return -3;
     return 0;

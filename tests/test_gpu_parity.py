@@ -387,3 +387,32 @@ def test_maximum_line_lengths(dims, precision):
     lc2 = fields.random_smooth(*big, (0.1, 0.1, 0.1), seed=59, corr_length=0.3)
     with pytest.raises(bpm_backend.BpmBackendError, match=str(max(dims))):
         _gpu(s2, lc2, precision=precision)
+
+
+def test_single_plane_mesh():
+    """A mesh of one plane (Nz = 1; the reference's loop over Nz-1 slabs runs
+    zero times): entrance and exit interfaces and the screen filter only, each
+    point of the duplicate column and row with its own permittivity; FP64 and
+    FP32 against the oracle, with and without a mask, caller-supplied fields,
+    and a graph context (nothing to capture) equal to the one-shot run."""
+    dims, sp = (20, 14, 1), (0.1, 0.1, 0.1)
+    s = _base(dims, sp, no="1.5", ne="1.7", nhost="1.45", nin="1.2", nout="1.33", na_objective=0.5,
+              wavevectors=[(0.0, 0.0), (0.05, 0.02)])
+    lc = fields.random_smooth(*dims, sp, seed=61, corr_length=0.3)
+    want = oracle.propagate(s, lc)
+    assert _rel(_gpu(s, lc), want) < TOL64
+    assert _rel(_gpu(s, lc, precision=32), want) < TOL32
+    mask = np.where(np.arange(dims[0])[None, None, :] < 10, 1.0, -1.0) * np.ones(dims[::-1])
+    assert _rel(_gpu(s, lc, mask=mask), oracle.propagate(s, lc, mask=mask)) < TOL64
+    sn = _base(dims, sp, no="1.5", ne="1.7", nhost="1.45", nout="1.33", na_objective=0.5, beam_profile="None")
+    rng = np.random.default_rng(67)
+    shape = fields.output_shape(sn)
+    Ein = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    assert _rel(_gpu(sn, lc, E_in=Ein), oracle.propagate(sn, lc, E=Ein)) < TOL64
+    one = _gpu(s, lc, precision=32, wl_batch=0)
+    shape = fields.output_shape(s)
+    with bpm_backend.Context(json.dumps(s), lc, int(np.prod(shape)), precision=32, wl_batch=0,
+                             use_graph=True) as ctx:
+        ctx.propagate()
+        got = ctx.download(np.empty(shape, np.complex128))
+    assert np.array_equal(got, one)

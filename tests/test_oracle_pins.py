@@ -514,3 +514,22 @@ def test_oracle_mask_weights_mismatch_falls_back():
     dims, sp = (6, 5, 6), (0.2, 0.2, 0.2)
     w, bad = mask_weights.interface_weights(lambda x, y, z: 0.05 - z + 0 * x, dims, sp, np.ones(dims[::-1]))
     assert bad and np.all(w == 0.5)
+
+
+def test_oracle_single_plane_is_the_two_fresnel_interfaces():
+    """A mesh of one plane (Nz = 1): no slab is marched (BPMIteration.cpp:127
+    loops Nz-1 times), so a uniform beam through an isotropic host medium of
+    index n comes out multiplied by the textbook transmissions
+    2 nin / (nin + n) (FresnelOperator.cpp:16-38) and 2 n / (n + nout)
+    (:45-69); the screen filter passes the uniform field (q = 0 inside the
+    objective NA, no focus shift)."""
+    dims, sp = (6, 5, 1), (0.1, 0.1, 0.1)
+    nin, n, nout = 1.2, 1.45, 1.33
+    s = fields.make_settings(dims, sp, [0.6], no="1.5", ne="1.7", nhost=str(n), nin=str(nin), nout=str(nout),
+                             na_objective=0.5)
+    lc = fields.random_smooth(*dims, sp, seed=3, corr_length=0.2)
+    E = oracle.propagate(s, lc, mask=-np.ones(dims[::-1]))
+    t = 4 * nin * n / ((nin + n) * (n + nout))
+    np.testing.assert_allclose(E[0, 0, 0], t, rtol=1e-13)        # x input -> Ex
+    np.testing.assert_allclose(E[0, 0, 3], t, rtol=1e-13)        # y input -> Ey
+    assert np.abs(E[0, 0, 1]).max() < 1e-15 and np.abs(E[0, 0, 2]).max() < 1e-13
