@@ -70,11 +70,10 @@ def algorithmic_bytes(fam, c, nprops, nslots, NP, Nfull, fd, nw):
     return None
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_random512.txt")
 TRAFFIC_FILE_IMAGING = os.path.join(ROOT, "profiles", "hbm_traffic_imaging_droplet256.txt")
 
 
-def measured_traffic(kernel, path=TRAFFIC_FILE):
+def measured_traffic(kernel, path):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain
     copy); (None, None) when there is none."""
@@ -283,9 +282,9 @@ def rel_errors(a, b):
 
 def droplet_leg(steps, warmup):
     """The 256x256x80 droplet sweep (21 wavelengths x 7 wavevectors x 2
-    polarisations = 294 propagations marched together, raw mask), FP32 with
-    graph replay and the built-in launch shapes: wall time per step with the
-    director resident in HBM."""
+    polarisations = 294 propagations marched together, mask and mask formula),
+    FP32 with graph replay and the built-in launch shapes: wall time per step
+    with the director resident in HBM."""
     from nemaktis_amd import bpm_backend, fields
     s, lc, mask, formula = fields.workload("droplet256")
     n_E = int(np.prod(fields.output_shape(s)))
@@ -413,6 +412,26 @@ def drop_in_leg(js, lc, shape, precision):
             "download_gbs": E.nbytes / 1e9 / max(down, 1e-9),
             "note": "one-shot run_backend_without_mask on host arrays (copy-in of the director, eager launches, "
                     "copy-out of E); context_create = allocation + director upload"}
+
+
+def drop_in_masked_leg():
+    """The masked one-shot call on the droplet sweep (run_backend_with_mask
+    with the mask formula, FP32, all wavelengths at once), host arrays in and
+    out, timed end to end beside its device time; the call uploads the director
+    and the LC flags and computes the interface weights on the device."""
+    from nemaktis_amd import bpm_backend, fields
+    s, lc, mask, formula = fields.workload("droplet256")
+    js = json.dumps(s)
+    E = np.zeros(fields.output_shape(s), np.complex128)
+    bpm_backend.run_backend_with_mask(js, formula, lc, mask, E, precision=32, wl_batch=0, strict=True)
+    st = {}
+    t0 = time.perf_counter()
+    bpm_backend.run_backend_with_mask(js, formula, lc, mask, E, precision=32, wl_batch=0, strict=True, stats=st)
+    wall = time.perf_counter() - t0
+    return {"call_ms": 1e3 * wall, "device_ms": st.get("total_ms"), "director_gb": lc.nbytes / 1e9,
+            "mask_gb": mask.nbytes / 1e9, "output_gb": E.nbytes / 1e9,
+            "note": "one-shot run_backend_with_mask, droplet 256x256x80 with its mask formula, 294 propagations, "
+                    "FP32, eager launches"}
 
 
 def roofline_of(js, lc, n_E, precision, wl_batch, steps, Nx, Ny, nprops, nq, fd, nw, tuning=None):
@@ -558,7 +577,7 @@ def main():
     dropin = None
     if not args.no_dropin:
         log("one-shot drop-in call ...")
-        dropin = {"fp64": drop_in_leg(js, lc, shape, 64)}
+        dropin = {"fp64": drop_in_leg(js, lc, shape, 64), "fp32_droplet_masked": drop_in_masked_leg()}
 
     imaging = None
     if not args.no_imaging:
