@@ -23,7 +23,8 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 
 HIP_SOURCES = ["kernels.hip", "viewer_kernels.hip", "noalloc.hip"]
-CXX_SOURCES = ["json.cpp", "expr.cpp", "settings.cpp", "vti.cpp", "engine.cpp", "server.cpp", "viewer.cpp", "capi.cpp"]
+CXX_SOURCES = ["json.cpp", "expr.cpp", "settings.cpp", "vti.cpp", "staging.cpp", "engine.cpp", "server.cpp",
+               "viewer.cpp", "capi.cpp"]
 
 
 def _run(cmd):

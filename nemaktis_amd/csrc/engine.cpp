@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "staging.h"
 #include "vti.h"
 
 namespace nmk {
@@ -188,10 +189,10 @@ public:
         if (!o.profile && !o.no_side_stream) HIPCHK(hipStreamCreateWithFlags(&st2_, hipStreamNonBlocking));
 
         // inputs resident in HBM
-        if (!dev_in) HIPCHK(hipMemcpyAsync(lc_.ptr, p.lc, lc_.bytes(), hipMemcpyHostToDevice, st_));
+        if (!dev_in) staged_h2d(lc_.ptr, p.lc, lc_.bytes(), st_);
         if (mask_.ptr) {
             if (dev_in) launch_mask_flags(o.dev.mask, mask_.ptr, p.voxels(), st_);
-            else HIPCHK(hipMemcpyAsync(mask_.ptr, p.mask.data(), mask_.bytes(), hipMemcpyHostToDevice, st_));
+            else staged_h2d(mask_.ptr, p.mask.data(), mask_.bytes(), st_);
         }
         if (dev_weights) {
             // the interface weights of the mask formula, from the LC flags just uploaded
@@ -200,7 +201,7 @@ public:
                                      flag_.ptr, st_);
             HIPCHK(hipMemcpyAsync(&mismatch_, flag_.ptr, sizeof(int), hipMemcpyDeviceToHost, st_));
         } else if (weights_.ptr) {
-            HIPCHK(hipMemcpyAsync(weights_.ptr, p.weights.data(), weights_.bytes(), hipMemcpyHostToDevice, st_));
+            staged_h2d(weights_.ptr, p.weights.data(), weights_.bytes(), st_);
         }
         hq_.clear();
         for (int i = 0; i < sz_.nq; i++) {
@@ -261,14 +262,11 @@ public:
     }
 
     void upload_fields(const c64 *E_host) override {
-        HIPCHK(hipMemcpyAsync(E_p_, E_host, out_bytes(), hipMemcpyHostToDevice, st_));
+        staged_h2d(E_p_, E_host, out_bytes(), st_);
         HIPCHK(hipStreamSynchronize(st_));
     }
 
-    void download(c64 *E_host) override {
-        HIPCHK(hipMemcpyAsync(E_host, E_p_, out_bytes(), hipMemcpyDeviceToHost, st_));
-        HIPCHK(hipStreamSynchronize(st_));
-    }
+    void download(c64 *E_host) override { staged_d2h(E_host, E_p_, out_bytes(), st_); }
 
     void propagate(Stats *stats) override {
         // no_timing: no events at all (the run creates no HIP objects); the
