@@ -13,8 +13,9 @@ fields stay in HBM between calls and only the image (or what the caller asks
 for) comes back to the host.  There is no CPU fallback.
 
 Differences from the reference, by construction:
-  * VTI import/export (``OpticalFields(vti_file=...)``, ``save_to_vti``) and
-    the GUI (``FieldViewer.plot``) are not provided.
+  * The GUI (``FieldViewer.plot``) is not provided; ``OpticalFields(vti_file=...)``
+    and ``save_to_vti`` read and write the reference's file layout without VTK
+    (``vti.py``).
   * Without ``cmf`` the colour table is ``colour.default_cmf`` (CIE 1931
     functions by a published analytic fit, a 6504 K black body for D65,
     normalised to the D65 white point) instead of dtmm's tabulated
@@ -24,14 +25,24 @@ Differences from the reference, by construction:
     focus altitude).
 """
 import ctypes
+import os
 
 import numpy as np
 
-from . import _lib, colour
+from . import _lib, colour, vti
 
 
 class ViewerError(RuntimeError):
     pass
+
+
+def _eval_index(expr, lam):
+    """An index expression of the settings at wavelength lam (um), evaluated as the backend evaluates it."""
+    lib = _lib.load()
+    v = ctypes.c_double()
+    err = _lib.err_buf()
+    _check(lib.nmk_bpm_eval_expression(expr.encode(), float(lam), ctypes.byref(v), err, len(err)), err)
+    return v.value
 
 
 def _check(rc, err):
@@ -95,8 +106,9 @@ class OpticalFields:
     """
 
     def __init__(self, **kwargs):
-        if "vti_file" in kwargs:
-            raise NotImplementedError("VTI import is not provided by nemaktis_amd")
+        if len(kwargs) == 1 and "vti_file" in kwargs:
+            self._init_from_vti(kwargs["vti_file"])
+            return
         if len(kwargs["mesh_dimensions"]) != 2:
             raise Exception("mesh_dimensions should be an array-like object of size 2")
         if len(kwargs["mesh_lengths"]) != 2:
@@ -120,6 +132,89 @@ class OpticalFields:
         shape = (len(self._wavelengths), len(self._wavevectors), 4, self._Ny, self._Nx)
         self._vals = np.zeros(shape, dtype=np.complex128)
         self._init_common()
+
+    def _init_from_vti(self, path):
+        """Fields, wavelengths, wavevectors and objective data of a file written by ``save_to_vti`` (or by the
+        reference's, light_propagator.py:449-541: the same array names, checks and messages), read without VTK."""
+        if not os.path.isfile(path):
+            raise Exception("VTI file does not exists")
+        print("{ Initializing optical fields from " + path + " }")
+        geo, fd, pd = vti.read_vti(path)
+        dims, spacings = geo["dims"], geo["spacing"]
+        if dims[2] != 1:
+            raise Exception("The specified vti file should include 2D data")
+        self._zfoc_NA_corr = float(np.ravel(fd["zfoc_NA_corr"])[0]) if "zfoc_NA_corr" in fd else 0
+        if "lambda" not in fd:
+            raise Exception("VTI file is missing the field array \"lambda\" for the wavelengths")
+        self._wavelengths = np.asarray(fd["lambda"], dtype=np.float64).reshape(-1)
+        Nl = len(self._wavelengths)
+        if "n_out" in fd:
+            n_out = np.asarray(fd["n_out"], dtype=np.float64).reshape(-1)
+            self._n_out = float(n_out[0]) if n_out.size == 1 else n_out
+        else:
+            self._n_out = np.ones(Nl)
+        for k in ("qx", "qy"):
+            if k not in fd:
+                raise Exception("VTI file is missing the field array \"%s\" for the wavevectors" % k)
+        self._wavevectors = np.stack((np.asarray(fd["qx"], dtype=np.float64).reshape(-1),
+                                      np.asarray(fd["qy"], dtype=np.float64).reshape(-1)), axis=-1)
+        Nq = len(self._wavevectors)
+        self._koehler_1D = bool(np.ravel(fd["koehler_1D"])[0] == 1) if "koehler_1D" in fd else False
+        if (dims[0] == 1 or dims[1] == 1) and self._koehler_1D:
+            Nr = Nq // 2
+            if Nq != 1 + 2 * Nr:
+                raise Exception("VTI file contain the wrong number of wavevectors")
+        else:
+            Nr = int(np.round((1 + np.sqrt((4 * Nq - 1) / 3)) / 2))
+            if Nq != 1 + 3 * Nr * (Nr - 1):
+                raise Exception("VTI file contain the wrong number of wavevectors")
+        self._N_radial_wavevectors = Nr
+        self._max_NA_condenser = float(np.sqrt(np.sum(self._wavevectors ** 2, axis=1))[-1])
+        if "max_NA_objective" not in fd:
+            raise Exception("VTI file is missing the field scalar \"max_NA_objective\"")
+        self._max_NA_objective = float(np.ravel(fd["max_NA_objective"])[0])
+        self._NA = self._max_NA_objective
+        (self._Nx, self._Ny) = (int(dims[0]), int(dims[1]))
+        (self._dx, self._dy) = (float(spacings[0]), float(spacings[1]))
+        (self._Lx, self._Ly) = (self._dx * (self._Nx - 1), self._dy * (self._Ny - 1))
+        self._device = -1
+        vals = np.zeros((Nl, Nq, 4, self._Ny, self._Nx), dtype=np.complex128)
+        for w in range(Nl):
+            for q in range(Nq):
+                names = ["E_%s_input%s_%d_%d" % (part, pol, w, q) for pol in "XY" for part in ("real", "imag")]
+                for name in names:
+                    if name not in pd:
+                        raise Exception("Missing array \"" + name + "\" in VTI file")
+                for k, pol in enumerate("XY"):
+                    re_ = pd["E_real_input%s_%d_%d" % (pol, w, q)].reshape(self._Ny, self._Nx, -1)
+                    im_ = pd["E_imag_input%s_%d_%d" % (pol, w, q)].reshape(self._Ny, self._Nx, -1)
+                    vals[w, q, 2 * k] = re_[..., 0] + 1j * im_[..., 0]
+                    vals[w, q, 2 * k + 1] = re_[..., 1] + 1j * im_[..., 1]
+        self._vals = vals
+        self._init_common()
+
+    def save_to_vti(self, filename):
+        """Writes the fields and the data needed to read them back (wavelengths, wavevectors, n_out, focus
+        correction, Koehler flag, objective NA) as a VTK image-data file (reference:
+        light_propagator.py:722-790; uncompressed raw appended arrays, readable by VTK and ``vti.read_vti``);
+        ".vti" is appended unless the name ends with it."""
+        path = filename if filename[-4:] == ".vti" else filename + ".vti"
+        print("{ Saving optical fields to " + path + " }")
+        vals = self.vals
+        Np = self._Nx * self._Ny
+        point = {}
+        for w in range(len(self._wavelengths)):
+            for q in range(len(self._wavevectors)):
+                for k, pol in enumerate("XY"):
+                    E = vals[w, q, [2 * k, 2 * k + 1], :, :].reshape(2, Np).T
+                    point["E_real_input%s_%d_%d" % (pol, w, q)] = np.real(E)
+                    point["E_imag_input%s_%d_%d" % (pol, w, q)] = np.imag(E)
+        field = {"lambda": self._wavelengths, "zfoc_NA_corr": [self.zfoc_NA_corr],
+                 "n_out": np.atleast_1d(np.asarray(self.n_out, dtype=np.float64)),
+                 "koehler_1D": [1.0 if self._koehler_1D else 0.0], "qx": self._wavevectors[:, 0],
+                 "qy": self._wavevectors[:, 1], "max_NA_objective": [self._max_NA_objective]}
+        vti.write_vti(path, (self._Nx, self._Ny, 1), (self._dx, self._dy, 0.0), (-self._Lx / 2, -self._Ly / 2, 0.0),
+                      field, point)
 
     def _init_common(self):
         if self._N_radial_wavevectors > 1 and self._max_NA_condenser > 0:
@@ -161,8 +256,14 @@ class OpticalFields:
         self._N_radial_wavevectors = int(np.round((1 + np.sqrt((4 * Nq - 1) / 3)) / 2))
         self._max_NA_condenser = float(np.sqrt(np.sum(self._wavevectors ** 2, axis=1))[-1])
         self._koehler_1D = False
-        self._n_out = None
-        self._zfoc_NA_corr = None
+        # n_out per wavelength and the focus correction of the isotropic layers, as the device view has them
+        # (light_propagator.py:194-207)
+        nout = str(co.get("nout", "1"))
+        self._n_out = np.array([_eval_index(nout, lam) for lam in self._wavelengths])
+        n0 = _eval_index(nout, 0.6)
+        self._zfoc_NA_corr = 0.0
+        for ek, nk in zip(scr.get("Isotropic layer thicknesses", []), scr.get("Isotropic layer refractive indices", [])):
+            self._zfoc_NA_corr += 3 * n0 * ek / (14 * nk) * (1 / n0 ** 2 - 1 / nk ** 2)
         self._device = -1
         self._vals = None
         self._init_common()
@@ -308,9 +409,6 @@ class OpticalFields:
 
     def get_n_vertices(self):
         return self._Nx * self._Ny
-
-    def save_to_vti(self, filename):
-        raise NotImplementedError("VTI export is not provided by nemaktis_amd")
 
 
 class FieldViewer:

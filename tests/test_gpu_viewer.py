@@ -137,7 +137,7 @@ def test_image_without_gamma_is_clipped_and_tiles_flip():
     assert img.min() >= 0 and img.max() <= 1
 
 
-def test_view_from_context_equals_view_from_host_fields():
+def test_view_from_context_equals_view_from_host_fields(tmp_path):
     s, lc, _, _ = fields.workload("tn64")
     n_E = len(s["Physics settings"]["Coefficients"]["Wavelengths"]) * 1 * 4 * 64 * 64
     with bpm_backend.Context(json.dumps(s), lc, n_E) as ctx:
@@ -158,6 +158,14 @@ def test_view_from_context_equals_view_from_host_fields():
         fh.update_image()
         np.testing.assert_array_equal(fd.get_image(), fh.get_image())
         assert dev.vals.shape == E.shape and np.array_equal(dev.vals, E)
+        # the fields taken from the context saved and read back as the reference's VTI file
+        np.testing.assert_array_equal(dev.n_out, [1.0])
+        dev.save_to_vti(str(tmp_path / "fields"))
+        back = viewer.OpticalFields(vti_file=str(tmp_path / "fields.vti"))
+        assert np.array_equal(back.vals, E) and back.zfoc_NA_corr == dev.zfoc_NA_corr == 0
+        back.focus_fields(1.2)
+        # the file's spacing against lengths / (N - 1): the filter may differ in the last place
+        np.testing.assert_allclose(back.focused_vals, host.focused_vals, rtol=0, atol=1e-14)
 
 
 def test_view_errors_are_reported():

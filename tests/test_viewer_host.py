@@ -175,8 +175,39 @@ def test_optical_fields_host_interface():
         of.vals = np.zeros((1, 7, 4, 31, 41))
     of.update_NA_objective(0.9)
     assert of._NA == 0.4
-    with pytest.raises(NotImplementedError):
-        viewer.OpticalFields(vti_file="x.vti")
+    with pytest.raises(Exception, match="VTI file does not exists"):
+        viewer.OpticalFields(vti_file="no_such_file.vti")
+
+
+def test_optical_fields_vti_round_trip(tmp_path):
+    """save_to_vti then OpticalFields(vti_file=...) (light_propagator.py:449-541, 722-790): the same fields
+    bit for bit and the same wavelengths, wavevectors, n_out, focus correction, Koehler flag, objective NA
+    and mesh; the file carries the reference's array names; a missing array is reported as the reference
+    reports it."""
+    from nemaktis_amd import vti
+    rng = np.random.default_rng(4)
+    of = viewer.OpticalFields(wavelengths=[0.5, 0.6, 0.7], max_NA_objective=0.4, max_NA_condenser=0.2,
+                              N_radial_wavevectors=2, mesh_lengths=(4.0, 3.0), mesh_dimensions=(41, 31),
+                              n_out=[1.0, 1.33, 1.5], zfoc_NA_corr=-0.7)
+    v = rng.standard_normal(of.vals.shape) + 1j * rng.standard_normal(of.vals.shape)
+    of.vals = v
+    of.save_to_vti(str(tmp_path / "fields"))
+    path = tmp_path / "fields.vti"
+    geo, fd, pd = vti.read_vti(str(path))
+    assert geo["dims"] == (41, 31, 1) and "E_imag_inputY_2_6" in pd and pd["E_real_inputX_0_0"].shape[-1] == 2
+    back = viewer.OpticalFields(vti_file=str(path))
+    assert np.array_equal(back.vals, v)
+    np.testing.assert_array_equal(back.get_wavelengths(), [0.5, 0.6, 0.7])
+    np.testing.assert_array_equal(back.get_wavevectors(), of.get_wavevectors())
+    np.testing.assert_array_equal(back.n_out, [1.0, 1.33, 1.5])
+    assert back.zfoc_NA_corr == -0.7 and back._koehler_1D is False and back._max_NA_objective == 0.4
+    assert back._N_radial_wavevectors == 2 and back._max_NA_condenser == pytest.approx(0.2)
+    assert back.get_mesh_dimensions() == (41, 31) and back.get_mesh_spacings() == pytest.approx((0.1, 0.1))
+    field = {k: v for k, v in fd.items() if k != "lambda"}
+    point = {k: v.reshape(41 * 31, -1) for k, v in pd.items()}
+    vti.write_vti(str(tmp_path / "broken.vti"), (41, 31, 1), (0.1, 0.1, 0.0), (-2.0, -1.5, 0.0), field, point)
+    with pytest.raises(Exception, match="lambda"):
+        viewer.OpticalFields(vti_file=str(tmp_path / "broken.vti"))
 
 
 def test_viewer_fails_loudly_without_a_device():
