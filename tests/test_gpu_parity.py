@@ -416,3 +416,29 @@ def test_single_plane_mesh():
         ctx.propagate()
         got = ctx.download(np.empty(shape, np.complex128))
     assert np.array_equal(got, one)
+
+
+def test_staged_and_direct_host_copies_agree(tmp_path):
+    """Host copies of 16 MB and more go through pinned chunks filled by host
+    threads; a child process with NMK_NO_STAGING=1 copies from the caller's
+    memory directly.  Same output bit for bit (a 126 MB director in, 8 MB
+    out; a 617 MB output out on the droplet sweep is covered by the
+    benchmark-workload tests)."""
+    import os
+    import subprocess
+    import sys
+    dims = (256, 256, 80)
+    s = _base(dims, no="1.5", ne="1.7", na_objective=0.4)
+    lc = fields.random_smooth(*dims, (0.1, 0.1, 0.1), seed=71, corr_length=0.5)
+    np.save(tmp_path / "lc.npy", lc)
+    (tmp_path / "s.json").write_text(json.dumps(s))
+    staged = _gpu(s, lc, precision=32, wl_batch=0)
+    code = ("import json, sys, numpy as np; sys.path.insert(0, %r); from nemaktis_amd import bpm_backend; "
+            "d = %r; s = open(d + '/s.json').read(); lc = np.load(d + '/lc.npy'); "
+            "np.save(d + '/E.npy', bpm_backend.propagate(s, lc, precision=32, wl_batch=0))"
+            % (os.path.dirname(os.path.dirname(os.path.abspath(__file__))), str(tmp_path)))
+    print("[gpu case] direct host copies in a child process", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], env=dict(os.environ, NMK_NO_STAGING="1"), timeout=240,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert np.array_equal(np.load(tmp_path / "E.npy"), staged)
