@@ -42,8 +42,9 @@ def build_library(verbose=False):
         o = os.path.join(OBJ, src + ".o")
         jobs.append([HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", o])
     for src in CXX_SOURCES:
+        # host code; hipcc still runs a device pass over the HIP headers: for gfx950 only, with the same flags
         o = os.path.join(OBJ, src + ".o")
-        jobs.append([HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", o])
+        jobs.append([HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", o])
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         outs = list(ex.map(_run, jobs))
     if verbose:

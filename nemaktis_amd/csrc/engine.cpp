@@ -334,7 +334,7 @@ private:
         bg.qy = p_.qy;
         const size_t blk = (size_t)p_.Nz * 2 * sz_.Nfull;
         write_bulk_vti(p_.bulk_path, p_.bulk_dir, bg, [&](size_t k, c64 *dst) {
-            HIPCHK(hipMemcpy(dst, bulk_.ptr + k * blk, blk * sizeof(c64), hipMemcpyDeviceToHost));
+            staged_d2h(dst, bulk_.ptr + k * blk, blk * sizeof(c64), st_);
         });
     }
 

@@ -7,6 +7,9 @@
 #include <cstring>
 #include <memory>
 #include <sstream>
+#include <thread>
+#include <vector>
+#include <algorithm>
 
 #include "settings.h"
 
@@ -96,9 +99,11 @@ void write_bulk_vti(const std::string &path, const std::string &dir, const BulkG
     vec(g.qy);
     std::vector<c64> blk(npts * 2);
     std::vector<double> re(npts * 3), im(npts * 3);
-    for (size_t k = 0; k < nblocks; k++) {
-        fetch(k, blk.data());
-        for (int z = 0; z < g.Nz; z++) {
+    // planes split over host threads: (Ex, Ey) complex planes -> 3-component real and imaginary arrays
+    const int nthreads = (int)std::max<size_t>(1, std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()),
+                                                                     (size_t)g.Nz, npts >> 18}));
+    auto planes = [&](int z0, int z1) {
+        for (int z = z0; z < z1; z++) {
             const c64 *ex = blk.data() + (size_t)z * 2 * N, *ey = ex + N;
             double *r = re.data() + (size_t)z * N * 3, *i = im.data() + (size_t)z * N * 3;
             for (size_t m = 0; m < N; m++) {
@@ -109,6 +114,17 @@ void write_bulk_vti(const std::string &path, const std::string &dir, const BulkG
                 i[3 * m + 1] = ey[m].y;
                 i[3 * m + 2] = 0.0;
             }
+        }
+    };
+    for (size_t k = 0; k < nblocks; k++) {
+        fetch(k, blk.data());
+        if (nthreads == 1) {
+            planes(0, g.Nz);
+        } else {
+            std::vector<std::thread> th;
+            for (int t = 1; t < nthreads; t++) th.emplace_back(planes, g.Nz * t / nthreads, g.Nz * (t + 1) / nthreads);
+            planes(0, g.Nz / nthreads);
+            for (auto &x : th) x.join();
         }
         put(out.f, &arr_bytes, 8, path);
         put(out.f, re.data(), arr_bytes, path);
