@@ -114,6 +114,17 @@ private:
     const std::string &s_;
     const std::vector<std::string> &vars_;
     size_t pos_ = 0;
+    int depth_ = 0;  // nested sub-expressions and unary signs (recursion depth)
+
+    // bounds the recursion: a formula nested deeper is refused rather than
+    // overflowing the stack
+    struct Nest {
+        Parser &p;
+        explicit Nest(Parser &q) : p(q) {
+            if (++p.depth_ > 1000) p.fail("expression nested too deeply");
+        }
+        ~Nest() { --p.depth_; }
+    };
 
     [[noreturn]] void fail(const std::string &m) const {
         throw ExprError("expression error at position " + std::to_string(pos_) + ": " + m + " in \"" + s_ + "\"");
@@ -140,6 +151,7 @@ private:
     }
 
     NodeP ternary() {
+        Nest guard(*this);
         NodeP c = lor();
         if (eat("?")) {
             NodeP a = ternary();
@@ -196,6 +208,7 @@ private:
     }
     // unary sign binds looser than ^ : -2^2 = -4
     NodeP unary() {
+        Nest guard(*this);
         if (eat("-")) {
             NodeP n(new Expr::Node);
             n->kind = Expr::Node::Neg;

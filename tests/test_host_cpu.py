@@ -74,6 +74,17 @@ def test_expression_errors(bad):
         _eval(bad)
 
 
+def test_expression_nesting_is_bounded():
+    """Nesting is refused beyond a bound instead of overflowing the parser's
+    stack (host-code sanitizer finding: 5000 nested parentheses); 400 levels
+    of parentheses and of unary signs still evaluate."""
+    assert _eval("(" * 400 + "1.5" + ")" * 400) == 1.5
+    assert _eval("-" * 400 + "2") == 2.0
+    for deep in ("(" * 5000 + "1" + ")" * 5000, "-" * 100000 + "1", "2^" * 3000 + "1"):
+        with pytest.raises(ValueError, match="nested too deeply"):
+            _eval(deep)
+
+
 def _small():
     s = fields.make_settings((8, 6, 3), (0.1, 0.1, 0.1), [0.6], no="1.5", ne="1.7")
     lc = np.zeros((3, 6, 8, 3)); lc[..., 2] = 1
