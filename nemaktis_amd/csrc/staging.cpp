@@ -48,12 +48,16 @@ void par_memcpy(void *dst, const void *src, size_t n) {
 struct Stager {
     std::mutex mu;
     bool tried = false, ok = false;
+    int dev = -1;  // the device the events belong to: copies on other devices go straight through
     void *buf[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
 
     bool init() {  // under mu
-        if (tried) return ok;
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess) return false;
+        if (tried) return ok && cur == dev;
         tried = true;
+        dev = cur;
         const char *off = std::getenv("NMK_NO_STAGING");
         if (off && *off && *off != '0') return ok = false;
         for (int b = 0; b < 2; b++) {
