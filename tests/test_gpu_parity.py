@@ -145,6 +145,23 @@ def test_iso_layers_focus_and_caller_fields_fp64():
     assert err < TOL64, err
 
 
+def test_dispersive_nout_with_iso_layers_and_focus_fp64():
+    """A dispersive exit index with isotropic layers and a focus shift over
+    three wavelengths: the focus correction uses nout at 0.6 um
+    (PostprocessorLib/ScreenOutput.cpp:34-38) while each wavelength's filter
+    uses its own nout, as the oracle restates; FP64 and FP32 against it."""
+    dims = (18, 14, 5)
+    s = _base(dims, wl=[0.45, 0.6, 0.75], wavevectors=[(0.0, 0.0), (0.06, -0.03)], no="1.5",
+              ne="1.7+0.01/lambda^2", nout="1.33+0.004/lambda^2", na_objective=0.6)
+    s["Postprocessor settings"]["Screen output"].update({
+        "Isotropic layer thicknesses": [0.8, 0.3], "Isotropic layer refractive indices": [1.45, 1.38],
+        "Focalisation z-shift": 0.9})
+    lc = fields.random_smooth(*dims, (0.1, 0.1, 0.1), seed=73, corr_length=0.3)
+    want = oracle.propagate(s, lc)
+    assert _rel(_gpu(s, lc), want) < TOL64
+    assert _rel(_gpu(s, lc, precision=32, wl_batch=0), want) < TOL32
+
+
 @pytest.mark.parametrize("dims", [(15, 12, 5), (3, 20, 4), (18, 1, 4), (21, 10, 3), (12, 10, 2), (2, 2, 3)])
 def test_odd_and_degenerate_meshes_fp64(dims):
     s = _base(dims, wavevectors=[(0.08, -0.12)], no="1.5", ne="1.7", na_objective=0.6)
