@@ -100,3 +100,24 @@ def test_bulk_output_unwritable_folder_is_reported(tmp_path):
     E = np.zeros(fields.output_shape(s), np.complex128)
     with pytest.raises(bpm_backend.BpmBackendError, match="results folder|bulk output file"):
         bpm_backend.run_backend_without_mask(json.dumps(s), lc, E, strict=True)
+
+
+def test_bulk_output_of_a_serving_job(tmp_path):
+    """A serving job with the bulk output active (its planes carved from the
+    arena with the job's other buffers) writes the same file as the one-shot
+    call, bit for bit, beside a job without it."""
+    from nemaktis_amd import serving
+    s, lc = _case(folder=str(tmp_path / "srv"), name="job")
+    plain, lc2 = _case(seed=4)
+    plain["Postprocessor settings"]["Bulk output"]["Activate"] = False
+    with serving.Server(max_active=2) as srv:
+        print("[gpu case] serving job with bulk output", flush=True)
+        j = srv.submit(json.dumps(s), lc, precision=32)
+        k = srv.submit(json.dumps(plain), lc2, precision=32)
+        assert j.wait()["state"] == "done" and k.wait()["state"] == "done"
+    s1 = json.loads(json.dumps(s))
+    s1["Algorithm settings"]["General"]["Results folder name"] = str(tmp_path / "one")
+    E = bpm_backend.propagate(json.dumps(s1), lc, precision=32, wl_batch=0)
+    assert np.array_equal(j.E, E)
+    a, b = vti.bulk_fields(str(tmp_path / "srv" / "job.vti")), vti.bulk_fields(str(tmp_path / "one" / "job.vti"))
+    assert sorted(a) == sorted(b) and all(np.array_equal(a[key], b[key]) for key in a)
