@@ -574,30 +574,25 @@ def main():
         if args.cpu_nz == Nz:
             cpu["fp64_gpu_rel_l2_vs_oracle_max"] = float(rel_errors(out64, cpu_out).max())
 
-    dropin = None
-    if not args.no_dropin:
-        log("one-shot drop-in call ...")
-        dropin = {"fp64": drop_in_leg(js, lc, shape, 64), "fp32_droplet_masked": drop_in_masked_leg()}
+    def side_leg(name, fn):
+        """a leg beside the headline: an error in it is reported in its place
+        instead of losing the whole line"""
+        log(name + " ...")
+        try:
+            return fn()
+        except Exception as e:  # noqa: BLE001
+            import traceback
+            traceback.print_exc()
+            return {"error": "%s: %s" % (type(e).__name__, e)}
 
-    imaging = None
-    if not args.no_imaging:
-        log("refocus + micrograph leg ...")
-        imaging = imaging_leg(args.steps, args.warmup, not args.no_cpu)
-
-    bulk = None
-    if not args.no_bulk:
-        log("bulk-output leg ...")
-        bulk = bulk_leg(s, lc, tuning32, args.bulk_dir)
-
-    droplet = None
-    if not args.no_droplet:
-        log("droplet sweep leg ...")
-        droplet = droplet_leg(3, 1)
-
-    serve = None
-    if not args.no_serve:
-        log("serving loop trace ...")
-        serve = serve_leg(args.serve_jobs, args.serve_rate, 5, args.serve_active)
+    dropin = None if args.no_dropin else side_leg("one-shot drop-in call", lambda: {
+        "fp64": drop_in_leg(js, lc, shape, 64), "fp32_droplet_masked": drop_in_masked_leg()})
+    imaging = None if args.no_imaging else side_leg(
+        "refocus + micrograph leg", lambda: imaging_leg(args.steps, args.warmup, not args.no_cpu))
+    bulk = None if args.no_bulk else side_leg("bulk-output leg", lambda: bulk_leg(s, lc, tuning32, args.bulk_dir))
+    droplet = None if args.no_droplet else side_leg("droplet sweep leg", lambda: droplet_leg(3, 1))
+    serve = None if args.no_serve else side_leg(
+        "serving loop trace", lambda: serve_leg(args.serve_jobs, args.serve_rate, 5, args.serve_active))
 
     line = {
         "metric": "voxel-slice updates/s at 512x512x100 (FP64 path; FP32 batched path beside it); "
