@@ -55,6 +55,30 @@ __device__ __forceinline__ void fft_stage(const c64 *__restrict__ b0, c64 *__res
             o[os] = t1 + t3;
             o[2 * os] = t0 - t2;
             o[3 * os] = t1 - t3;
+        } else if constexpr (R == 8) {
+            // two 4-point transforms of the even and odd inputs, joined by
+            // w_8^k (k = 0..3; w_8^2 = -+i, w_8^1 and w_8^3 from the table)
+            auto dft4 = [&](c64 a, c64 b, c64 c, c64 d, c64 (&y)[4]) {
+                const c64 t0 = a + c, t1 = a - c, t2 = b + d, e = b - d;
+                const c64 t3 = sign > 0 ? mk(-e.y, e.x) : mk(e.y, -e.x);
+                y[0] = t0 + t2;
+                y[1] = t1 + t3;
+                y[2] = t0 - t2;
+                y[3] = t1 - t3;
+            };
+            c64 ev[4], od[4];
+            dft4(u[0], u[2], u[4], u[6], ev);
+            dft4(u[1], u[3], u[5], u[7], od);
+            const c64 o1 = od[1] * wr[1], o3 = od[3] * wr[3];
+            const c64 o2 = sign > 0 ? mk(-od[2].y, od[2].x) : mk(od[2].y, -od[2].x);
+            o[0] = ev[0] + od[0];
+            o[os] = ev[1] + o1;
+            o[2 * os] = ev[2] + o2;
+            o[3 * os] = ev[3] + o3;
+            o[4 * os] = ev[0] - od[0];
+            o[5 * os] = ev[1] - o1;
+            o[6 * os] = ev[2] - o2;
+            o[7 * os] = ev[3] - o3;
         } else {
 #pragma unroll
             for (int pq = 0; pq < R; pq++) {
@@ -70,7 +94,7 @@ __device__ __forceinline__ void fft_stage(const c64 *__restrict__ b0, c64 *__res
 // runs every stage of plan pl on the C lines in b0 (all threads of the block
 // take part; the caller has synchronised after filling b0 and the twiddle
 // table tw, normally an LDS copy) and returns the buffer that holds the result
-// (ODD: the plan may hold radices 3 to 13; without it only 2 and 4 are
+// (ODD: the plan may hold radices 3 to 13; without it only 2, 4 and 8 are
 // compiled, which keeps the power-of-two kernels small)
 template <int C, bool LM, bool ODD>
 __device__ c64 *fft_stages(c64 *b0, c64 *b1, const FftPlan &pl, const c64 *__restrict__ tw, int sign) {
@@ -79,6 +103,7 @@ __device__ c64 *fft_stages(c64 *b0, c64 *b1, const FftPlan &pl, const c64 *__res
     for (int st = 0; st < pl.nst; st++) {
         const int r = (int)((pl.radices >> (4 * st)) & 15);
         if (r == 4) fft_stage<C, LM, 4>(b0, b1, n, L, tw, sign);
+        else if (r == 8) fft_stage<C, LM, 8>(b0, b1, n, L, tw, sign);
         else if (r == 2) fft_stage<C, LM, 2>(b0, b1, n, L, tw, sign);
         else if constexpr (ODD) {
             switch (r) {
@@ -98,11 +123,11 @@ __device__ c64 *fft_stages(c64 *b0, c64 *b1, const FftPlan &pl, const c64 *__res
     return b0;
 }
 
-// true when every stage of the plan has radix 2 or 4
+// true when every stage of the plan has radix 2, 4 or 8
 __host__ __device__ inline bool fft_pow2(const FftPlan &pl) {
     for (int st = 0; st < pl.nst; st++) {
         const int r = (int)((pl.radices >> (4 * st)) & 15);
-        if (r != 2 && r != 4) return false;
+        if (r != 2 && r != 4 && r != 8) return false;
     }
     return true;
 }

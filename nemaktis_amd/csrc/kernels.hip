@@ -2218,7 +2218,7 @@ void launch_bulk_plane(const C2<T> *F, const Geo &g, const SlotParams *sp, const
 bool fft_supported(int n) {
     if (n < 1) return false;
     int m = n;
-    for (int r : {4, 2, 3, 5, 7, 11, 13})
+    for (int r : {8, 4, 2, 3, 5, 7, 11, 13})
         while (m % r == 0) m /= r;
     return m == 1;
 }
@@ -2233,7 +2233,7 @@ FftPlan make_fft_plan(int n) {
     }
     int m = n;
     pl.nst = 0;
-    for (int r : {4, 2, 3, 5, 7, 11, 13})
+    for (int r : {8, 4, 2, 3, 5, 7, 11, 13})
         while (m % r == 0 && m > 1) {
             if (pl.nst >= 16) throw std::runtime_error("FFT plan too deep");
             pl.radices |= (uint64_t)r << (4 * pl.nst++);
