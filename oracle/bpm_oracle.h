@@ -23,13 +23,18 @@
  * march by x,y-invariant media (products of 2x2 matrices); Fresnel by the
  * textbook isotropic transmission; the shift by translation and linear
  * precision; the FFT by numpy; the screen filter by the NA low-pass, a single
- * layer at normal incidence and the angular-spectrum propagator.
+ * layer at normal incidence and the angular-spectrum propagator; Q-tensor
+ * input by the Q of a director reproducing director mode.  The assembled
+ * march as a whole (tests/test_physics_uniaxial.py): a Gaussian beam through
+ * a uniform tilted uniaxial crystal converges to the exact plane-wave
+ * eigenmode propagation (the reference's own accuracy benchmark), walk-off
+ * included.
  * Parity unpinned (no reference vectors and no closed form): the bit-level
  * composition of the R assembly in inhomogeneous media with walk-off, the
- * reference's Eigen summation order and FFTW rounding, the mask-formula
- * bisection weights, the outcome of the reference's racy W loop (the full
- * sum is computed here) and the seam rule (+x/+y neighbours read the
- * duplicate column/row), which is followed from reading the reference only.
+ * reference's Eigen summation order and FFTW rounding, the outcome of the
+ * reference's racy W loop (the full sum is computed here) and the seam rule
+ * (+x/+y neighbours read the duplicate column/row), which is followed from
+ * reading the reference only.
  */
 #ifndef NEMAKTIS_BPM_ORACLE_H
 #define NEMAKTIS_BPM_ORACLE_H

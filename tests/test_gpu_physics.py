@@ -1,0 +1,57 @@
+"""The HIP march against Maxwell's equations: the reference's tilted uniaxial
+accuracy benchmark (HighLevelPythonInterface/examples/benchmarks/3D/
+01_run_bpm_benchmarks.py:112-196), run through the drop-in call with the bulk
+output active; the exit plane is compared with the exact propagation of the
+march's own entrance plane (``exact_uniaxial``; the same measure as
+test_physics_uniaxial).  At N = 81 the planes must also equal the oracle's
+(FP64 1e-10, FP32 1e-4 relative L2); at N = 161 the mean paraxial error falls
+to about 0.025 (x-polarised) / 0.020 (y-polarised) and the extraordinary beam's
+walk-off centroid comes within 0.01 um of the exact 1.51 um.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from nemaktis_amd import bpm_backend, fields, vti
+from test_physics_uniaxial import _case, march_errors
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_bulk(N, precision, folder):
+    s, lc, d = _case(N)
+    s["Algorithm settings"]["General"]["Results folder name"] = folder
+    s["Postprocessor settings"]["Bulk output"] = {"Activate": True, "Base name": "bulk"}
+    print("[gpu case] tilted uniaxial N=%d precision %d" % (N, precision), flush=True)
+    E = np.zeros(fields.output_shape(s), np.complex128)
+    bpm_backend.run_backend_without_mask(json.dumps(s), lc, E, strict=True, precision=precision)
+    g = vti.bulk_fields(os.path.join(folder, "bulk.vti"))[(0, 0)]    # (pol, Nz, Ny, Nx, 2)
+    return s, lc, d, np.moveaxis(g, -1, 2)[None, None]               # (1, 1, pol, Nz, 2, Ny, Nx)
+
+
+@pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 1e-4)])
+def test_tilted_uniaxial_81_against_oracle_and_exact(tmp_path, precision, tol):
+    s, lc, d, bulk = _gpu_bulk(81, precision, str(tmp_path))
+    _, want = oracle.propagate_bulk(s, lc)
+    for pol in (0, 1):
+        err = np.linalg.norm(bulk[0, 0, pol] - want[0, 0, pol]) / np.linalg.norm(want[0, 0, pol])
+        assert err < tol, (pol, err)
+    e = march_errors(bulk, 81, d)
+    print(e)
+    for pol in (0, 1):
+        assert e[pol][0] < 0.06 and e[pol][1] < 0.08
+    assert abs(e[0][2] - e[0][3]) < 0.05 and 1.4 < e[0][3] < 1.6
+    assert abs(e[1][2]) < 0.05
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_tilted_uniaxial_161_against_exact(tmp_path, precision):
+    _, _, d, bulk = _gpu_bulk(161, precision, str(tmp_path))
+    e = march_errors(bulk, 161, d)
+    print(e)
+    assert e[0][0] < 0.035 and e[1][0] < 0.03
+    assert abs(e[0][2] - e[0][3]) < 0.02
+    assert abs(e[1][2]) < 0.03
