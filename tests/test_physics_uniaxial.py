@@ -13,8 +13,10 @@ march leaves out (01_run_bpm_benchmarks.py:133-137).  Measured (mean paraxial
 error as the reference defines it, x- / y-polarised input): N = 21 0.318 /
 0.263, N = 41 0.114 / 0.086, N = 81 0.041 / 0.033, N = 161 0.025 / 0.020; the
 x-polarised (extraordinary) beam walks off by 1.19 / 1.42 / 1.48 / 1.50 um
-against the exact 1.51 um.  A wrong sign or factor in the walk-off terms moves
-that centroid by micrometres; a wrong diffraction term breaks the convergence.
+against the exact 1.51 um.  On N x 1 x N meshes (one transverse axis) the
+errors fall to about 0.01 at N = 161.  A wrong sign or factor in the walk-off
+terms moves that centroid by micrometres; a wrong diffraction term breaks the
+convergence.
 """
 import numpy as np
 import pytest
@@ -27,27 +29,32 @@ NO, NE, WL, L, THETA = 1.5, 1.75, 0.6, 10.0, np.pi / 4
 P_MAX = 0.4
 
 
-def _case(N):
+def _case(N, Ny=None, phi=0.0):
+    """N x Ny x N mesh (Ny = N unless given; Ny = 1 is a mesh in the x-z plane) of the 10 um cube, optical
+    axis at THETA from z with azimuth phi."""
+    Ny = N if Ny is None else Ny
     d = L / (N - 1)
-    s = fields.make_settings((N, N, N), (d, d, d), [WL], no=str(NO), ne=str(NE), nhost=str(NO), nin=str(NO),
+    s = fields.make_settings((N, Ny, N), (d, d, d), [WL], no=str(NO), ne=str(NE), nhost=str(NO), nin=str(NO),
                              beam_profile="GaussianBeam(0.5)", n_woodbury=4, na_objective=0.4)
-    lc = np.zeros((N, N, N, 3))
-    lc[..., 0] = np.sin(THETA)
+    lc = np.zeros((N, Ny, N, 3))
+    lc[..., 0] = np.sin(THETA) * np.cos(phi)
+    lc[..., 1] = np.sin(THETA) * np.sin(phi)
     lc[..., 2] = np.cos(THETA)
     return s, lc, d
 
 
-def march_errors(bulk, N, d):
+def march_errors(bulk, N, d, phi=0.0):
     """(mean paraxial error, relative L2 error, centroid x of the march, exact centroid x) per input
-    polarisation, from the march's planes (n_wl, n_q, 2, Nz, 2, Ny, Nx)."""
+    polarisation, from the march's planes (n_wl, n_q, 2, Nz, 2, Ny, Nx) of a _case mesh."""
     k0 = 2 * np.pi / WL
-    eps = ex.uniaxial_eps(NO, NE, THETA)
-    M = N - 1  # the march grid: the last column and row repeat the first
+    eps = ex.uniaxial_eps(NO, NE, THETA, phi)
+    Ny = bulk.shape[-2]
+    M, My = N - 1, (Ny - 1 if Ny > 1 else 1)  # the march grid: the last column and row repeat the first
     x = (np.arange(M) - M / 2) * d
     out = []
     for pol in (0, 1):
-        E0 = np.moveaxis(bulk[0, 0, pol, 0], 0, -1)[:M, :M]
-        EL = np.moveaxis(bulk[0, 0, pol, N - 1], 0, -1)[:M, :M]
+        E0 = np.moveaxis(bulk[0, 0, pol, 0], 0, -1)[:My, :M]
+        EL = np.moveaxis(bulk[0, 0, pol, N - 1], 0, -1)[:My, :M]
         Fz, _, mask = ex.propagate(E0, d, d, WL, eps, L, P_MAX)
         Fz = Fz * np.exp(-1j * k0 * NO * L)
         Fb = np.fft.fft2(EL, axes=(0, 1))
@@ -133,6 +140,23 @@ def test_oracle_converges_to_the_exact_tilted_uniaxial_beam():
     assert 1.4 < exact < 1.6
     assert abs(cx[2] - exact) < 0.05 and abs(cx[2] - exact) < abs(cx[1] - exact) < abs(cx[0] - exact)
     assert all(abs(errs[N][1][2]) < 0.05 for N in (21, 41, 81))
+
+
+def test_oracle_converges_on_a_mesh_in_the_xz_plane():
+    """The benchmark on N x 1 x N meshes (one transverse axis, 2-D in the reference's benchmark folder), optical
+    axis tilted in the mesh plane (walk-off along x) and across it: errors fall to about 1 %."""
+    for phi in (0.0, np.pi / 2):
+        errs = {}
+        for N in (41, 81, 161):
+            s, lc, d = _case(N, Ny=1, phi=phi)
+            _, bulk = oracle.propagate_bulk(s, lc)
+            errs[N] = march_errors(bulk, N, d, phi)
+            print(phi, N, errs[N])
+        for pol in (0, 1):
+            mean = [errs[N][pol][0] for N in (41, 81, 161)]
+            assert mean[0] > mean[1] > mean[2] and mean[2] < 0.02, (phi, pol, mean)
+        if phi == 0.0:
+            assert abs(errs[161][0][2] - errs[161][0][3]) < 0.02 and 1.4 < errs[161][0][3] < 1.6
 
 
 @pytest.mark.parametrize("planted", ["walk_off_sign"])
