@@ -297,9 +297,38 @@ def droplet_leg(steps, warmup):
             ctx.propagate()
         ms = (time.perf_counter() - t0) * 1e3 / steps
     updates = fields.updates_per_run(s)
-    return {"workload": "droplet 256x256x80, 21 wavelengths x 7 wavevectors x 2 polarisations, FP32, graph, "
-                        "built-in shapes", "ms_per_step": ms, "value": updates / ms * 1e3,
-            "unit": "voxel-slice updates/s", "steps": steps, "warmup": warmup}
+    out = {"workload": "droplet 256x256x80, 21 wavelengths x 7 wavevectors x 2 polarisations, FP32, graph, "
+                       "built-in shapes", "ms_per_step": ms, "value": updates / ms * 1e3,
+           "unit": "voxel-slice updates/s", "steps": steps, "warmup": warmup}
+    # roofline of the batch's dominant march kernel: HIP-event launch times of an eager pass, the same
+    # per-update algorithmic bytes as the headline (64 B of FP32 field traffic + 24 B of director per slot,
+    # shared by the slot's 2 Nq propagations) times the updates one launch advances
+    with bpm_backend.Context(json.dumps(s), lc, n_E, mask_vals=mask, mask_formula=formula or "", precision=32,
+                             wl_batch=0, profile=True) as ctx:
+        ctx.propagate()
+        st = ctx.propagate()
+    fam_ms, fam_n = st["family_ms"], st["family_launches"]
+    march = [f for f in FAMILIES if f not in ("setup", "output", "filter") and fam_n.get(f)]
+    dom = max(march, key=lambda f: fam_ms[f])
+    avg_ms = fam_ms[dom] / fam_n[dom]
+    co = s["Physics settings"]["Coefficients"]
+    nq = len(co["Wavevectors"]) // 2
+    Nx, Ny, _ = s["Physics settings"]["Initial conditions"]["Mesh dimensions"]
+    nprops = len(co["Wavelengths"]) * nq * 2
+    per_update = 64 + 24.0 / (2 * nq)
+    algo = per_update * nprops * Nx * Ny
+    achieved = algo / (avg_ms * 1e-3) / 1e9
+    traffic, src = measured_traffic(KERNELS.get(dom, dom),
+                                    os.path.join(ROOT, "profiles", "hbm_traffic_droplet256_fp32.txt"))
+    out["roofline"] = {"bound": "hbm", "kernel": KERNELS.get(dom, dom), "family": dom,
+                       "avg_launch_ms": avg_ms, "launches": fam_n[dom], "achieved": round(achieved, 2),
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                       "traffic": traffic, "traffic_source": src, "algorithmic_bytes_per_launch": algo,
+                       "algorithmic_basis": "%.1f B per voxel-slice update x %d updates per launch "
+                                            "(%d propagations x %dx%d)" % (per_update, nprops * Nx * Ny, nprops,
+                                                                           Nx, Ny),
+                       "step_frac": round(updates * per_update / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+    return out
 
 
 def run_gpu_leg(js, lc, n_E, shape, precision, wl_batch, use_graph, steps, warmup, tuning=None):
