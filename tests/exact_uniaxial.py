@@ -82,3 +82,16 @@ def propagate(E0, dx, dy, wavelength, eps, z, p_max):
     Fz[mask] = np.einsum("mij,mj->mi", T, c * np.exp(1j * k0 * s * z))
     F0m = np.where(mask[..., None], F0, 0)
     return Fz, F0m, mask
+
+
+def twisted_nematic_jones(no, ne, wavelength, thickness, twist):
+    """Jones matrix, relative to the exp(i k0 no z) carrier, of a planar twisted nematic (director
+    (cos qz, sin qz, 0), q = twist / thickness) for normally incident plane waves: in the frame turning with
+    the director, E' = R(-qz) E obeys dE'/dz = (i k0 (ne - no) e_x e_x^T - q J) E' with J the rotation
+    generator, a constant matrix (the Gooch-Tarry solution)."""
+    import scipy.linalg
+    k0 = 2 * np.pi / wavelength
+    q = twist / thickness
+    A = 1j * k0 * (ne - no) * np.array([[1.0, 0.0], [0.0, 0.0]]) - q * np.array([[0.0, -1.0], [1.0, 0.0]])
+    R = np.array([[np.cos(twist), -np.sin(twist)], [np.sin(twist), np.cos(twist)]])
+    return R @ scipy.linalg.expm(thickness * A)

@@ -7,7 +7,9 @@ test_physics_uniaxial).  At N = 81 the planes must also equal the oracle's
 (FP64 1e-10, FP32 1e-4 relative L2); at N = 161 the mean paraxial error falls
 to about 0.025 (x-polarised) / 0.020 (y-polarised) and the extraordinary beam's
 walk-off centroid comes within 0.01 um of the exact 1.51 um; on an N x 1 x N
-mesh (one transverse axis) it is about 0.01 at N = 161.
+mesh (one transverse axis) it is about 0.01 at N = 161.  The twisted nematic
+with a uniform beam converges to the exact Jones matrix of its twist at second
+order, as the oracle does.
 """
 import json
 import os
@@ -17,7 +19,7 @@ import pytest
 
 import oracle
 from nemaktis_amd import bpm_backend, fields, vti
-from test_physics_uniaxial import _case, march_errors
+from test_physics_uniaxial import _case, march_errors, tn_case, tn_error
 
 pytestmark = pytest.mark.gpu
 
@@ -71,3 +73,24 @@ def test_tilted_uniaxial_xz_plane_mesh(tmp_path, precision, tol):
     print(e)
     assert e[0][0] < 0.02 and e[1][0] < 0.02
     assert abs(e[0][2] - e[0][3]) < 0.02
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_twisted_nematic_jones_convergence(tmp_path, precision):
+    """Uniform beam through the 90-degree twisted nematic: the exit field against the exact Jones matrix of the
+    continuous twist at Nz = 41 and 81 (the oracle's errors 0.00172 and 0.000431, second order)."""
+    errs = []
+    for Nz in (41, 81):
+        s, lc, U = tn_case(Nz)
+        folder = str(tmp_path / ("tn%d" % Nz))
+        s["Algorithm settings"]["General"]["Results folder name"] = folder
+        s["Postprocessor settings"]["Bulk output"] = {"Activate": True, "Base name": "bulk"}
+        print("[gpu case] twisted nematic Nz=%d precision %d" % (Nz, precision), flush=True)
+        E = np.zeros(fields.output_shape(s), np.complex128)
+        bpm_backend.run_backend_without_mask(json.dumps(s), lc, E, strict=True, precision=precision)
+        g = vti.bulk_fields(os.path.join(folder, "bulk.vti"))[(0, 0)]
+        err, dev = tn_error(np.moveaxis(g, -1, 2)[None, None], U)
+        assert dev < (1e-13 if precision == 64 else 1e-5)
+        errs.append(err)
+    print(errs)
+    assert abs(errs[0] - 0.0017246) < 2e-6 and abs(errs[1] - 0.00043127) < 2e-6
