@@ -553,11 +553,14 @@ private:
         return sync_evs_[i];
     }
 
-    // The z-march.  Coefficient planes of slab k+1 are computed on a second
-    // stream (double-buffered) while slab k marches; the slab boundary
-    // kernel joins the two streams.  Each join costs the march ~10 us at the
-    // boundary, so small slabs may take the coefficients on the march stream
-    // instead (coef_serial, chosen by measurement).
+    // The z-march.  Coefficient planes are double-buffered.  On the march
+    // stream (coef_serial, the built-in choice) slab k+1's are computed right
+    // before the boundary between slabs k and k+1, the first kernel that reads
+    // them, so that the slab marches while its planes are still in the
+    // Infinity Cache (512^2 FP64: 155 MB per slab, 26.1 -> 24.8 ms per step
+    // against computing them a slab ahead).  On a second stream (an autotuner
+    // option) slab k+2's are computed while slab k marches, and the boundary
+    // joins the two streams (about 10 us per join).
     C2<T> *march_launches(int nslots, int nprops) {
         const int nsl = p_.Nz - 1, nw = p_.n_woodbury;
         const size_t cfsz = (size_t)sz_.nslots_max * cf_slot_size(g_.Mx, g_.My);
@@ -593,7 +596,7 @@ private:
         if (overlap) HIPCHK(hipStreamWaitEvent(cs, record(st_), 0));
         std::vector<hipEvent_t> ready(nsl, nullptr);
         ready[0] = coef(0);
-        if (nsl > 1) ready[1] = coef(1);
+        if (nsl > 1 && overlap) ready[1] = coef(1);
 
         C2<T> *cur = fb_[1].ptr, *other = fb_[0].ptr, *A = fb_[2].ptr, *B = fb_[3].ptr;
         // XCD work order (measured): a lone polarisation pair shares L2 best
@@ -658,15 +661,19 @@ private:
             }
             // P_k, Sh | Sh, P_{k+1}
             if (k + 1 < nsl) {
+                // coefficients on the march stream: slab k+1's are computed here, just
+                // before the boundary and the slab that read them (so they are still in
+                // the Infinity Cache), into the buffer of slab k-1, whose last reader has run
+                if (!overlap) coef(k + 1);
                 if (ready[k + 1]) HIPCHK(hipStreamWaitEvent(st_, ready[k + 1], 0));
                 timed(KF_BOUNDARY, [&] { launch_boundary<T>(cur, other, CF(k), CF(k + 1), g_, pp_.ptr, nprops, 2, unshifted_[2], st_); });
             } else {
                 timed(KF_BOUNDARY, [&] { launch_boundary<T>(cur, other, CF(k), nullptr, g_, pp_.ptr, nprops, 1, unshifted_[1], st_); });
             }
             std::swap(cur, other);
-            if (k + 2 < nsl) {
+            if (overlap && k + 2 < nsl) {
                 // CF(k) and XT(k) are free once the boundary above has run
-                if (overlap) HIPCHK(hipStreamWaitEvent(cs, record(st_), 0));
+                HIPCHK(hipStreamWaitEvent(cs, record(st_), 0));
                 ready[k + 2] = coef(k + 2);
             }
         }
