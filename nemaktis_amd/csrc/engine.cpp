@@ -92,6 +92,7 @@ bool identity_taps(const double *w, int o) {
 struct Sizes {
     int nwl, nq, batch, nslots_max, nprops_max;
     int64_t Nfull, NP;
+    int coef_batch, ring;  // slabs per coefficient launch, slab buffers of coefficients (2 x coef_batch)
 };
 
 Sizes sizes_of(const Problem &p, const Options &o) {
@@ -104,6 +105,23 @@ Sizes sizes_of(const Problem &p, const Options &o) {
     s.Nfull = (int64_t)p.Nx * p.Ny;
     int Mx = p.Nx > 3 ? p.Nx - 1 : p.Nx, My = p.Ny > 3 ? p.Ny - 1 : p.Ny;
     s.NP = (int64_t)Mx * My;
+    // slabs per coefficient launch (on the march stream): the launch of a small
+    // slab is a fraction of one round of blocks, and several slabs per launch
+    // fill the device (64^2 x 20: 1.26 -> 1.13 ms per step with 8).  But a
+    // batch is computed ahead of its slabs, and the solvers depend on reading
+    // planes that are still in the Infinity Cache: batches of 156 MB already
+    // slowed the 512^2 FP32 step by 3.5 %, so a batch holds at most 32 MB of
+    // coefficients (and at most 8 slabs, about 8192 blocks).
+    const int64_t blocks = slab_coef_tiles(Mx, My) * s.nslots_max;
+    const int64_t slab_bytes = (int64_t)s.nslots_max * cf_slot_size(Mx, My) * (o.precision == 32 ? 8 : 16);
+    int B = 1;
+    if (o.coef_batch > 0) {
+        B = o.coef_batch;
+    } else {
+        while (B < 8 && 2 * B * blocks <= 8192 && 2 * B * slab_bytes <= (int64_t(32) << 20)) B *= 2;
+    }
+    s.coef_batch = std::max(1, std::min({B, std::max(1, p.Nz - 1), (int)(65535 / std::max(1, s.nslots_max))}));
+    s.ring = 2 * s.coef_batch;
     return s;
 }
 
@@ -172,7 +190,7 @@ public:
         sp_.alloc(sz_.nslots_max);
         pp_.alloc(sz_.nprops_max);
         partials_.alloc((size_t)kNrefBlocks * sz_.nslots_max);
-        cf_.alloc(2 * (size_t)sz_.nslots_max * cf_slot_size(g_.Mx, g_.My));
+        cf_.alloc((size_t)sz_.ring * sz_.nslots_max * cf_slot_size(g_.Mx, g_.My));
         set_tuning(o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, g_.Mx, g_.My, sz_.nprops_max));
         for (auto &b : fb_) b.alloc((size_t)sz_.nprops_max * 2 * sz_.NP);
         if (!dev_in) E_.alloc((size_t)sz_.nwl * sz_.nq * 4 * sz_.Nfull);
@@ -553,20 +571,20 @@ private:
         return sync_evs_[i];
     }
 
-    // The z-march.  Coefficient planes are double-buffered.  On the march
-    // stream (coef_serial, the built-in choice) slab k+1's are computed right
-    // before the boundary between slabs k and k+1, the first kernel that reads
-    // them, so that the slab marches while its planes are still in the
-    // Infinity Cache (512^2 FP64: 155 MB per slab, 26.1 -> 24.8 ms per step
-    // against computing them a slab ahead).  On a second stream (an autotuner
-    // option) slab k+2's are computed while slab k marches, and the boundary
-    // joins the two streams (about 10 us per join).
+    // The z-march.  Coefficient planes live in a ring of sz_.ring slab
+    // buffers.  On the march stream (coef_serial, the built-in choice) each
+    // batch of sz_.coef_batch slabs is computed in one launch right before the
+    // boundary that first reads it, so that its slabs march while their planes
+    // are still in the Infinity Cache (512^2 FP64: 155 MB per slab, 26.1 ->
+    // 24.8 ms per step against computing them a slab ahead).  On a second
+    // stream (an autotuner option) slab k+2's are computed while slab k
+    // marches, and the boundary joins the two streams (about 10 us per join).
     C2<T> *march_launches(int nslots, int nprops) {
-        const int nsl = p_.Nz - 1, nw = p_.n_woodbury;
+        const int nsl = p_.Nz - 1, nw = p_.n_woodbury, R = sz_.ring, CB = sz_.coef_batch;
         const size_t cfsz = (size_t)sz_.nslots_max * cf_slot_size(g_.Mx, g_.My);
-        const size_t xtsz = xt_.count / 2;
-        auto CF = [&](int k) { return cf_.ptr + (size_t)(k % 2) * cfsz; };
-        auto XT = [&](int k) { return xt_.ptr ? xt_.ptr + (size_t)(k % 2) * xtsz : nullptr; };
+        const size_t xtsz = xt_.count / R;
+        auto CF = [&](int k) { return cf_.ptr + (size_t)(k % R) * cfsz; };
+        auto XT = [&](int k) { return xt_.ptr ? xt_.ptr + (size_t)(k % R) * xtsz : nullptr; };
         const bool overlap = st2_ != nullptr && !tune_.coef_serial;
         hipStream_t cs = overlap ? st2_ : st_;
         size_t evn = 0;
@@ -577,10 +595,15 @@ private:
         };
         // the event that slab k's coefficients are ready (only when they run on
         // the second stream; on the march stream the order is the stream's)
-        auto coef = [&](int k) {
+        // coefficients of slabs k0 .. k0+n-1 in one launch on stream s
+        auto coefs = [&](int k0, int n, hipStream_t s) {
             timed(KF_COEFS, [&] {
-                launch_slab_coefs<T>(lc_p_, mask_p_, w_p_, g_, sp_.ptr, nslots, k, CF(k), XT(k), lpx_, cs);
+                launch_slab_coefs<T>(lc_p_, mask_p_, w_p_, g_, sp_.ptr, nslots, k0, n, cf_.ptr, (int64_t)cfsz,
+                                     xt_.ptr, (int64_t)xtsz, R, lpx_, s);
             });
+        };
+        auto coef = [&](int k) {
+            coefs(k, 1, cs);
             return overlap ? record(cs) : (hipEvent_t) nullptr;
         };
         if (nsl == 0) {
@@ -595,8 +618,12 @@ private:
         }
         if (overlap) HIPCHK(hipStreamWaitEvent(cs, record(st_), 0));
         std::vector<hipEvent_t> ready(nsl, nullptr);
-        ready[0] = coef(0);
-        if (nsl > 1 && overlap) ready[1] = coef(1);
+        if (overlap) {
+            ready[0] = coef(0);
+            if (nsl > 1) ready[1] = coef(1);
+        } else {
+            coefs(0, std::min(CB, nsl), st_);
+        }
 
         C2<T> *cur = fb_[1].ptr, *other = fb_[0].ptr, *A = fb_[2].ptr, *B = fb_[3].ptr;
         // XCD work order (measured): a lone polarisation pair shares L2 best
@@ -661,10 +688,11 @@ private:
             }
             // P_k, Sh | Sh, P_{k+1}
             if (k + 1 < nsl) {
-                // coefficients on the march stream: slab k+1's are computed here, just
-                // before the boundary and the slab that read them (so they are still in
-                // the Infinity Cache), into the buffer of slab k-1, whose last reader has run
-                if (!overlap) coef(k + 1);
+                // coefficients on the march stream: the batch that starts at slab k+1 is
+                // computed here, just before the boundary and the slabs that read it (so
+                // it is still in the Infinity Cache), into the buffers of the batch before
+                // the current one, whose last reader has run
+                if (!overlap && (k + 1) % CB == 0) coefs(k + 1, std::min(CB, nsl - k - 1), st_);
                 if (ready[k + 1]) HIPCHK(hipStreamWaitEvent(st_, ready[k + 1], 0));
                 timed(KF_BOUNDARY, [&] { launch_boundary<T>(cur, other, CF(k), CF(k + 1), g_, pp_.ptr, nprops, 2, unshifted_[2], st_); });
             } else {
@@ -705,7 +733,7 @@ private:
             throw std::runtime_error("y line solver tuning exceeds the block limits");
         drop_graphs();
         if (g_.px) {
-            const size_t need = 2 * (size_t)sz_.nslots_max * 2 * 3 * g_.My * lx.lmax * lx.P;
+            const size_t need = (size_t)sz_.ring * sz_.nslots_max * 2 * 3 * g_.My * lx.lmax * lx.P;
             if (need > xt_.count) {
                 DevBuf<C2<T>> grown;
                 grown.source = xt_.source;
@@ -831,11 +859,11 @@ int64_t estimate_footprint(const Problem &p, const Options &o) {
     add((int64_t)s.nslots_max * (int64_t)sizeof(SlotParams));
     add((int64_t)s.nprops_max * (int64_t)sizeof(PropParams));
     add((int64_t)kNrefBlocks * s.nslots_max * 8);
-    add(2 * (int64_t)s.nslots_max * cf_slot_size(Mx, My) * fsz);
+    add((int64_t)s.ring * s.nslots_max * cf_slot_size(Mx, My) * fsz);
     if (p.Nx > 3) {
         const Tuning t = o.tuning.x_chunks > 0 ? o.tuning : default_tuning(o.precision, Mx, My, s.nprops_max);
         const LinePlan lp = make_line_plan(Mx, t.x_chunks);
-        add(2 * (int64_t)s.nslots_max * 2 * 3 * My * lp.lmax * lp.P * fsz);
+        add((int64_t)s.ring * s.nslots_max * 2 * 3 * My * lp.lmax * lp.P * fsz);
     }
     for (int i = 0; i < 4; i++) add((int64_t)s.nprops_max * 2 * s.NP * fsz);
     if (!dev_in) add((int64_t)s.nwl * s.nq * 4 * s.Nfull * 16);

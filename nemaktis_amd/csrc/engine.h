@@ -58,6 +58,7 @@ struct Options {
     DeviceIO dev;                // dev.lc set: inputs and output are the caller's device arrays
     bool no_timing = false;      // no HIP events (stage times stay 0)
     bool no_side_stream = false; // no second stream (coefficients on the march stream)
+    int coef_batch = 0;          // slabs per coefficient launch on the march stream (0: by size)
 };
 
 // per-kernel-family device time of one propagate() (profile mode); the slab

@@ -20,13 +20,18 @@ void launch_interface_weights(const ExprProgram &prog, const uint8_t *mask, int 
 void launch_nref(const double *lc, const uint8_t *mask, const Geo &g, SlotParams *sp, int nslots, double *partials,
                  int nblocks, hipStream_t st);
 
-// slab material of slab iz (FP64) and from it the mu*R stencil coefficients
-// and the phase operator on the march grid, for every slot (xt: the x-line
-// tridiagonal rows in the chunk-interleaved layout of lpx)
+// blocks of one slot's coefficient launch per slab (march-grid tiles)
+int64_t slab_coef_tiles(int Mx, int My);
+
+// slab material of slabs iz0 .. iz0+nz-1 (FP64) and from it the mu*R stencil
+// coefficients and the phase operator on the march grid, for every slot, in
+// one launch; slab iz writes buffer iz % ring of cf (cfstride elements per
+// buffer) and of xt (the x-line tridiagonal rows in the chunk-interleaved
+// layout of lpx, xtstride elements per buffer; xt may be null)
 template <class T>
 void launch_slab_coefs(const double *lc, const uint8_t *mask, const double *weights, const Geo &g,
-                       const SlotParams *sp, int nslots, int iz, C2<T> *cf, C2<T> *xt, const LinePlan &lpx,
-                       hipStream_t st);
+                       const SlotParams *sp, int nslots, int iz0, int nz, C2<T> *cf, int64_t cfstride, C2<T> *xt,
+                       int64_t xtstride, int ring, const LinePlan &lpx, hipStream_t st);
 
 // input beam + input Fresnel (beam != None) or packing of caller fields
 template <class T>

@@ -290,20 +290,31 @@ __device__ __forceinline__ int wrap_index(int v, int M) {
     return v < 0 ? v + M : v;
 }
 
-template <class T>
+// BATCH: one launch computes slabs iz0 .. iz0 + gridDim.y / nslots - 1
+// (blockIdx.y = slab offset * nslots + slot), slab iz writing buffer iz % ring
+// of the ring cf (cfstride elements per buffer) and of the x-line rows xt
+// (xtstride); otherwise slab iz0 into cf and xt as given (blockIdx.y = slot).
+template <class T, bool BATCH>
 __global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ lc, const uint8_t *__restrict__ mask,
                                                     const double *__restrict__ weights, Geo g,
-                                                    const SlotParams *__restrict__ sp, int iz,
-                                                    C2<T> *__restrict__ cf, C2<T> *__restrict__ xt, LinePlan lpx) {
+                                                    const SlotParams *__restrict__ sp, int iz0, int nslots,
+                                                    C2<T> *__restrict__ cf, int64_t cfstride,
+                                                    C2<T> *__restrict__ xt, int64_t xtstride, int ring,
+                                                    LinePlan lpx) {
     __shared__ double mat[SM_COUNT * kMatHalo];
     __shared__ double seam_x[kMatTy], seam_y[kMatTx];
     const int ntx = (g.Mx + kMatTx - 1) / kMatTx, nty = (g.My + kMatTy - 1) / kMatTy;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= ntx * nty) return;
     const int ix0 = (tile % ntx) * kMatTx, iy0 = (tile / ntx) * kMatTy;
-    const int slot = blockIdx.y;
+    const int slot = BATCH ? (int)blockIdx.y % nslots : (int)blockIdx.y;
+    const int iz = BATCH ? iz0 + (int)blockIdx.y / nslots : iz0;
     const SlotParams s = sp[slot];
     const int64_t N = g.Nfull;
+    if (BATCH) {
+        cf += (iz % ring) * cfstride;
+        if (xt) xt += (iz % ring) * xtstride;
+    }
     C2<T> *cfs = cf + (size_t)slot * cf_slot_size(g.Mx, g.My);
 
     // material of the tile and its halo (periodic neighbours on the march grid)
@@ -506,13 +517,26 @@ __global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ l
     store_mu_r<T, CF_XY, CF_NMU>(o, oi, re, im, hz);
 }
 
+int64_t slab_coef_tiles(int Mx, int My) {
+    return (int64_t)((Mx + kMatTx - 1) / kMatTx) * ((My + kMatTy - 1) / kMatTy);
+}
+
 template <class T>
 void launch_slab_coefs(const double *lc, const uint8_t *mask, const double *weights, const Geo &g,
-                       const SlotParams *sp, int nslots, int iz, C2<T> *cf, C2<T> *xt, const LinePlan &lpx,
-                       hipStream_t st) {
-    const int64_t tiles = (int64_t)((g.Mx + kMatTx - 1) / kMatTx) * ((g.My + kMatTy - 1) / kMatTy);
-    hipLaunchKernelGGL(k_slab_coefs<T>, dim3((unsigned)((tiles + 7) / 8 * 8), nslots), dim3(256), 0, st, lc, mask,
-                       weights, g, sp, iz, cf, xt, lpx);
+                       const SlotParams *sp, int nslots, int iz0, int nz, C2<T> *cf, int64_t cfstride, C2<T> *xt,
+                       int64_t xtstride, int ring, const LinePlan &lpx, hipStream_t st) {
+    if (nz < 1 || nz > ring || (int64_t)nz * nslots > 65535)
+        throw std::runtime_error("slab coefficients: bad slab batch");
+    const int64_t tiles = slab_coef_tiles(g.Mx, g.My);
+    const dim3 grid((unsigned)((tiles + 7) / 8 * 8), nslots * nz);
+    if (nz == 1) {
+        C2<T> *x1 = xt ? xt + (iz0 % ring) * xtstride : nullptr;
+        hipLaunchKernelGGL((k_slab_coefs<T, false>), grid, dim3(256), 0, st, lc, mask, weights, g, sp, iz0, nslots,
+                           cf + (iz0 % ring) * cfstride, cfstride, x1, xtstride, ring, lpx);
+    } else {
+        hipLaunchKernelGGL((k_slab_coefs<T, true>), grid, dim3(256), 0, st, lc, mask, weights, g, sp, iz0, nslots,
+                           cf, cfstride, xt, xtstride, ring, lpx);
+    }
     NMK_LAUNCH_CHECK();
 }
 
@@ -2525,8 +2549,8 @@ void launch_filter(c64 *E, const Geo &g, const SlotParams *sp, int nslots, const
 // explicit instantiations
 #define NMK_INST(T)                                                                                                   \
     template void launch_slab_coefs<T>(const double *, const uint8_t *, const double *, const Geo &,                \
-                                       const SlotParams *, int, int, C2<T> *, C2<T> *, const LinePlan &,           \
-                                       hipStream_t);                                                                \
+                                       const SlotParams *, int, int, int, C2<T> *, int64_t, C2<T> *, int64_t, int, \
+                                       const LinePlan &, hipStream_t);                                              \
     template void launch_init_fields<T>(const double *, const uint8_t *, const Geo &, const SlotParams *,             \
                                         const PropParams *, int, int, double, double, double, C2<T> *, hipStream_t);  \
     template void launch_pack_fields<T>(const c64 *, const Geo &, const PropParams *, int, int, C2<T> *, hipStream_t); \

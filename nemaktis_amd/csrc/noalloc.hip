@@ -83,6 +83,7 @@ void run_noalloc(const Problem &p, int precision, const DeviceIO &io, hipStream_
     o.dev = io;
     o.no_timing = true;
     o.no_side_stream = true;
+    o.coef_batch = 1;  // two slab buffers of coefficients: the smallest footprint
     const int64_t need = estimate_footprint(p, o);
     if (need > (int64_t)kScratchBytes)
         throw ScratchExhausted("the run needs " + std::to_string(need) +
