@@ -459,3 +459,25 @@ def test_staged_and_direct_host_copies_agree(tmp_path):
                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert r.returncode == 0, r.stdout[-2000:]
     assert np.array_equal(np.load(tmp_path / "E.npy"), staged)
+
+
+def test_batched_slab_coefficients_with_a_partial_last_batch():
+    """A small mesh computes the coefficients of several slabs per launch (a ring of twice the batch): 22 slabs
+    of a 40 x 36 mesh come in batches of 8, 8 and 6, with shifted wavevectors (the shift boundary reads two
+    slabs' phase planes) and the Neumann depth 2: one-shot FP64 (one wavelength at a time) and FP32 (all
+    together), and graph-replaying contexts propagated twice."""
+    s, lc = _random_case((40, 36, 23), seed=29)
+    s["Physics settings"]["Coefficients"]["Wavelengths"] = [0.55, 0.65]
+    s["Physics settings"]["Coefficients"]["Wavevectors"] = [0.0, 0.0, 0.12, -0.05]
+    want = oracle.propagate(s, lc)
+    assert _rel(_gpu(s, lc, precision=64), want) < TOL64
+    assert _rel(_gpu(s, lc, precision=32, wl_batch=0), want) < TOL32
+    n = int(np.prod(fields.output_shape(s)))
+    for prec, tol in ((64, TOL64), (32, TOL32)):
+        print("[gpu case] batched slab coefficients, graph context, precision %d" % prec, flush=True)
+        with bpm_backend.Context(json.dumps(s), lc, n, precision=prec, wl_batch=0, use_graph=True) as ctx:
+            outs = []
+            for _ in range(2):
+                ctx.propagate()
+                outs.append(ctx.download(np.empty(fields.output_shape(s), np.complex128)))
+        assert np.array_equal(outs[0], outs[1]) and _rel(outs[0], want) < tol
