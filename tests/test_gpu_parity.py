@@ -218,7 +218,8 @@ def test_random512_reduced_depth_fp32_vs_fp64():
     assert err < TOL32, err
 
 
-@pytest.mark.parametrize("tuning", [
+# launch shapes of the autotuner's candidate forms on 69 x 65-point lines
+SOLVER_SHAPES = [
     {"x_chunks": 8, "x_rows": 4, "y_chunks": 8, "y_cols": 4},     # chunks of 8-9 points (16-point solver)
     {"x_chunks": 32, "x_rows": 1, "y_chunks": 32, "y_cols": 1},   # chunks of 2-3 points
     {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8},   # chunks of 4-5 points
@@ -228,7 +229,10 @@ def test_random512_reduced_depth_fp32_vs_fp64():
     {"x_chunks": 32, "x_rows": 4, "y_chunks": 16, "y_cols": 8, "x_both": 3},  # four rows of both components
     {"x_chunks": 16, "x_rows": 1, "y_chunks": 16, "y_cols": 8, "x_both": 1},  # both components, first launches only
     {"x_chunks": 16, "x_rows": 2, "y_chunks": 16, "y_cols": 8, "coef_serial": 1},  # coefficients on the march stream
-])
+]
+
+
+@pytest.mark.parametrize("tuning", SOLVER_SHAPES)
 def test_line_solver_shapes_fp64(tuning):
     """Every autotuner candidate shape must give the oracle's answer."""
     s, lc = _random_case((70, 66, 5), seed=17)
