@@ -294,6 +294,34 @@ def test_retuned_context_matches_fresh_context():
     assert np.array_equal(got, want) and np.array_equal(again, want)
 
 
+def test_graph_contexts_return_their_memory():
+    """Graph-mode contexts created, replayed and destroyed one after another
+    give their device memory back: after one warm-up context (the runtime's
+    own one-time graph state), the free memory the runtime reports does not
+    drift over three more."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+
+    def free_bytes():
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        assert hip.hipMemGetInfo(ctypes.byref(f), ctypes.byref(t)) == 0
+        return f.value
+
+    s = _base((128, 128, 8), wl=[0.5, 0.6], no="1.5", ne="1.7", na_objective=0.4)
+    lc = fields.random_smooth(128, 128, 8, (0.1, 0.1, 0.1), seed=3, corr_length=0.3)
+    js, n_E = json.dumps(s), int(np.prod(fields.output_shape(s)))
+    free = []
+    for i in range(4):
+        with bpm_backend.Context(js, lc, n_E, precision=32, wl_batch=0, use_graph=True) as ctx:
+            ctx.propagate()
+            ctx.propagate()
+            held = free_bytes()
+        free.append(free_bytes())
+        assert free[-1] - held > 16 << 20      # the context's buffers came back
+    print("free memory after each context (MB):", [f >> 20 for f in free])
+    assert max(free[1:]) - min(free[1:]) <= 4 << 20
+
+
 def test_xcd_work_orders_are_bitwise_equal():
     """The XCD work orders only move blocks between XCDs: identical output
     (two wavelengths x two wavevectors, so that every index varies; orders 8-15
