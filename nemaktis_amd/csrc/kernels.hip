@@ -36,12 +36,30 @@ static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs
 
 // ---------------------------------------------------------------------------
 // permittivity of one voxel (PhysicsLib/PermittivityTensorField.cpp:18-93):
-// e[0..5] eps xx,yy,zz,xy,xz,yz; e[6..8] K = transverse eps; e[9..11] sqrt(K)
+// e[0..5] eps xx,yy,zz,xy,xz,yz; e[6..8] K = transverse eps; e[9..11] sqrt(K),
+// from the director (fd = 3) or Q-tensor (fd = 6) values of the voxel and its
+// LC flag, fetched by load_voxel
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double eps_voxel(const double *__restrict__ lc, const uint8_t *__restrict__ mask, int fd,
-                                            int64_t v, const SlotParams &s, double *e) {
+struct VoxelData {
+    double n[6];
+    bool m;
+};
+__device__ __forceinline__ void load_voxel(const double *__restrict__ lc, const uint8_t *__restrict__ mask, int fd,
+                                           int64_t v, VoxelData &d) {
     const double *n = lc + (int64_t)fd * v;
-    const bool m = mask ? (mask[v] != 0) : true;
+    d.m = mask ? (mask[v] != 0) : true;
+    d.n[0] = n[0];
+    d.n[1] = n[1];
+    d.n[2] = n[2];
+    if (fd == 6) {
+        d.n[3] = n[3];
+        d.n[4] = n[4];
+        d.n[5] = n[5];
+    }
+}
+__device__ __forceinline__ double eps_of(const VoxelData &d, int fd, const SlotParams &s, double *e) {
+    const double *n = d.n;
+    const bool m = d.m;
     if (fd == 3) {
         double n0 = n[0], n1 = n[1], n2 = n[2];
         e[0] = m ? s.eperp + s.ea * (n0 * n0) : s.ehost;
@@ -76,6 +94,12 @@ __device__ __forceinline__ double eps_voxel(const double *__restrict__ lc, const
     e[10] = neff + (e[7] - e[6]) / (4 * neff);
     e[11] = e[8] / (2 * neff);
     return neff;
+}
+__device__ __forceinline__ double eps_voxel(const double *__restrict__ lc, const uint8_t *__restrict__ mask, int fd,
+                                            int64_t v, const SlotParams &s, double *e) {
+    VoxelData d;
+    load_voxel(lc, mask, fd, v, d);
+    return eps_of(d, fd, s, e);
 }
 
 __global__ __launch_bounds__(256) void k_mask_flags(const double *__restrict__ mask, uint8_t *__restrict__ flags,
@@ -317,14 +341,32 @@ __global__ __launch_bounds__(256) void k_slab_coefs(const double *__restrict__ l
     }
     C2<T> *cfs = cf + (size_t)slot * cf_slot_size(g.Mx, g.My);
 
-    // material of the tile and its halo (periodic neighbours on the march grid)
-    for (int h = threadIdx.x; h < kMatHalo; h += blockDim.x) {
+    // material of the tile and its halo (periodic neighbours on the march
+    // grid): a thread takes halo points t and t + 256 and fetches the voxels
+    // of both before it evaluates either
+    static_assert(kMatHalo <= 2 * 256, "two halo points per thread of a 256-thread block");
+    int64_t hm[2];
+    VoxelData va[2], vb[2];
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int h = (int)threadIdx.x + 256 * r;
+        if (h < kMatHalo) {
+            const int lx = h % kMatHx, ly = h / kMatHx;
+            const int jx = wrap_index(ix0 - 1 + lx, g.Mx), jy = wrap_index(iy0 - 1 + ly, g.My);
+            hm[r] = jx + (int64_t)g.Nx * jy;
+            load_voxel(lc, mask, g.field_dim, hm[r] + N * iz, va[r]);
+            load_voxel(lc, mask, g.field_dim, hm[r] + N * (iz + 1), vb[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const int h = (int)threadIdx.x + 256 * r;
+        if (h >= kMatHalo) break;
         const int lx = h % kMatHx, ly = h / kMatHx;
-        const int jx = wrap_index(ix0 - 1 + lx, g.Mx), jy = wrap_index(iy0 - 1 + ly, g.My);
-        const int64_t m = jx + (int64_t)g.Nx * jy;
+        const int64_t m = hm[r];
         double a[12], b[12], f[SM_COUNT];
-        eps_voxel(lc, mask, g.field_dim, m + N * iz, s, a);
-        eps_voxel(lc, mask, g.field_dim, m + N * (iz + 1), s, b);
+        eps_of(va[r], g.field_dim, s, a);
+        eps_of(vb[r], g.field_dim, s, b);
         slab_fields(a, b, f);
 #pragma unroll
         for (int c = 0; c < SM_COUNT; c++) mat[c * kMatHalo + h] = f[c];
