@@ -734,25 +734,66 @@ __global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, 
     __syncthreads();
     const C *f = in + (size_t)prop0 * 2 * NP;  // planes: pol 0 (x, y), pol 1 (x, y)
     const C *cp = cf_prev ? cf_prev + (size_t)p.slot * cf_slot_size(g.Mx, g.My) + CF_P * kCfTile : nullptr;
-    // input window with the previous slab's phase operator applied
-    for (int e = threadIdx.x; e < WH * WW; e += blockDim.x) {
-        const int r = e / WW, c = e - r * WW;
-        const unsigned q = colw[c] + (unsigned)g.Mx * roww[r];
-        C a0 = f[q], b0 = f[NP + q], a1 = f[2 * NP + q], b1 = f[3 * NP + q];
-        if (cp) {
-            const C *c3 = cp + cf_point(colw[c], roww[r], g.Mx);
-            const C pxx = c3[0], pyy = c3[kCfTile], pxy = c3[2 * kCfTile];
-            const C na0 = pxx * a0 + pxy * b0, nb0 = pxy * a0 + pyy * b0;
-            const C na1 = pxx * a1 + pxy * b1, nb1 = pxy * a1 + pyy * b1;
-            a0 = na0;
-            b0 = nb0;
-            a1 = na1;
-            b1 = nb1;
+    // input window with the previous slab's phase operator applied: a
+    // thread takes elements t, t + 256, ... and fetches all of them (fields
+    // and phase operators) before it evaluates any
+    constexpr int NWE = (WH * WW + 255) / 256;
+    C wf[NWE][4], wp[NWE][3];
+#pragma unroll
+    for (int k = 0; k < NWE; k++) {
+        const int e = threadIdx.x + k * 256;
+        if (e < WH * WW) {
+            const int r = e / WW, c = e - r * WW;
+            const unsigned q = colw[c] + (unsigned)g.Mx * roww[r];
+#pragma unroll
+            for (int pl = 0; pl < 4; pl++) wf[k][pl] = f[pl * NP + q];
+            if (cp) {
+                const C *c3 = cp + cf_point(colw[c], roww[r], g.Mx);
+                wp[k][0] = c3[0];
+                wp[k][1] = c3[kCfTile];
+                wp[k][2] = c3[2 * kCfTile];
+            }
         }
-        tile[0][r][c] = a0;
-        tile[1][r][c] = b0;
-        tile[2][r][c] = a1;
-        tile[3][r][c] = b1;
+    }
+    // the next slab's phase operators at the thread's output points t and
+    // t + 256, fetched with the window
+    const C *cn = cf_next ? cf_next + (size_t)p.slot * cf_slot_size(g.Mx, g.My) + CF_P * kCfTile : nullptr;
+    static_assert(kBTH * kBTW == 2 * 256, "two output points per thread of a 256-thread block");
+    C np3[2][3];
+    if (cn) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int e = threadIdx.x + k * 256;
+            const int r = e / kBTW, c = e - r * kBTW;
+            const int x = x0 + c, y = y0 + r;
+            if (x < g.Mx && y < g.My) {
+                const C *c3 = cn + cf_point(x, y, g.Mx);
+                np3[k][0] = c3[0];
+                np3[k][1] = c3[kCfTile];
+                np3[k][2] = c3[2 * kCfTile];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NWE; k++) {
+        const int e = threadIdx.x + k * 256;
+        if (e < WH * WW) {
+            const int r = e / WW, c = e - r * WW;
+            C a0 = wf[k][0], b0 = wf[k][1], a1 = wf[k][2], b1 = wf[k][3];
+            if (cp) {
+                const C pxx = wp[k][0], pyy = wp[k][1], pxy = wp[k][2];
+                const C na0 = pxx * a0 + pxy * b0, nb0 = pxy * a0 + pyy * b0;
+                const C na1 = pxx * a1 + pxy * b1, nb1 = pxy * a1 + pyy * b1;
+                a0 = na0;
+                b0 = nb0;
+                a1 = na1;
+                b1 = nb1;
+            }
+            tile[0][r][c] = a0;
+            tile[1][r][c] = b0;
+            tile[2][r][c] = a1;
+            tile[3][r][c] = b1;
+        }
     }
     __syncthreads();
     T wx[7], wy[7];
@@ -791,9 +832,10 @@ __global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, 
     }
     __syncthreads();
     // y pass and the next slab's phase operator
-    const C *cn = cf_next ? cf_next + (size_t)p.slot * cf_slot_size(g.Mx, g.My) + CF_P * kCfTile : nullptr;
     C *o = out + (size_t)prop0 * 2 * NP;
-    for (int e = threadIdx.x; e < kBTH * kBTW; e += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int e = threadIdx.x + k * 256;
         const int r = e / kBTW, c = e - r * kBTW;
         const int x = x0 + c, y = y0 + r;
         if (x >= g.Mx || y >= g.My) continue;
@@ -807,8 +849,7 @@ __global__ __launch_bounds__(256) void k_boundary(const C2<T> *__restrict__ in, 
         }
         const unsigned q = x + (unsigned)g.Mx * y;
         if (cn) {
-            const C *c3 = cn + cf_point(x, y, g.Mx);
-            const C pxx = c3[0], pyy = c3[kCfTile], pxy = c3[2 * kCfTile];
+            const C pxx = np3[k][0], pyy = np3[k][1], pxy = np3[k][2];
 #pragma unroll
             for (int pol = 0; pol < 2; pol++) {
                 const C a = pxx * v[2 * pol] + pxy * v[2 * pol + 1], b = pxy * v[2 * pol] + pyy * v[2 * pol + 1];
