@@ -29,12 +29,14 @@
  * a uniform tilted uniaxial crystal converges to the exact plane-wave
  * eigenmode propagation (the reference's own accuracy benchmark), walk-off
  * included.
+ * The seam rule (+x/+y neighbours of the last march points read 1/eps_zz of
+ * the duplicate column/row) is pinned in its structure: the duplicate plane
+ * enters the march through its eps_zz and through nothing else.
  * Parity unpinned (no reference vectors and no closed form): the bit-level
  * composition of the R assembly in inhomogeneous media with walk-off, the
  * reference's Eigen summation order and FFTW rounding, the outcome of the
- * reference's racy W loop (the full sum is computed here) and the seam rule
- * (+x/+y neighbours read the duplicate column/row), which is followed from
- * reading the reference only.
+ * reference's racy W loop (the full sum is computed here) and the seam
+ * coefficient's form, which is followed from reading the reference only.
  */
 #ifndef NEMAKTIS_BPM_ORACLE_H
 #define NEMAKTIS_BPM_ORACLE_H

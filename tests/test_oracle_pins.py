@@ -535,3 +535,49 @@ def test_oracle_single_plane_is_the_two_fresnel_interfaces():
     np.testing.assert_allclose(E[0, 0, 0], t, rtol=1e-13)        # x input -> Ex
     np.testing.assert_allclose(E[0, 0, 3], t, rtol=1e-13)        # y input -> Ey
     assert np.abs(E[0, 0, 1]).max() < 1e-15 and np.abs(E[0, 0, 2]).max() < 1e-13
+
+
+# --------------------------------------------------------------------------- seam rule
+def _tilted(theta, phi):
+    return np.array([math.sin(theta) * math.cos(phi), math.sin(theta) * math.sin(phi), math.cos(theta)])
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+def test_duplicate_plane_enters_the_march_only_through_eps_zz(axis):
+    """The march grid excludes the duplicate column (x) or row (y) of the mesh,
+    but the D1 seam rule reads 1/eps_zz there for the +x / +y neighbour of the
+    last march point (ParaxialPrimaryEvolutionOperator.cpp:428, 430).  In a
+    uniform tilted medium at normal incidence, with a Gaussian beam (a uniform
+    beam has no transverse derivatives for the seam coefficients to act on):
+    turning the duplicate plane's
+    director about z (eps_zz unchanged, every other component changed) leaves
+    every marched bulk plane bit for bit as it was, while tilting it further
+    (eps_zz changed) moves them.  Reading the periodic neighbour instead of the
+    duplicate fails the second check; reading more than eps_zz there fails the
+    first.  Plane 0 is the entrance field after the input interface, which the
+    duplicate points pass with their own permittivity (BPMIteration.cpp:87-122):
+    there only the duplicate plane itself changes."""
+    Nx, Ny, Nz = 9, 8, 4
+    s = _settings((Nx, Ny, Nz), no="1.5", ne="1.7", beam_profile="GaussianBeam(0.3)")
+    base = np.zeros((Nz, Ny, Nx, 3)) + _tilted(0.5, 0.4)
+
+    def dup(n):
+        lc = base.copy()
+        if axis == "x":
+            lc[:, :, Nx - 1, :] = n
+        else:
+            lc[:, Ny - 1, :, :] = n
+        return lc
+
+    _, b0 = oracle.propagate_bulk(s, base)
+    _, b_turned = oracle.propagate_bulk(s, dup(_tilted(0.5, 1.5)))
+    _, b_tilted = oracle.propagate_bulk(s, dup(_tilted(0.8, 0.4)))
+    march = (slice(None),) * 3 + (slice(1, None),)
+    assert np.array_equal(b_turned[march], b0[march])
+    rel = np.linalg.norm(b_tilted[march] - b0[march]) / np.linalg.norm(b0[march])
+    assert rel > 1e-6, rel
+    # plane 0: equal at the march points, the duplicate plane through its own interface
+    for b in (b_turned, b_tilted):
+        assert np.array_equal(b[:, :, :, 0, :, :Ny - 1, :Nx - 1], b0[:, :, :, 0, :, :Ny - 1, :Nx - 1])
+    edge = (Ellipsis, Nx - 1) if axis == "x" else (Ellipsis, Ny - 1, slice(None))
+    assert not np.array_equal(b_turned[:, :, :, 0][edge], b0[:, :, :, 0][edge])
