@@ -24,7 +24,9 @@
  * textbook isotropic transmission; the shift by translation and linear
  * precision; the FFT by numpy; the screen filter by the NA low-pass, a single
  * layer at normal incidence and the angular-spectrum propagator; Q-tensor
- * input by the Q of a director reproducing director mode.  The assembled
+ * input by the Q of a director reproducing director mode; the focus
+ * correction's exit index at 0.6 um by a layer index-matched to a dispersive
+ * exit medium at another wavelength (the angular-spectrum propagator).  The assembled
  * march as a whole (tests/test_physics_uniaxial.py): a Gaussian beam through
  * a uniform tilted uniaxial crystal converges to the exact plane-wave
  * eigenmode propagation (the reference's own accuracy benchmark), walk-off

@@ -423,6 +423,37 @@ def test_filter_index_matched_layer_is_angular_spectrum_propagator():
     np.testing.assert_allclose(got[0], want, atol=1e-11)
 
 
+def test_focus_correction_takes_the_exit_index_at_0_6_um():
+    """The focus correction subtracts sum(e_p * nout(0.6 um) / n_p) over the
+    isotropic layers, at every wavelength (ScreenOutput.cpp:34-38).  A layer
+    index-matched to the exit medium at the propagated wavelength (0.5 um) is
+    more of that medium, so the filter must equal the angular-spectrum
+    propagator over z_foc + e - e * nout(0.6) / nout(0.5); with nout taken at
+    the propagated wavelength the layer would cancel exactly (over z_foc)."""
+    rng = np.random.default_rng(17)
+    nout = "1.3+0.01/lambda^2"
+    n5, n6 = 1.3 + 0.01 / 0.25, 1.3 + 0.01 / 0.36
+    zf, e = 2.5, 1.7
+    Nx, Ny = 16, 16
+    s = _settings((Nx, Ny, 2), sp=(0.3, 0.3, 0.1), wl=(0.5,), no="1.5", ne="1.7", nout=nout, na_objective=0.8,
+                  iso_thicknesses=[e], iso_indices=[n5], z_shift=zf)
+    ses = oracle.Session(s, np.zeros((2, Ny, Nx, 3)) + [0, 0, 1])
+    E = rng.standard_normal((1, 4, Ny, Nx)) + 1j * rng.standard_normal((1, 4, Ny, Nx))
+    got = np.ascontiguousarray(E.copy())
+    ses.filter_wavelength(0, got)
+    k0 = 2 * math.pi / 0.5
+    kx = 2 * np.pi * np.fft.fftfreq(Nx, d=0.3) / k0
+    qq = kx[None, :] ** 2 + kx[:, None] ** 2
+    kz = np.sqrt(np.maximum(n5 * n5 - qq, 0.0))
+
+    def propagated(z):
+        H = np.where(np.sqrt(qq) < 0.8, np.exp(1j * (kz - n5) * k0 * z), 0)
+        return np.fft.ifft2(np.fft.fft2(E[0]) * H)
+
+    np.testing.assert_allclose(got[0], propagated(zf + e - e * n6 / n5), atol=1e-10)
+    assert np.abs(got[0] - propagated(zf)).max() > 1e-3
+
+
 # --------------------------------------------------------------------------- Q-tensor input
 def test_qtensor_of_a_director_reproduces_director_mode():
     """PermittivityTensorField.cpp:41-65 (Q-tensor branch) against :18-40 (director
