@@ -121,3 +121,43 @@ def test_bulk_output_of_a_serving_job(tmp_path):
     assert np.array_equal(j.E, E)
     a, b = vti.bulk_fields(str(tmp_path / "srv" / "job.vti")), vti.bulk_fields(str(tmp_path / "one" / "job.vti"))
     assert sorted(a) == sorted(b) and all(np.array_equal(a[key], b[key]) for key in a)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_duplicate_plane_enters_the_march_only_through_eps_zz(tmp_path, precision):
+    """The HIP march reads the duplicate column of the mesh only for the seam
+    rule's 1/eps_zz (ParaxialPrimaryEvolutionOperator.cpp:428): turning the
+    duplicate column's director about z (eps_zz unchanged) leaves every marched
+    plane bit for bit as it was, tilting it further moves them as the oracle
+    says; plane 0 changes only in the duplicate column (its own input
+    interface)."""
+    import math
+    Nx, Ny, Nz = 9, 8, 4
+
+    def tilted(theta, phi):
+        return np.array([math.sin(theta) * math.cos(phi), math.sin(theta) * math.sin(phi), math.cos(theta)])
+
+    s = fields.make_settings((Nx, Ny, Nz), (0.1, 0.1, 0.1), [0.6], no="1.5", ne="1.7",
+                             beam_profile="GaussianBeam(0.3)")
+    base = np.zeros((Nz, Ny, Nx, 3)) + tilted(0.5, 0.4)
+    runs = {}
+    for name, n in (("base", None), ("turned", tilted(0.5, 1.5)), ("tilted", tilted(0.8, 0.4))):
+        lc = base.copy()
+        if n is not None:
+            lc[:, :, Nx - 1, :] = n
+        folder = str(tmp_path / name)
+        s["Algorithm settings"]["General"]["Results folder name"] = folder
+        s["Postprocessor settings"]["Bulk output"] = {"Activate": True, "Base name": "bulk"}
+        print("[gpu case] duplicate column %s, precision %d" % (name, precision), flush=True)
+        E = np.zeros(fields.output_shape(s), np.complex128)
+        bpm_backend.run_backend_without_mask(json.dumps(s), lc, E, strict=True, precision=precision)
+        runs[name] = (vti.bulk_fields(os.path.join(folder, "bulk.vti"))[(0, 0)], lc)   # (pol, Nz, Ny, Nx, 2)
+    b0, bt, bs = runs["base"][0], runs["turned"][0], runs["tilted"][0]
+    assert np.array_equal(bt[:, 1:], b0[:, 1:])
+    assert np.array_equal(bt[:, 0, :, :Nx - 1], b0[:, 0, :, :Nx - 1])
+    assert not np.array_equal(bt[:, 0, :, Nx - 1], b0[:, 0, :, Nx - 1])
+    _, want = oracle.propagate_bulk(s, runs["tilted"][1])
+    want = np.moveaxis(want[0, 0], 2, -1)                                            # (pol, Nz, Ny, Nx, 2)
+    tol = 1e-10 if precision == 64 else 1e-4
+    assert np.linalg.norm(bs - want) / np.linalg.norm(want) < tol
+    assert np.linalg.norm(bs[:, 1:] - b0[:, 1:]) / np.linalg.norm(b0[:, 1:]) > 1e-4
