@@ -21,7 +21,8 @@
  * coefficients by second-order convergence to the continuum operator; the ADI
  * step by its exact Fourier symbol in homogeneous anisotropic media; the whole
  * march by x,y-invariant media (products of 2x2 matrices); Fresnel by the
- * textbook isotropic transmission; the shift by translation and linear
+ * textbook isotropic transmission and, for an in-plane uniaxial director, the
+ * eigen-polarisation transmissions; the shift by translation and linear
  * precision; the FFT by numpy; the screen filter by the NA low-pass, a single
  * layer at normal incidence and the angular-spectrum propagator; Q-tensor
  * input by the Q of a director reproducing director mode; the focus

@@ -568,6 +568,26 @@ def test_oracle_single_plane_is_the_two_fresnel_interfaces():
     assert np.abs(E[0, 0, 1]).max() < 1e-15 and np.abs(E[0, 0, 2]).max() < 1e-13
 
 
+def test_oracle_single_plane_anisotropic_interfaces_are_the_eigenmode_transmissions():
+    """A mesh of one plane of uniaxial LC with an in-plane director at angle
+    phi: S = sqrt(K) has the eigen-polarisations along and across the
+    director with indices ne and no, and each passes both interfaces with the
+    textbook normal-incidence transmissions, 2 nin / (nin + n_j) then
+    2 n_j / (n_j + nout) (FresnelOperator.cpp:16-38, 45-69): the output is
+    R diag(T_e, T_o) R^T applied to the two input polarisations.  A sign slip
+    on S_xy or a transposed adjugate fails it for phi off the axes."""
+    dims, sp = (6, 5, 1), (0.1, 0.1, 0.1)
+    nin, nout, no, ne, phi = 1.2, 1.33, 1.5, 1.7, 0.7
+    s = fields.make_settings(dims, sp, [0.6], no=str(no), ne=str(ne), nin=str(nin), nout=str(nout), na_objective=0.5)
+    lc = np.zeros(dims[::-1] + (3,)) + [math.cos(phi), math.sin(phi), 0.0]
+    E = oracle.propagate(s, lc)
+    R = np.array([[math.cos(phi), -math.sin(phi)], [math.sin(phi), math.cos(phi)]])
+    T = R @ np.diag([4 * nin * n / ((nin + n) * (n + nout)) for n in (ne, no)]) @ R.T
+    for pol in (0, 1):
+        for comp in (0, 1):
+            np.testing.assert_allclose(E[0, 0, 2 * pol + comp], T[comp, pol], rtol=0, atol=1e-13)
+
+
 # --------------------------------------------------------------------------- seam rule
 def _tilted(theta, phi):
     return np.array([math.sin(theta) * math.cos(phi), math.sin(theta) * math.sin(phi), math.cos(theta)])
