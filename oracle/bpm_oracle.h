@@ -20,7 +20,9 @@
  * exponential; cyclic / block line solves by dense solves; the R
  * coefficients by second-order convergence to the continuum operator; the ADI
  * step by its exact Fourier symbol in homogeneous anisotropic media; the whole
- * march by x,y-invariant media (products of 2x2 matrices); Fresnel by the
+ * march by x,y-invariant media (products of 2x2 matrices) and the uniform
+ * wave plate (retardation k0 (ne - no) L, Beer-Lambert damping of the
+ * extraordinary wave by ne_imag); Fresnel by the
  * textbook isotropic transmission and, for an in-plane uniaxial director, the
  * eigen-polarisation transmissions; the shift by translation and linear
  * precision; the FFT by numpy; the screen filter by the NA low-pass, a single

@@ -300,6 +300,27 @@ def test_xy_invariant_march_is_product_of_2x2_matrices():
         np.testing.assert_allclose(got, np.broadcast_to(want[:, None, None], got.shape), rtol=0, atol=1e-12)
 
 
+def test_uniaxial_slab_retardation_and_absorption():
+    """A uniform uniaxial slab with the optic axis along x at normal incidence
+    (the textbook wave plate): the x-polarised (extraordinary) and y-polarised
+    (ordinary) waves keep their polarisation, each passes both interfaces
+    with its own normal-incidence transmissions, the extraordinary wave gains
+    the retardation k0 (ne - no) L over the ordinary one (whose phase the march
+    takes as its reference) and is damped by exp(-k0 ne_imag L): ne_imag is
+    the imaginary part of the extraordinary index (Beer-Lambert)."""
+    Nx, Ny, Nz = 6, 5, 11
+    no, ne, ki, nin, nout = 1.5, 1.7, 0.01, 1.2, 1.3
+    s = _settings((Nx, Ny, Nz), no=str(no), ne=str(ne), ne_imag=str(ki), nin=str(nin), nout=str(nout),
+                  na_objective=0.5)
+    n = np.zeros((Nz, Ny, Nx, 3)); n[..., 0] = 1
+    E = oracle.propagate(s, n)
+    k0, L = 2 * math.pi / 0.6, 0.1 * (Nz - 1)
+    t = lambda nj: 2 * nin / (nin + nj) * 2 * nj / (nj + nout)
+    np.testing.assert_allclose(E[0, 0, 0], t(ne) * np.exp(1j * k0 * (ne - no) * L - k0 * ki * L), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(E[0, 0, 3], t(no), rtol=0, atol=1e-12)
+    assert np.abs(E[0, 0, 1]).max() < 1e-14 and np.abs(E[0, 0, 2]).max() < 1e-14
+
+
 def test_isotropic_slab_textbook_transmission():
     Nx, Ny, Nz = 6, 6, 5
     n = np.zeros((Nz, Ny, Nx, 3)); n[..., 2] = 1
